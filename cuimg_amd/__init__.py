@@ -1,0 +1,418 @@
+"""Python host mirror of the cuimg tracking hot path on MI355X.
+
+Thin ctypes front end of libcuimg_hip.so (C ABI in include/cuimg_hip.h).  It
+mirrors the reference's C++ operator surface for this path — image handles
+(host_image2d / device_image2d), the meta-template Gaussian, subsample /
+fill_border_clamp, the FAST detector and tracker<pyrlk>(domain, nscales) with
+run / scale(s).pset() accessors — so tests and benchmarks read like the
+reference's own call sites.  There is no CPU fallback: every operation runs on
+the HIP device and raises if the library or the device is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcuimg_hip.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "cuimg_hip.h")
+
+U8, F32, F32X4, U32, F32X2 = 1, 2, 3, 4, 5
+_NP = {U8: (np.uint8, ()), F32: (np.float32, ()), F32X4: (np.float32, (4,)),
+       U32: (np.uint32, ()), F32X2: (np.float32, (2,))}
+_ESIZE = {U8: 1, F32: 4, F32X4: 16, U32: 4, F32X2: 8}
+
+PARTICLE_DTYPE = np.dtype([("speed_r", "<f4"), ("speed_c", "<f4"), ("pos_r", "<f4"),
+                           ("pos_c", "<f4"), ("age", "<u2"), ("pad", "<u2")])
+
+
+class CuimgError(RuntimeError):
+    pass
+
+
+class Img(ctypes.Structure):
+    _fields_ = [("begin", ctypes.c_void_p), ("nrows", ctypes.c_int32), ("ncols", ctypes.c_int32),
+                ("pitch", ctypes.c_int32), ("border", ctypes.c_int32), ("dtype", ctypes.c_int32)]
+
+
+class TrackerParams(ctypes.Structure):
+    _fields_ = [("nscales", ctypes.c_int32), ("profile", ctypes.c_int32),
+                ("lk_ws", ctypes.c_int32), ("lk_max_iters", ctypes.c_int32),
+                ("lk_eps", ctypes.c_float), ("lk_top_n", ctypes.c_int32),
+                ("fast_n", ctypes.c_int32), ("fast_th", ctypes.c_float),
+                ("box_size", ctypes.c_int32), ("detector_frequency", ctypes.c_int32),
+                ("filtering_frequency", ctypes.c_int32), ("k", ctypes.c_int32),
+                ("with_merge", ctypes.c_int32), ("blur_sigma", ctypes.c_int32),
+                ("blur_K", ctypes.c_int32)]
+
+
+def profile_params(profile, nscales, fast_th=20.0, **kw):
+    """'R': the reference matcher defaults on u8 frames (11x11 window, <=30
+    iterations, 25 strongest-gradient pixels); 'B': 7x7 window, <=5 iterations,
+    all pixels, on sigma=1 / 7-tap blurred FP32 frames."""
+    p = TrackerParams()
+    p.nscales = nscales
+    if profile == "R":
+        p.profile, p.lk_ws, p.lk_max_iters, p.lk_top_n = 0, 11, 30, 25
+    elif profile == "B":
+        p.profile, p.lk_ws, p.lk_max_iters, p.lk_top_n = 1, 7, 5, 49
+    else:
+        raise ValueError(profile)
+    p.lk_eps, p.fast_n, p.fast_th, p.box_size = 0.1, 9, fast_th, 3
+    p.detector_frequency, p.filtering_frequency, p.k, p.with_merge = 1, 1, 300, 1
+    p.blur_sigma, p.blur_K = 1, 3
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+_lib = None
+
+
+def lib():
+    """Loads libcuimg_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CuimgError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, c_int, P = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER
+        L.cuimg_hip_last_error.restype = ctypes.c_char_p
+        L.cuimg_hip_last_error.argtypes = [vp]
+        L.cuimg_hip_stream.restype = vp
+        L.cuimg_hip_ctx_create.argtypes = [c_int, P(vp)]
+        for name in ("cuimg_hip_ctx_destroy", "cuimg_hip_sync", "cuimg_hip_profile_reset"):
+            getattr(L, name).argtypes = [vp]
+        L.cuimg_hip_profile_enable.argtypes = [vp, c_int]
+        L.cuimg_hip_profile_query.argtypes = [vp, ctypes.c_char_p, P(ctypes.c_int64), P(ctypes.c_double)]
+        L.cuimg_hip_malloc.argtypes = [vp, ctypes.c_size_t, P(vp)]
+        L.cuimg_hip_free.argtypes = [vp, vp]
+        L.cuimg_hip_memcpy_h2d.argtypes = [vp, vp, vp, ctypes.c_size_t]
+        L.cuimg_hip_memcpy_d2h.argtypes = [vp, vp, vp, ctypes.c_size_t]
+        L.cuimg_hip_memset.argtypes = [vp, vp, c_int, ctypes.c_size_t]
+        L.cuimg_hip_image_alloc.argtypes = [vp, c_int, c_int, c_int, c_int, P(Img)]
+        L.cuimg_hip_image_free.argtypes = [vp, P(Img)]
+        L.cuimg_hip_image_upload.argtypes = [vp, P(Img), vp, ctypes.c_size_t]
+        L.cuimg_hip_image_download.argtypes = [vp, P(Img), vp, ctypes.c_size_t]
+        L.cuimg_hip_fill_border_clamp.argtypes = [vp, P(Img)]
+        L.cuimg_hip_gaussian_coefs.argtypes = [c_int, c_int, P(ctypes.c_float)]
+        L.cuimg_hip_gaussian_sep.argtypes = [vp, P(Img), P(Img), c_int, c_int]
+        L.cuimg_hip_convolve_sep.argtypes = [vp, P(Img), P(Img), P(ctypes.c_float), c_int]
+        L.cuimg_hip_convolve_sep_th.argtypes = [vp, P(Img), P(Img), P(ctypes.c_float), c_int, c_int]
+        L.cuimg_hip_gaussian_sep_multi.argtypes = [vp, P(Img), P(Img), c_int, c_int, c_int]
+        L.cuimg_hip_pyr_down.argtypes = [vp, P(Img), P(Img)]
+        L.cuimg_hip_pyramid_build.argtypes = [vp, P(Img), P(Img), c_int]
+        L.cuimg_hip_fast_saliency.argtypes = [vp, P(Img), P(Img), P(Img), c_int, c_int]
+        L.cuimg_synth_video.argtypes = [ctypes.c_uint64, c_int, c_int, c_int, c_int, vp]
+        L.cuimg_hip_tracker_create.argtypes = [vp, c_int, c_int, P(TrackerParams), P(vp)]
+        L.cuimg_hip_tracker_destroy.argtypes = [vp]
+        L.cuimg_hip_tracker_run.argtypes = [vp, P(Img)]
+        L.cuimg_hip_tracker_run_host.argtypes = [vp, vp, ctypes.c_size_t]
+        L.cuimg_hip_tracker_clear.argtypes = [vp]
+        L.cuimg_hip_tracker_nparticles.argtypes = [vp, c_int, P(ctypes.c_uint32)]
+        L.cuimg_hip_tracker_get_particles.argtypes = [vp, c_int, vp, ctypes.c_uint32, P(ctypes.c_uint32)]
+        L.cuimg_hip_tracker_set_particles.argtypes = [vp, c_int, vp, ctypes.c_uint32]
+        L.cuimg_hip_tracker_get_matches.argtypes = [vp, c_int, vp, ctypes.c_uint32, P(ctypes.c_uint32)]
+        L.cuimg_hip_tracker_compact_has_run.argtypes = [vp, c_int, P(c_int)]
+        L.cuimg_hip_tracker_sparse.argtypes = [vp, c_int, P(Img)]
+        L.cuimg_hip_tracker_level.argtypes = [vp, c_int, P(Img)]
+        L.cuimg_hip_tracker_saliency.argtypes = [vp, c_int, P(Img)]
+        L.cuimg_hip_tracker_get_decisions.argtypes = [vp, c_int, vp, vp, vp, ctypes.c_uint32, P(ctypes.c_uint32)]
+        L.cuimg_hip_tracker_matcher_images.argtypes = [vp, c_int, P(Img), P(Img), P(Img)]
+        L.cuimg_hip_tracker_frame_count.argtypes = [vp, P(c_int)]
+        _lib = L
+    return _lib
+
+
+def synth_video(seed, nrows, ncols, nframes, motion):
+    """Seeded synthetic texture video (host code of the library)."""
+    out = np.zeros((nframes, nrows, ncols), np.uint8)
+    r = lib().cuimg_synth_video(seed, nrows, ncols, nframes, motion, out.ctypes.data)
+    if r:
+        raise CuimgError(f"cuimg_synth_video failed ({r})")
+    return out
+
+
+def gaussian_coefs(sigma, K):
+    out = np.zeros(2 * K + 1, np.float32)
+    r = lib().cuimg_hip_gaussian_coefs(sigma, K, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    if r:
+        raise CuimgError("gaussian_coefs failed")
+    return out
+
+
+class Context:
+    """One HIP device + stream (cuimg_hip_ctx)."""
+
+    def __init__(self, device=0):
+        L = lib()
+        self.L = L
+        h = ctypes.c_void_p()
+        r = L.cuimg_hip_ctx_create(device, ctypes.byref(h))
+        if r:
+            raise CuimgError(f"cuimg_hip_ctx_create({device}) failed with {r} (no HIP device?)")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cuimg_hip_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, r, what):
+        if r:
+            msg = self.L.cuimg_hip_last_error(self.h)
+            raise CuimgError(f"{what}: error {r}: {msg.decode() if msg else ''}")
+
+    def sync(self):
+        self.check(self.L.cuimg_hip_sync(self.h), "sync")
+
+    # profiling
+    def profile(self, on=True):
+        self.check(self.L.cuimg_hip_profile_enable(self.h, 1 if on else 0), "profile_enable")
+
+    def profile_reset(self):
+        self.check(self.L.cuimg_hip_profile_reset(self.h), "profile_reset")
+
+    def profile_query(self, name):
+        n, ms = ctypes.c_int64(), ctypes.c_double()
+        self.check(self.L.cuimg_hip_profile_query(self.h, name.encode(), ctypes.byref(n), ctypes.byref(ms)),
+                   "profile_query")
+        return n.value, ms.value
+
+    # images
+    def image(self, nrows, ncols, dtype, border=0):
+        return DeviceImage(self, nrows, ncols, dtype, border)
+
+    def upload(self, arr, border=0):
+        arr = np.ascontiguousarray(arr)
+        if arr.dtype == np.uint8:
+            dt = U8
+        elif arr.dtype == np.float32 and arr.ndim == 2:
+            dt = F32
+        elif arr.dtype == np.float32 and arr.ndim == 3 and arr.shape[2] == 4:
+            dt = F32X4
+        else:
+            raise ValueError(f"unsupported array {arr.dtype} {arr.shape}")
+        im = self.image(arr.shape[0], arr.shape[1], dt, border)
+        im.upload(arr)
+        return im
+
+    def gaussian(self, src, dst, sigma, K):
+        self.check(self.L.cuimg_hip_gaussian_sep(self.h, ctypes.byref(src.d), ctypes.byref(dst.d), sigma, K),
+                   "gaussian_sep")
+
+    def convolve(self, src, dst, coefs, K, th=0):
+        c = np.ascontiguousarray(coefs, np.float32)
+        self.check(self.L.cuimg_hip_convolve_sep_th(self.h, ctypes.byref(src.d), ctypes.byref(dst.d),
+                                                    c.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), K, th),
+                   "convolve_sep")
+
+    def gaussian_multi(self, srcs, dsts, sigma, K):
+        a = (Img * len(srcs))(*[s.d for s in srcs])
+        b = (Img * len(dsts))(*[d.d for d in dsts])
+        self.check(self.L.cuimg_hip_gaussian_sep_multi(self.h, a, b, len(srcs), sigma, K), "gaussian_sep_multi")
+
+    def pyr_down(self, src, dst):
+        self.check(self.L.cuimg_hip_pyr_down(self.h, ctypes.byref(src.d), ctypes.byref(dst.d)), "pyr_down")
+
+    def pyramid(self, frame, levels):
+        a = (Img * len(levels))(*[l.d for l in levels])
+        self.check(self.L.cuimg_hip_pyramid_build(self.h, ctypes.byref(frame.d), a, len(levels)),
+                   "pyramid_build")
+
+    def fill_border(self, img):
+        self.check(self.L.cuimg_hip_fill_border_clamp(self.h, ctypes.byref(img.d)), "fill_border_clamp")
+
+    def fast(self, src, mask, dst, n=9, th=20):
+        m = ctypes.byref(mask.d) if mask is not None else None
+        self.check(self.L.cuimg_hip_fast_saliency(self.h, ctypes.byref(src.d), m, ctypes.byref(dst.d), n, th),
+                   "fast_saliency")
+
+
+class DeviceImage:
+    """device_image2d<V>(domain, border): owned device image."""
+
+    def __init__(self, ctx, nrows, ncols, dtype, border=0):
+        self.ctx = ctx
+        self.d = Img()
+        ctx.check(ctx.L.cuimg_hip_image_alloc(ctx.h, nrows, ncols, border, dtype, ctypes.byref(self.d)),
+                  "image_alloc")
+
+    @property
+    def shape(self):
+        return (self.d.nrows, self.d.ncols)
+
+    def free(self):
+        if self.d.begin and self.ctx.h:
+            self.ctx.L.cuimg_hip_image_free(self.ctx.h, ctypes.byref(self.d))
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def upload(self, arr):
+        np_t, tail = _NP[self.d.dtype]
+        arr = np.ascontiguousarray(arr, np_t)
+        assert arr.shape == (self.d.nrows, self.d.ncols) + tail, (arr.shape, self.shape)
+        pitch = self.d.ncols * _ESIZE[self.d.dtype]
+        self.ctx.check(self.ctx.L.cuimg_hip_image_upload(self.ctx.h, ctypes.byref(self.d), arr.ctypes.data, pitch),
+                       "image_upload")
+
+    def download(self):
+        np_t, tail = _NP[self.d.dtype]
+        out = np.zeros((self.d.nrows, self.d.ncols) + tail, np_t)
+        pitch = self.d.ncols * _ESIZE[self.d.dtype]
+        self.ctx.check(self.ctx.L.cuimg_hip_image_download(self.ctx.h, ctypes.byref(self.d), out.ctypes.data, pitch),
+                       "image_download")
+        return out
+
+    def download_with_border(self):
+        """Domain plus border as one array (for border checks)."""
+        b = self.d.border
+        np_t, tail = _NP[self.d.dtype]
+        es = _ESIZE[self.d.dtype]
+        nr, nc = self.d.nrows + 2 * b, self.d.ncols + 2 * b
+        out = np.zeros((nr, nc) + tail, np_t)
+        src = Img(self.d.begin - b * self.d.pitch - b * es, nr, nc, self.d.pitch, 0, self.d.dtype)
+        self.ctx.check(self.ctx.L.cuimg_hip_image_download(self.ctx.h, ctypes.byref(src), out.ctypes.data, nc * es),
+                       "image_download")
+        return out
+
+
+class View:
+    """Non-owning device image view (kernel_image2d) returned by the tracker."""
+
+    def __init__(self, ctx, d):
+        self.ctx = ctx
+        self.d = d
+
+    @property
+    def shape(self):
+        return (self.d.nrows, self.d.ncols)
+
+    def download(self):
+        np_t, tail = _NP[self.d.dtype]
+        out = np.zeros((self.d.nrows, self.d.ncols) + tail, np_t)
+        pitch = self.d.ncols * _ESIZE[self.d.dtype]
+        self.ctx.check(self.ctx.L.cuimg_hip_image_download(self.ctx.h, ctypes.byref(self.d), out.ctypes.data, pitch),
+                       "image_download")
+        return out
+
+
+class Tracker:
+    """tracker<pyrlk>(domain, nscales) on the GPU (cuimg_hip_tracker).
+
+    run(frame) is tracker::run; particles(s) / matches(s) / sparse(s) are
+    scale(s).pset().dense_particles() / matches() / sparse_particles()."""
+
+    def __init__(self, ctx, nrows, ncols, params):
+        self.ctx = ctx
+        self.L = ctx.L
+        self.params = params
+        h = ctypes.c_void_p()
+        ctx.check(self.L.cuimg_hip_tracker_create(ctx.h, nrows, ncols, ctypes.byref(params), ctypes.byref(h)),
+                  "tracker_create")
+        self.h = h
+        self.nscales = params.nscales
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cuimg_hip_tracker_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, frame):
+        if isinstance(frame, (DeviceImage, View)):
+            self.ctx.check(self.L.cuimg_hip_tracker_run(self.h, ctypes.byref(frame.d)), "tracker_run")
+        else:
+            frame = np.ascontiguousarray(frame, np.uint8)
+            self.ctx.check(self.L.cuimg_hip_tracker_run_host(self.h, frame.ctypes.data, frame.shape[1]),
+                           "tracker_run_host")
+
+    def clear(self):
+        self.ctx.check(self.L.cuimg_hip_tracker_clear(self.h), "tracker_clear")
+
+    def nparticles(self, s):
+        n = ctypes.c_uint32()
+        self.ctx.check(self.L.cuimg_hip_tracker_nparticles(self.h, s, ctypes.byref(n)), "nparticles")
+        return n.value
+
+    def particles(self, s):
+        n = ctypes.c_uint32()
+        self.ctx.check(self.L.cuimg_hip_tracker_get_particles(self.h, s, None, 0, ctypes.byref(n)), "get_particles")
+        out = np.zeros(n.value, PARTICLE_DTYPE)
+        if n.value:
+            self.ctx.check(self.L.cuimg_hip_tracker_get_particles(self.h, s, out.ctypes.data, n.value,
+                                                                  ctypes.byref(n)), "get_particles")
+        return out
+
+    def set_particles(self, s, parts):
+        parts = np.ascontiguousarray(parts, PARTICLE_DTYPE)
+        self.ctx.check(self.L.cuimg_hip_tracker_set_particles(self.h, s, parts.ctypes.data, len(parts)),
+                       "set_particles")
+
+    def matches(self, s):
+        n = ctypes.c_uint32()
+        self.ctx.check(self.L.cuimg_hip_tracker_get_matches(self.h, s, None, 0, ctypes.byref(n)), "get_matches")
+        out = np.zeros(n.value, np.uint32)
+        if n.value:
+            self.ctx.check(self.L.cuimg_hip_tracker_get_matches(self.h, s, out.ctypes.data, n.value,
+                                                                ctypes.byref(n)), "get_matches")
+        return out
+
+    def compact_has_run(self, s):
+        v = ctypes.c_int()
+        self.ctx.check(self.L.cuimg_hip_tracker_compact_has_run(self.h, s, ctypes.byref(v)), "compact_has_run")
+        return bool(v.value)
+
+    def _view(self, fn, s):
+        d = Img()
+        self.ctx.check(fn(self.h, s, ctypes.byref(d)), "tracker view")
+        return View(self.ctx, d)
+
+    def sparse(self, s):
+        return self._view(self.L.cuimg_hip_tracker_sparse, s).download().view(np.int32)
+
+    def level(self, s):
+        return self._view(self.L.cuimg_hip_tracker_level, s).download()
+
+    def saliency(self, s):
+        return self._view(self.L.cuimg_hip_tracker_saliency, s).download()
+
+    def matcher_images(self, s):
+        a, b, g = Img(), Img(), Img()
+        self.ctx.check(self.L.cuimg_hip_tracker_matcher_images(self.h, s, ctypes.byref(a), ctypes.byref(b),
+                                                               ctypes.byref(g)), "matcher_images")
+        return View(self.ctx, a).download(), View(self.ctx, b).download()
+
+    def decisions(self, s):
+        n = ctypes.c_uint32()
+        self.ctx.check(self.L.cuimg_hip_tracker_get_decisions(self.h, s, None, None, None, 0, ctypes.byref(n)),
+                       "get_decisions")
+        acc = np.zeros(n.value, np.int32)
+        tr = np.zeros((n.value, 2), np.float32)
+        dist = np.zeros(n.value, np.float32)
+        if n.value:
+            self.ctx.check(self.L.cuimg_hip_tracker_get_decisions(self.h, s, acc.ctypes.data, tr.ctypes.data,
+                                                                  dist.ctypes.data, n.value, ctypes.byref(n)),
+                           "get_decisions")
+        return acc, tr, dist
+
+    @property
+    def frame_count(self):
+        v = ctypes.c_int()
+        self.ctx.check(self.L.cuimg_hip_tracker_frame_count(self.h, ctypes.byref(v)), "frame_count")
+        return v.value

@@ -1,0 +1,101 @@
+// Shared host/device helpers of the cuimg HIP library (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/cuimg_hip.h"
+
+struct cuimg_hip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  // allocation bookkeeping: image begin -> hipMalloc base
+  std::map<void*, void*> image_base;
+  // per-kernel event timing
+  bool profiling = false;
+  struct timed_launch {
+    const char* name;
+    hipEvent_t a, b;
+  };
+  std::vector<timed_launch> pending;
+  std::vector<hipEvent_t> free_events;
+  std::map<std::string, std::pair<int64_t, double>> stats;
+};
+
+namespace cuimg {
+
+int set_error(cuimg_hip_ctx* ctx, int code, const char* fmt, ...);
+int hip_error(cuimg_hip_ctx* ctx, hipError_t e, const char* what, const char* file, int line);
+
+#define CUIMG_HIP(ctx, expr)                                                   \
+  do {                                                                         \
+    hipError_t e__ = (expr);                                                   \
+    if (e__ != hipSuccess) return ::cuimg::hip_error(ctx, e__, #expr, __FILE__, __LINE__); \
+  } while (0)
+
+#define CUIMG_TRY(expr)          \
+  do {                           \
+    int r__ = (expr);            \
+    if (r__ != 0) return r__;    \
+  } while (0)
+
+inline int dtype_size(int dtype) {
+  switch (dtype) {
+    case CUIMG_U8: return 1;
+    case CUIMG_F32: return 4;
+    case CUIMG_F32X4: return 16;
+    case CUIMG_U32: return 4;
+    case CUIMG_F32X2: return 8;
+    default: return 0;
+  }
+}
+
+// Launch bracket: records events around the launch when profiling is on and
+// turns launch errors into error codes.
+struct launch_scope {
+  cuimg_hip_ctx* ctx;
+  const char* name;
+  hipEvent_t a = nullptr, b = nullptr;
+  launch_scope(cuimg_hip_ctx* c, const char* n);
+  int done();
+};
+
+template <typename F>
+inline int launch(cuimg_hip_ctx* ctx, const char* name, F&& f) {
+  launch_scope s(ctx, name);
+  f();
+  return s.done();
+}
+
+inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+// Entry points shared between translation units.
+int image_alloc(cuimg_hip_ctx* ctx, int nrows, int ncols, int border, int dtype, cuimg_img* out);
+int image_free(cuimg_hip_ctx* ctx, cuimg_img* img);
+int fill_border(cuimg_hip_ctx* ctx, const cuimg_img* img);
+int convolve_sep_multi(cuimg_hip_ctx* ctx, const cuimg_img* ins, const cuimg_img* outs, int n,
+                       const float* coefs, int K, int th_override);
+int gaussian_coefs(int sigma, int K, float* out);
+int pyramid_build(cuimg_hip_ctx* ctx, const cuimg_img* frame, const cuimg_img* levels, int n);
+int fast_saliency(cuimg_hip_ctx* ctx, const cuimg_img* in, const cuimg_img* mask, const cuimg_img* out,
+                  int n, int th);
+
+// ------------------------------------------------------------ device helpers
+template <typename T>
+__device__ __forceinline__ T* px(void* begin, int pitch, int r, int c) {
+  return reinterpret_cast<T*>(static_cast<char*>(begin) + (ptrdiff_t)r * pitch) + c;
+}
+template <typename T>
+__device__ __forceinline__ const T* px(const void* begin, int pitch, int r, int c) {
+  return reinterpret_cast<const T*>(static_cast<const char*>(begin) + (ptrdiff_t)r * pitch) + c;
+}
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+}  // namespace cuimg

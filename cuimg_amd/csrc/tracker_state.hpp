@@ -1,0 +1,99 @@
+// Device-side state of the pyramid LK tracker, shared by particles.hip,
+// lk.hip and tracker.hip.
+#pragma once
+
+#include "common.hpp"
+
+namespace cuimg {
+
+constexpr int kMaxLevels = 8;
+
+// Per-scale counters in device memory: the particle count lives on the device so
+// a frame step needs no host round trip.
+struct level_counters {
+  uint32_t n;        // dense_particles().size()
+  uint32_t n_old;    // size before the last compaction (= matches().size() - n_new)
+  uint32_t n_live;   // survivors of the last compaction
+  uint32_t n_new;    // particles added by the last detection
+  uint32_t overflow; // capacity exceeded (particles dropped)
+  uint32_t actors;   // merge: particles with an eligible neighbour
+  uint32_t n_matched; // size of the dense vector seen by the last matcher launch
+  uint32_t pad;
+};
+
+// One pyramid level as the matcher sees it: A = previous frame, B = current.
+struct lk_level_view {
+  const void* A;
+  const void* B;
+  int pitchA, pitchB;
+  int nr, nc;
+};
+
+struct lk_params_dev {
+  int ws, max_iters, top_n;
+  float eps, min_eig;
+  int k;
+  int template_u8;
+};
+
+struct lk_args {
+  lk_level_view lv[kMaxLevels];
+  int nlevels;
+  int s;                        // scale of the particles
+  cuimg_particle_f* parts;
+  level_counters* cnt;
+  uint32_t* sparse;             // pixel (0,0) of the scale's sparse image
+  int sparse_pitch;
+  int32_t* dec_accept;
+  float2* dec_tr;
+  float* dec_dist;
+  int pd_r1, pd_c1, pd_r2, pd_c2;  // particle domain (inclusive box)
+  lk_params_dev P;
+};
+
+// Host launchers implemented in lk.hip / particles.hip.
+int launch_lk(cuimg_hip_ctx* ctx, const lk_args& a, int image_u8);
+
+struct detect_args {
+  // level geometry
+  int nr, nc;
+  int box;                      // detector box size
+  int nbr, nbc;                 // boxes per column / row
+  float fast_th;
+  // images
+  const uint8_t* sal; int sal_pitch;
+  uint8_t* mask; int mask_pitch;
+  uint32_t* sparse; int sparse_pitch;
+  // particles
+  cuimg_particle_f* parts;      // current dense vector
+  cuimg_particle_f* parts_out;  // compaction target
+  uint32_t* matches;
+  uint8_t* flags;               // per particle: bit0 alive, bit1 owns its sparse pixel
+  uint32_t* blockcnt;           // per 1024-particle block
+  uint8_t* boxcode;             // per detector box: 0xFF none, else dr*box+dc
+  uint32_t* subcnt;             // per (box row, dr) sub-row
+  level_counters* cnt;
+  uint32_t cap;
+  int nblocks;                  // compaction blocks covering cap
+};
+
+int launch_fill_u32(cuimg_hip_ctx* ctx, uint32_t* begin, int pitch, int r0, int r1, int c0, int c1,
+                    uint32_t v);
+int launch_fill_u8(cuimg_hip_ctx* ctx, uint8_t* begin, int pitch, int nr, int nc, uint8_t v);
+int launch_particle_mask(cuimg_hip_ctx* ctx, const detect_args& a);
+int launch_detect_and_compact(cuimg_hip_ctx* ctx, const detect_args& a);
+
+struct merge_args {
+  int nr, nc;
+  cuimg_particle_f* parts;
+  const uint32_t* sparse; int sparse_pitch;
+  level_counters* cnt;
+  uint32_t* actors;     // cap
+  int32_t* zeroer;      // cap
+  int32_t* minunres;    // cap
+  uint8_t* resolved;    // cap
+  uint32_t cap;
+};
+int launch_merge(cuimg_hip_ctx* ctx, const merge_args& a);
+
+}  // namespace cuimg

@@ -1,0 +1,209 @@
+/* cuimg_hip — MI355X (gfx950) implementation of cuimg's frame-to-frame tracking
+ * hot path: separable Gaussian, pyramid, FAST detector, pyramid Lucas-Kanade
+ * particle tracker and its particle container, behind a C ABI.
+ *
+ * Conventions (same as the reference):
+ *   - 2-vectors are (row, col); images are row-major with a pitch in BYTES and a
+ *     border of `border` pixels around the domain; `begin` points at pixel (0,0)
+ *     (cuimg/cpu/host_image2d.hpp:78-112, cuimg/gpu/kernel_image2d.h:17-85);
+ *   - every function returns 0 on success, a negative cuimg error code, or a
+ *     positive hipError_t; cuimg_hip_last_error() gives the message.  No C++
+ *     exception crosses this boundary (the reference throws std::runtime_error
+ *     from check_cuda_error, cuimg/error.h:17-28; the C++ wrapper rethrows);
+ *   - device buffers passed in are plain device pointers; the library never
+ *     takes ownership of caller memory;
+ *   - a context owns one HIP stream; all work is enqueued on it in call order.
+ *     A context (and the trackers made from it) is used by one host thread at a
+ *     time, like the reference tracker.
+ */
+#ifndef CUIMG_HIP_H_
+#define CUIMG_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUIMG_OK 0
+#define CUIMG_E_INVALID (-1)     /* bad argument (shape, dtype, null pointer) */
+#define CUIMG_E_NOMEM (-2)       /* device allocation failed */
+#define CUIMG_E_UNSUPPORTED (-3) /* parameter combination not implemented */
+#define CUIMG_E_CAPACITY (-4)    /* particle capacity exceeded */
+#define CUIMG_E_NODEVICE (-5)    /* no HIP device */
+
+enum cuimg_dtype {
+  CUIMG_U8 = 1,    /* gl8u */
+  CUIMG_F32 = 2,   /* gl01f / i_float1 */
+  CUIMG_F32X4 = 3, /* i_float4 */
+  CUIMG_U32 = 4,   /* unsigned int (sparse particle index) */
+  CUIMG_F32X2 = 5, /* i_float2 (gradient) */
+};
+
+/* Non-owning image view (the kernel_image2d of the reference). */
+typedef struct {
+  void* begin;      /* device address of pixel (0,0) */
+  int32_t nrows;
+  int32_t ncols;
+  int32_t pitch;    /* bytes between rows */
+  int32_t border;   /* pixels of allocated border on every side */
+  int32_t dtype;    /* enum cuimg_dtype */
+} cuimg_img;
+
+/* particle_f<translation_transform> (tracking2/particle_container.h:45-57):
+ * 20 bytes, (row, col) vectors. */
+typedef struct {
+  float speed_r, speed_c;
+  float pos_r, pos_c;
+  uint16_t age;
+  uint16_t pad;
+} cuimg_particle_f;
+
+typedef struct cuimg_hip_ctx cuimg_hip_ctx;
+
+/* ---------------------------------------------------------------- context */
+int cuimg_hip_ctx_create(int device, cuimg_hip_ctx** out);
+int cuimg_hip_ctx_destroy(cuimg_hip_ctx* ctx);
+const char* cuimg_hip_last_error(const cuimg_hip_ctx* ctx);
+int cuimg_hip_sync(cuimg_hip_ctx* ctx);
+/* The context's hipStream_t. */
+void* cuimg_hip_stream(cuimg_hip_ctx* ctx);
+/* Per-kernel timing with HIP events recorded on the context stream. */
+int cuimg_hip_profile_enable(cuimg_hip_ctx* ctx, int on);
+int cuimg_hip_profile_reset(cuimg_hip_ctx* ctx);
+/* Aggregated (launch count, total milliseconds) of the kernel named `kernel`;
+ * synchronises the context. */
+int cuimg_hip_profile_query(cuimg_hip_ctx* ctx, const char* kernel, int64_t* launches,
+                            double* total_ms);
+
+/* --------------------------------------------------------- memory / images */
+int cuimg_hip_malloc(cuimg_hip_ctx* ctx, size_t bytes, void** out);
+int cuimg_hip_free(cuimg_hip_ctx* ctx, void* p);
+int cuimg_hip_memcpy_h2d(cuimg_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
+int cuimg_hip_memcpy_d2h(cuimg_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
+int cuimg_hip_memset(cuimg_hip_ctx* ctx, void* dst, int value, size_t bytes);
+
+/* Replaces device_image2d<V>(domain, border) (cuimg/gpu/device_image2d.hpp:23-38):
+ * rows are 128-byte aligned at pixel (0,0) and the pitch is a multiple of 128. */
+int cuimg_hip_image_alloc(cuimg_hip_ctx* ctx, int nrows, int ncols, int border, int dtype,
+                          cuimg_img* out);
+int cuimg_hip_image_free(cuimg_hip_ctx* ctx, cuimg_img* img);
+/* copy(host, device) / copy(device, host) of the domain (cuimg/copy.h:31-71). */
+int cuimg_hip_image_upload(cuimg_hip_ctx* ctx, const cuimg_img* dst, const void* host,
+                           size_t host_pitch);
+int cuimg_hip_image_download(cuimg_hip_ctx* ctx, const cuimg_img* src, void* host,
+                             size_t host_pitch);
+/* fill_border_clamp (cuimg/border.h:128-137). */
+int cuimg_hip_fill_border_clamp(cuimg_hip_ctx* ctx, const cuimg_img* img);
+
+/* --------------------------------------------------------------- Gaussian */
+/* meta_kernel<meta_gaussian<0, sigma>, x>::coef() for x = -half_width..half_width
+ * (generator tools/make_meta_gaussian/make_meta_gaussian.cpp:63-105). Host only. */
+int cuimg_hip_gaussian_coefs(int sigma, int half_width, float* out);
+/* local_jet_static<.., 0, 0, sigma, half_width>(in, out, tmp)
+ * (cuimg/gpu/local_jet_static.h:13-28) = meta_convolve_row2d then col2d
+ * (cuimg/gpu/meta_convolve.h:131-161, 288-314): clamp-to-edge, taps
+ * x = -K..K.  One fused pass, no temporary image.  in: U8, F32 or F32X4;
+ * out: F32 (F32X4 for F32X4 input); same domain.  half_width <= 12. */
+int cuimg_hip_gaussian_sep(cuimg_hip_ctx* ctx, const cuimg_img* in, const cuimg_img* out,
+                           int sigma, int half_width);
+/* Same with explicit coefficients (2*half_width+1 floats). */
+int cuimg_hip_convolve_sep(cuimg_hip_ctx* ctx, const cuimg_img* in, const cuimg_img* out,
+                           const float* coefs, int half_width);
+/* Several independent blurs in one launch (e.g. all pyramid levels). */
+int cuimg_hip_gaussian_sep_multi(cuimg_hip_ctx* ctx, const cuimg_img* ins, const cuimg_img* outs,
+                                 int n, int sigma, int half_width);
+
+/* ---------------------------------------------------------------- pyramid */
+/* subsample (cuimg/gpu/mipmap.h:206-227, kernel :25-50): 2x2 box mean, integer
+ * division for U8; out domain = in / 2 (floor).  U8 or F32. */
+int cuimg_hip_pyr_down(cuimg_hip_ctx* ctx, const cuimg_img* in, const cuimg_img* out);
+/* tracker::update_input (tracking2/tracker.hpp:53-69) fused: copies `frame`
+ * into levels[0], builds levels[1..n-1] with pyr_down and fills every level's
+ * border by clamping.  U8 (frame may be levels[0] itself) or F32 (no border
+ * fill needed: pass border 0). */
+int cuimg_hip_pyramid_build(cuimg_hip_ctx* ctx, const cuimg_img* frame, const cuimg_img* levels,
+                            int nlevels);
+
+/* ------------------------------------------------------------------- FAST */
+/* compute_saliency_kernel + fast_detector::update
+ * (tracking2/fast_detector.hpp:26-157, 267-313): out <- 0, then the FAST score
+ * (modulo 256) wherever mask != 0 (mask may be NULL = everywhere).  in: U8 with
+ * border >= 3 filled by clamping; mask, out: U8 of the same domain. */
+int cuimg_hip_fast_saliency(cuimg_hip_ctx* ctx, const cuimg_img* in, const cuimg_img* mask,
+                            const cuimg_img* out, int n, int th);
+
+/* ---------------------------------------------------------------- tracker */
+typedef struct {
+  int32_t nscales;
+  int32_t profile;              /* 0: reference matcher on u8 frames (ws 11, 30 it,
+                                   top-25); 1: blurred FP32 frames */
+  int32_t lk_ws;                /* window side (odd) */
+  int32_t lk_max_iters;
+  float lk_eps;                 /* stop when |delta| < eps */
+  int32_t lk_top_n;             /* strongest-gradient pixels; >= ws*ws = all */
+  int32_t fast_n;               /* fast_detector::set_n */
+  float fast_th;                /* fast_detector::set_fast_threshold */
+  int32_t box_size;             /* fast_detector::set_box_size */
+  int32_t detector_frequency;   /* pyrlk_cpu::set_detector_frequency */
+  int32_t filtering_frequency;  /* pyrlk_cpu::set_filtering_frequency */
+  int32_t k;                    /* pyrlk_cpu::set_k */
+  int32_t with_merge;           /* pyrlk_cpu::set_with_merge */
+  int32_t blur_sigma;           /* profile 1 blur */
+  int32_t blur_K;
+} cuimg_tracker_params;
+
+typedef struct cuimg_hip_tracker cuimg_hip_tracker;
+
+/* tracker<pyrlk_cpu<fast_detector<cpu>, iterative_lk_match_5x5>>(domain, nscales)
+ * (tracking2/tracker.hpp:12-26, pyrlk_tracker.hpp:771-793). */
+int cuimg_hip_tracker_create(cuimg_hip_ctx* ctx, int nrows, int ncols,
+                             const cuimg_tracker_params* params, cuimg_hip_tracker** out);
+int cuimg_hip_tracker_destroy(cuimg_hip_tracker* t);
+/* tracker::run(frame) (tracking2/tracker.hpp:73-80): one frame step over all
+ * scales.  `frame` is a U8 device image of the tracker domain (any border).
+ * Asynchronous on the context stream. */
+int cuimg_hip_tracker_run(cuimg_hip_tracker* t, const cuimg_img* frame);
+/* Same from a host buffer (includes the host-to-device copy). */
+int cuimg_hip_tracker_run_host(cuimg_hip_tracker* t, const uint8_t* frame, size_t pitch);
+/* tracker::clear (tracking2/tracker.hpp:95-102). */
+int cuimg_hip_tracker_clear(cuimg_hip_tracker* t);
+/* Number of entries of scale(s).pset().dense_particles(); synchronises. */
+int cuimg_hip_tracker_nparticles(cuimg_hip_tracker* t, int scale, uint32_t* n);
+/* Copies dense_particles() of scale s to host (cap entries); synchronises. */
+int cuimg_hip_tracker_get_particles(cuimg_hip_tracker* t, int scale, cuimg_particle_f* out,
+                                    uint32_t cap, uint32_t* n);
+/* Replaces dense_particles() of scale s (for replay and tests). */
+int cuimg_hip_tracker_set_particles(cuimg_hip_tracker* t, int scale, const cuimg_particle_f* in,
+                                    uint32_t n);
+/* matches() (old index -> new index, 0xFFFFFFFF = died) of the last compaction. */
+int cuimg_hip_tracker_get_matches(cuimg_hip_tracker* t, int scale, uint32_t* out, uint32_t cap,
+                                  uint32_t* n);
+int cuimg_hip_tracker_compact_has_run(cuimg_hip_tracker* t, int scale, int* out);
+/* sparse_particles(): U32 image of the scale domain, 0xFFFFFFFF = empty. */
+int cuimg_hip_tracker_sparse(cuimg_hip_tracker* t, int scale, cuimg_img* view);
+/* input() of scale s: U8 level with border 3. */
+int cuimg_hip_tracker_level(cuimg_hip_tracker* t, int scale, cuimg_img* view);
+/* detector().saliency() of scale s. */
+int cuimg_hip_tracker_saliency(cuimg_hip_tracker* t, int scale, cuimg_img* view);
+/* Per-particle matcher decisions of the last frame at scale s (before the
+ * container update): accept flag, translation (row, col), distance. */
+int cuimg_hip_tracker_get_decisions(cuimg_hip_tracker* t, int scale, int32_t* accept, float* tr2,
+                                    float* dist, uint32_t cap, uint32_t* n);
+/* Matcher images of scale s: previous / current frame as F32 and the F32X2
+ * gradient of the previous frame. */
+int cuimg_hip_tracker_matcher_images(cuimg_hip_tracker* t, int scale, cuimg_img* A, cuimg_img* B,
+                                     cuimg_img* G);
+int cuimg_hip_tracker_frame_count(cuimg_hip_tracker* t, int* out);
+
+/* -------------------------------------------------------- synthetic input */
+/* Seeded texture video used by the benchmarks (host code, OpenMP).  motion:
+ * 0 = translation U[-3,3]^2 per frame, 1 = affine (rotation +-2 deg, scale
+ * [0.98,1.02], translation +-8 px).  out: nframes * nrows * ncols bytes. */
+int cuimg_synth_video(uint64_t seed, int nrows, int ncols, int nframes, int motion, uint8_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,340 @@
+"""ctypes front end of the CPU oracle (liboracle_cuimg.so).
+
+TEST INFRASTRUCTURE ONLY.  The oracle is a C++ restatement of the reference
+tracking hot path (each function cites the reference file:line it follows).
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it,
+and only as the checker or as the timed CPU baseline; the product never does.
+
+Parity status of the restated functions (what pins each one):
+  * Gaussian coefficients: the reference's own generator, built from its source
+    under oracle/_ref, plus a committed fixture of its output;
+  * separable Gaussian: impulse / constant / ramp closed forms, brute-force 2D
+    sums, an FP64 twin;
+  * box pyramid: integer closed form;
+  * FAST saliency: known-answer corners and an independent Python restatement;
+  * LK matcher: bilinear exactness, zero motion, recovered translations,
+    degenerate-G failure, rejection branches;
+  * particle container / merge: scripted sequences and an independent Python
+    restatement of the serial semantics;
+  * the photometric loss gradient: central finite differences.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle_cuimg.so")
+
+
+class Particle(ctypes.Structure):
+    _fields_ = [("speed_r", ctypes.c_float), ("speed_c", ctypes.c_float),
+                ("pos_r", ctypes.c_float), ("pos_c", ctypes.c_float),
+                ("age", ctypes.c_uint16), ("pad", ctypes.c_uint16)]
+
+
+PARTICLE_DTYPE = np.dtype([("speed_r", "<f4"), ("speed_c", "<f4"), ("pos_r", "<f4"),
+                           ("pos_c", "<f4"), ("age", "<u2"), ("pad", "<u2")])
+assert PARTICLE_DTYPE.itemsize == 20
+
+
+class TrackerParams(ctypes.Structure):
+    _fields_ = [("nscales", ctypes.c_int32), ("profile", ctypes.c_int32),
+                ("lk_ws", ctypes.c_int32), ("lk_max_iters", ctypes.c_int32),
+                ("lk_eps", ctypes.c_float), ("lk_top_n", ctypes.c_int32),
+                ("fast_n", ctypes.c_int32), ("fast_th", ctypes.c_float),
+                ("box_size", ctypes.c_int32), ("detector_frequency", ctypes.c_int32),
+                ("filtering_frequency", ctypes.c_int32), ("k", ctypes.c_int32),
+                ("with_merge", ctypes.c_int32), ("blur_sigma", ctypes.c_int32),
+                ("blur_K", ctypes.c_int32)]
+
+
+def profile_params(profile, nscales, fast_th=20.0, **kw):
+    """Profile 'R' = the reference's matcher defaults on u8 frames
+    (tracking2/pyrlk_tracker.hpp:173,212,283); profile 'B' = 7x7 window,
+    <=5 iterations, on sigma=1 / 7-tap blurred FP32 frames."""
+    p = TrackerParams()
+    p.nscales = nscales
+    if profile == "R":
+        p.profile, p.lk_ws, p.lk_max_iters, p.lk_top_n = 0, 11, 30, 25
+    elif profile == "B":
+        p.profile, p.lk_ws, p.lk_max_iters, p.lk_top_n = 1, 7, 5, 49
+    else:
+        raise ValueError(profile)
+    p.lk_eps = 0.1
+    p.fast_n = 9
+    p.fast_th = fast_th
+    p.box_size = 3
+    p.detector_frequency = 1
+    p.filtering_frequency = 1
+    p.k = 300
+    p.with_merge = 1
+    p.blur_sigma = 1
+    p.blur_K = 3
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+class Oracle:
+    def __init__(self, path=LIB_PATH):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built (make -C oracle)")
+        L = ctypes.CDLL(path)
+        self.L = L
+        c_int, c_float, c_double, vp = ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_void_p
+        P = ctypes.POINTER
+        L.orc_gauss_coef_runtime.restype = c_float
+        L.orc_gauss_coef_runtime.argtypes = [c_float, c_int]
+        L.orc_gauss_coef_meta.restype = c_float
+        L.orc_gauss_coef_meta.argtypes = [c_int, c_int]
+        L.orc_gauss_coefs.argtypes = [c_int, c_int, c_int, P(c_float)]
+        L.orc_gaussian_sep_f32.argtypes = [P(c_float), c_int, c_int, c_int, P(c_float), c_int, P(c_float)]
+        L.orc_gaussian_sep_u8.argtypes = [P(ctypes.c_uint8), c_int, c_int, P(c_float), c_int, P(c_float)]
+        L.orc_gaussian_sep_f64.argtypes = [P(c_double), c_int, c_int, c_int, P(c_double), c_int, P(c_double)]
+        L.orc_pyr_down_u8.argtypes = [P(ctypes.c_uint8), c_int, c_int, P(ctypes.c_uint8)]
+        L.orc_pyr_down_f32.argtypes = [P(c_float), c_int, c_int, P(c_float)]
+        L.orc_fast_score.restype = ctypes.c_uint8
+        L.orc_fast_score.argtypes = [P(ctypes.c_uint8), c_int, c_int, c_int, c_int, c_int, c_int]
+        L.orc_fast_saliency.argtypes = [P(ctypes.c_uint8), c_int, c_int, P(ctypes.c_uint8), c_int, c_int,
+                                        P(ctypes.c_uint8)]
+        L.orc_sobel_gradient.argtypes = [P(c_float), c_int, c_int, c_int, P(c_float)]
+        L.orc_lk_match.argtypes = [P(c_float), P(c_float), P(c_float), c_int, c_int, c_int, c_int,
+                                   c_float, c_int, c_int, c_float, c_float, c_float, c_float,
+                                   P(c_float), P(c_float)]
+        L.orc_tracker_create.restype = vp
+        L.orc_tracker_create.argtypes = [c_int, c_int, P(TrackerParams)]
+        for name in ("orc_tracker_destroy", "orc_tracker_clear"):
+            getattr(L, name).argtypes = [vp]
+        L.orc_tracker_run.argtypes = [vp, P(ctypes.c_uint8)]
+        L.orc_tracker_nscales.argtypes = [vp]
+        L.orc_tracker_level_dims.argtypes = [vp, c_int, P(c_int), P(c_int)]
+        L.orc_tracker_nparticles.argtypes = [vp, c_int]
+        L.orc_tracker_get_particles.argtypes = [vp, c_int, vp, c_int]
+        L.orc_tracker_set_particles.argtypes = [vp, c_int, vp, c_int]
+        L.orc_tracker_get_matches.argtypes = [vp, c_int, P(ctypes.c_uint32), c_int]
+        L.orc_tracker_compact_has_run.argtypes = [vp, c_int]
+        L.orc_tracker_get_sparse.argtypes = [vp, c_int, P(ctypes.c_int32)]
+        L.orc_tracker_get_saliency.argtypes = [vp, c_int, P(ctypes.c_uint8)]
+        L.orc_tracker_get_level.argtypes = [vp, c_int, P(ctypes.c_uint8)]
+        L.orc_tracker_get_matcher_images.argtypes = [vp, c_int, vp, vp, vp]
+        L.orc_tracker_get_decisions.argtypes = [vp, c_int, P(ctypes.c_int32), P(c_float), P(c_float), c_int]
+        L.orc_tracker_lk_decide.argtypes = [vp, c_int, P(ctypes.c_int32), P(c_float), P(c_float), c_int]
+        L.orc_lk_loss_grad_f64.restype = c_double
+        L.orc_lk_loss_grad_f64.argtypes = [P(c_float), P(c_float), c_int, c_int, P(c_float), c_int,
+                                           P(c_float), P(c_float), c_int, c_int, vp, vp]
+        L.orc_lk_loss_grad_f64_d.restype = c_double
+        L.orc_lk_loss_grad_f64_d.argtypes = [P(c_double), P(c_double), c_int, c_int, P(c_double), c_int,
+                                             P(c_double), P(c_double), c_int, c_int, vp, vp]
+
+    # ---------------------------------------------------------------- gaussian
+    def coef_runtime(self, sigma, x):
+        return self.L.orc_gauss_coef_runtime(float(sigma), int(x))
+
+    def coef_meta(self, sigma, x):
+        return self.L.orc_gauss_coef_meta(int(sigma), int(x))
+
+    def coefs(self, sigma, K, meta=True):
+        out = np.zeros(2 * K + 1, np.float32)
+        self.L.orc_gauss_coefs(int(sigma), int(K), 1 if meta else 0, _ptr(out, ctypes.c_float))
+        return out
+
+    def gaussian(self, img, coef, K):
+        coef = np.ascontiguousarray(coef, np.float32)
+        if img.dtype == np.uint8:
+            img = np.ascontiguousarray(img)
+            out = np.zeros(img.shape, np.float32)
+            self.L.orc_gaussian_sep_u8(_ptr(img, ctypes.c_uint8), img.shape[0], img.shape[1],
+                                       _ptr(coef, ctypes.c_float), K, _ptr(out, ctypes.c_float))
+            return out
+        img = np.ascontiguousarray(img, np.float32)
+        ch = 1 if img.ndim == 2 else img.shape[2]
+        out = np.zeros(img.shape, np.float32)
+        self.L.orc_gaussian_sep_f32(_ptr(img, ctypes.c_float), img.shape[0], img.shape[1], ch,
+                                    _ptr(coef, ctypes.c_float), K, _ptr(out, ctypes.c_float))
+        return out
+
+    def gaussian_f64(self, img, coef, K):
+        img = np.ascontiguousarray(img, np.float64)
+        coef = np.ascontiguousarray(coef, np.float64)
+        ch = 1 if img.ndim == 2 else img.shape[2]
+        out = np.zeros(img.shape, np.float64)
+        self.L.orc_gaussian_sep_f64(_ptr(img, ctypes.c_double), img.shape[0], img.shape[1], ch,
+                                    _ptr(coef, ctypes.c_double), K, _ptr(out, ctypes.c_double))
+        return out
+
+    # ---------------------------------------------------------------- pyramid
+    def pyr_down(self, img):
+        img = np.ascontiguousarray(img)
+        nr, nc = img.shape
+        out = np.zeros((nr // 2, nc // 2), img.dtype)
+        if img.dtype == np.uint8:
+            self.L.orc_pyr_down_u8(_ptr(img, ctypes.c_uint8), nr, nc, _ptr(out, ctypes.c_uint8))
+        else:
+            self.L.orc_pyr_down_f32(_ptr(img, ctypes.c_float), nr, nc, _ptr(out, ctypes.c_float))
+        return out
+
+    # ---------------------------------------------------------------- FAST
+    def fast_score(self, img, r, c, n=9, th=20):
+        img = np.ascontiguousarray(img, np.uint8)
+        return self.L.orc_fast_score(_ptr(img, ctypes.c_uint8), img.shape[0], img.shape[1], r, c, n, th)
+
+    def fast_saliency(self, img, mask=None, n=9, th=20):
+        img = np.ascontiguousarray(img, np.uint8)
+        if mask is None:
+            mask = np.full(img.shape, 255, np.uint8)
+        mask = np.ascontiguousarray(mask, np.uint8)
+        out = np.zeros(img.shape, np.uint8)
+        self.L.orc_fast_saliency(_ptr(img, ctypes.c_uint8), img.shape[0], img.shape[1],
+                                 _ptr(mask, ctypes.c_uint8), n, th, _ptr(out, ctypes.c_uint8))
+        return out
+
+    # ---------------------------------------------------------------- LK
+    def sobel(self, prev, integer_source):
+        prev = np.ascontiguousarray(prev, np.float32)
+        out = np.zeros(prev.shape + (2,), np.float32)
+        self.L.orc_sobel_gradient(_ptr(prev, ctypes.c_float), prev.shape[0], prev.shape[1],
+                                  1 if integer_source else 0, _ptr(out, ctypes.c_float))
+        return out
+
+    def lk_match(self, A, B, G, p, pred, ws=7, max_iters=5, eps=0.1, top_n=49, template_u8=0):
+        A = np.ascontiguousarray(A, np.float32)
+        B = np.ascontiguousarray(B, np.float32)
+        G = np.ascontiguousarray(G, np.float32)
+        tr = np.zeros(2, np.float32)
+        dist = np.zeros(1, np.float32)
+        self.L.orc_lk_match(_ptr(A, ctypes.c_float), _ptr(B, ctypes.c_float), _ptr(G, ctypes.c_float),
+                            A.shape[0], A.shape[1], ws, max_iters, eps, top_n, template_u8,
+                            float(p[0]), float(p[1]), float(pred[0]), float(pred[1]),
+                            _ptr(tr, ctypes.c_float), _ptr(dist, ctypes.c_float))
+        return tr, float(dist[0])
+
+    # ---------------------------------------------------------------- tracker
+    def tracker(self, nr, nc, params):
+        return OracleTracker(self, nr, nc, params)
+
+    # ---------------------------------------------------------------- adjoint
+    def loss_grad(self, I0, I1, coef, K, pos, d, ws=7, want_dI=True):
+        I0 = np.ascontiguousarray(I0, np.float32)
+        I1 = np.ascontiguousarray(I1, np.float32)
+        coef = np.ascontiguousarray(coef, np.float32)
+        pos = np.ascontiguousarray(pos, np.float32)
+        d = np.ascontiguousarray(d, np.float32)
+        n = pos.shape[0]
+        gd = np.zeros((n, 2), np.float64)
+        gI = np.zeros(I1.shape, np.float64) if want_dI else None
+        L = self.L.orc_lk_loss_grad_f64(_ptr(I0, ctypes.c_float), _ptr(I1, ctypes.c_float),
+                                        I0.shape[0], I0.shape[1], _ptr(coef, ctypes.c_float), K,
+                                        _ptr(pos, ctypes.c_float), _ptr(d, ctypes.c_float), n, ws,
+                                        gd.ctypes.data, gI.ctypes.data if want_dI else None)
+        return L, gd, gI
+
+    def loss_f64(self, I0, I1, coef, K, pos, d, ws=7):
+        I0 = np.ascontiguousarray(I0, np.float64)
+        I1 = np.ascontiguousarray(I1, np.float64)
+        coef = np.ascontiguousarray(coef, np.float64)
+        pos = np.ascontiguousarray(pos, np.float64)
+        d = np.ascontiguousarray(d, np.float64)
+        return self.L.orc_lk_loss_grad_f64_d(_ptr(I0, ctypes.c_double), _ptr(I1, ctypes.c_double),
+                                             I0.shape[0], I0.shape[1], _ptr(coef, ctypes.c_double),
+                                             K, _ptr(pos, ctypes.c_double), _ptr(d, ctypes.c_double),
+                                             pos.shape[0], ws, None, None)
+
+
+class OracleTracker:
+    def __init__(self, orc, nr, nc, params):
+        self.o = orc
+        self.L = orc.L
+        self.params = params
+        self.h = self.L.orc_tracker_create(nr, nc, ctypes.byref(params))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_tracker_destroy(self.h)
+            self.h = None
+
+    def run(self, frame):
+        frame = np.ascontiguousarray(frame, np.uint8)
+        self.L.orc_tracker_run(self.h, _ptr(frame, ctypes.c_uint8))
+
+    def clear(self):
+        self.L.orc_tracker_clear(self.h)
+
+    @property
+    def nscales(self):
+        return self.L.orc_tracker_nscales(self.h)
+
+    def dims(self, s):
+        nr, nc = ctypes.c_int(), ctypes.c_int()
+        self.L.orc_tracker_level_dims(self.h, s, ctypes.byref(nr), ctypes.byref(nc))
+        return nr.value, nc.value
+
+    def particles(self, s):
+        n = self.L.orc_tracker_nparticles(self.h, s)
+        out = np.zeros(n, PARTICLE_DTYPE)
+        if n:
+            self.L.orc_tracker_get_particles(self.h, s, out.ctypes.data, n)
+        return out
+
+    def set_particles(self, s, parts):
+        parts = np.ascontiguousarray(parts, PARTICLE_DTYPE)
+        self.L.orc_tracker_set_particles(self.h, s, parts.ctypes.data, len(parts))
+
+    def matches(self, s):
+        cap = self.L.orc_tracker_get_matches(self.h, s, None, 0)
+        n = -cap if cap < 0 else 0
+        out = np.zeros(n, np.uint32)
+        if n:
+            self.L.orc_tracker_get_matches(self.h, s, _ptr(out, ctypes.c_uint32), n)
+        return out
+
+    def compact_has_run(self, s):
+        return bool(self.L.orc_tracker_compact_has_run(self.h, s))
+
+    def sparse(self, s):
+        nr, nc = self.dims(s)
+        out = np.zeros((nr, nc), np.int32)
+        self.L.orc_tracker_get_sparse(self.h, s, _ptr(out, ctypes.c_int32))
+        return out
+
+    def saliency(self, s):
+        nr, nc = self.dims(s)
+        out = np.zeros((nr, nc), np.uint8)
+        self.L.orc_tracker_get_saliency(self.h, s, _ptr(out, ctypes.c_uint8))
+        return out
+
+    def level(self, s):
+        nr, nc = self.dims(s)
+        out = np.zeros((nr, nc), np.uint8)
+        self.L.orc_tracker_get_level(self.h, s, _ptr(out, ctypes.c_uint8))
+        return out
+
+    def matcher_images(self, s):
+        nr, nc = self.dims(s)
+        A = np.zeros((nr, nc), np.float32)
+        B = np.zeros((nr, nc), np.float32)
+        G = np.zeros((nr, nc, 2), np.float32)
+        self.L.orc_tracker_get_matcher_images(self.h, s, A.ctypes.data, B.ctypes.data, G.ctypes.data)
+        return A, B, G
+
+    def _decisions(self, fn, s):
+        n = self.L.orc_tracker_nparticles(self.h, s) + 1
+        while True:
+            acc = np.zeros(n, np.int32)
+            tr = np.zeros((n, 2), np.float32)
+            dist = np.zeros(n, np.float32)
+            k = fn(self.h, s, _ptr(acc, ctypes.c_int32), _ptr(tr, ctypes.c_float),
+                   _ptr(dist, ctypes.c_float), n)
+            if k >= 0:
+                return acc[:k], tr[:k], dist[:k]
+            n = -k
+
+    def last_decisions(self, s):
+        return self._decisions(self.L.orc_tracker_get_decisions, s)
+
+    def lk_decide(self, s):
+        return self._decisions(self.L.orc_tracker_lk_decide, s)

@@ -1,0 +1,233 @@
+// C entry points of the oracle for ctypes.  TEST INFRASTRUCTURE ONLY.
+#include "oracle.h"
+
+#include "oracle_core.hpp"
+#include "oracle_tracker.hpp"
+
+namespace orc {
+double lk_loss_grad_f64(const image<double>& I0, const image<double>& I1, const double* coef,
+                        int K, const double* pos2, const double* d2, int np, int ws,
+                        double* dL_dd2, image<double>* dL_dI1);
+}
+
+using namespace orc;
+
+template <typename T>
+static image<T> wrap(const T* p, int nr, int nc) {
+  image<T> im(nr, nc);
+  std::memcpy(im.px.data(), p, sizeof(T) * (size_t)nr * nc);
+  return im;
+}
+
+extern "C" {
+
+float orc_gauss_coef_runtime(float sigma, int x) { return gauss_coef_runtime(sigma, x); }
+float orc_gauss_coef_meta(int sigma, int x) { return gauss_coef_meta(sigma, x); }
+void orc_gauss_coefs(int sigma, int K, int meta_rounding, float* out) {
+  gauss_coefs(sigma, K, meta_rounding != 0, out);
+}
+
+void orc_gaussian_sep_f32(const float* in, int nr, int nc, int ch, const float* coef, int K,
+                          float* out) {
+  for (int k = 0; k < ch; k++) {
+    image<float> a(nr, nc), o;
+    for (size_t i = 0; i < a.px.size(); i++) a.px[i] = in[i * ch + k];
+    gaussian_sep_f32<float>(a, coef, K, o);
+    for (size_t i = 0; i < o.px.size(); i++) out[i * ch + k] = o.px[i];
+  }
+}
+
+void orc_gaussian_sep_u8(const uint8_t* in, int nr, int nc, const float* coef, int K, float* out) {
+  image<float> o;
+  gaussian_sep_f32<uint8_t>(wrap(in, nr, nc), coef, K, o);
+  std::memcpy(out, o.px.data(), sizeof(float) * o.px.size());
+}
+
+void orc_gaussian_sep_f64(const double* in, int nr, int nc, int ch, const double* coef, int K,
+                          double* out) {
+  for (int k = 0; k < ch; k++) {
+    image<double> a(nr, nc), o;
+    for (size_t i = 0; i < a.px.size(); i++) a.px[i] = in[i * ch + k];
+    gaussian_sep_f64(a, coef, K, o);
+    for (size_t i = 0; i < o.px.size(); i++) out[i * ch + k] = o.px[i];
+  }
+}
+
+void orc_pyr_down_u8(const uint8_t* in, int nr, int nc, uint8_t* out) {
+  image<uint8_t> o;
+  pyr_down_u8(wrap(in, nr, nc), o);
+  std::memcpy(out, o.px.data(), o.px.size());
+}
+
+void orc_pyr_down_f32(const float* in, int nr, int nc, float* out) {
+  image<float> o;
+  pyr_down_f32(wrap(in, nr, nc), o);
+  std::memcpy(out, o.px.data(), sizeof(float) * o.px.size());
+}
+
+uint8_t orc_fast_score(const uint8_t* in, int nr, int nc, int r, int c, int n, int th) {
+  return fast_score(wrap(in, nr, nc), r, c, n, th);
+}
+
+void orc_fast_saliency(const uint8_t* in, int nr, int nc, const uint8_t* mask, int n, int th,
+                       uint8_t* out) {
+  image<uint8_t> s;
+  fast_saliency(wrap(in, nr, nc), wrap(mask, nr, nc), n, th, s);
+  std::memcpy(out, s.px.data(), s.px.size());
+}
+
+void orc_sobel_gradient(const float* prev, int nr, int nc, int integer_source, float* out2) {
+  image<f2> g;
+  sobel_gradient(wrap(prev, nr, nc), integer_source != 0, g);
+  std::memcpy(out2, g.px.data(), sizeof(f2) * g.px.size());
+}
+
+void orc_lk_match(const float* A, const float* B, const float* G, int nr, int nc, int ws,
+                  int max_iters, float eps, int top_n, int template_u8, float p_r, float p_c,
+                  float pred_r, float pred_c, float* out_tr2, float* out_dist) {
+  image<float> a = wrap(A, nr, nc), b = wrap(B, nr, nc);
+  image<f2> g = wrap(reinterpret_cast<const f2*>(G), nr, nc);
+  lk_params P;
+  P.ws = ws;
+  P.max_iters = max_iters;
+  P.eps = eps;
+  P.top_n = top_n;
+  P.template_u8 = template_u8;
+  lk_level L{&a, &b, &g};
+  match_result m = lk_match(P, f2{p_r, p_c}, f2{pred_r, pred_c}, L);
+  out_tr2[0] = m.tr.r;
+  out_tr2[1] = m.tr.c;
+  *out_dist = m.dist;
+}
+
+static tracker_params to_params(const orc_tracker_params* p) {
+  tracker_params t;
+  t.nscales = p->nscales;
+  t.profile = p->profile;
+  t.lk.ws = p->lk_ws;
+  t.lk.max_iters = p->lk_max_iters;
+  t.lk.eps = p->lk_eps;
+  t.lk.top_n = p->lk_top_n;
+  t.lk.template_u8 = p->profile == 0 ? 1 : 0;
+  t.fast_n = p->fast_n;
+  t.fast_th = p->fast_th;
+  t.box_size = p->box_size;
+  t.detector_frequency = p->detector_frequency;
+  t.filtering_frequency = p->filtering_frequency;
+  t.k = p->k;
+  t.with_merge = p->with_merge;
+  t.blur_sigma = p->blur_sigma;
+  t.blur_K = p->blur_K;
+  return t;
+}
+
+void* orc_tracker_create(int nr, int nc, const orc_tracker_params* p) {
+  return new tracker(nr, nc, to_params(p));
+}
+void orc_tracker_destroy(void* t) { delete static_cast<tracker*>(t); }
+void orc_tracker_clear(void* t) { static_cast<tracker*>(t)->clear(); }
+void orc_tracker_run(void* t, const uint8_t* frame) { static_cast<tracker*>(t)->run(frame); }
+int orc_tracker_nscales(void* t) { return (int)static_cast<tracker*>(t)->levels.size(); }
+
+void orc_tracker_level_dims(void* t, int s, int* nr, int* nc) {
+  const level_state& L = static_cast<tracker*>(t)->levels[s];
+  *nr = L.nr;
+  *nc = L.nc;
+}
+
+int orc_tracker_nparticles(void* t, int s) {
+  return (int)static_cast<tracker*>(t)->levels[s].parts.size();
+}
+
+int orc_tracker_get_particles(void* t, int s, orc_particle* out, int cap) {
+  const level_state& L = static_cast<tracker*>(t)->levels[s];
+  int n = (int)L.parts.size();
+  if (n > cap) return -n;
+  std::memcpy(out, L.parts.data(), sizeof(particle) * n);
+  return n;
+}
+
+int orc_tracker_set_particles(void* t, int s, const orc_particle* in, int n) {
+  level_state& L = static_cast<tracker*>(t)->levels[s];
+  L.parts.resize(n);
+  std::memcpy(L.parts.data(), in, sizeof(particle) * n);
+  return n;
+}
+
+int orc_tracker_get_matches(void* t, int s, uint32_t* out, int cap) {
+  const level_state& L = static_cast<tracker*>(t)->levels[s];
+  int n = (int)L.matches.size();
+  if (n > cap) return -n;
+  std::memcpy(out, L.matches.data(), sizeof(uint32_t) * n);
+  return n;
+}
+
+int orc_tracker_compact_has_run(void* t, int s) {
+  return static_cast<tracker*>(t)->levels[s].compact_has_run ? 1 : 0;
+}
+
+void orc_tracker_get_sparse(void* t, int s, int32_t* out) {
+  const level_state& L = static_cast<tracker*>(t)->levels[s];
+  std::memcpy(out, L.sparse.px.data(), sizeof(int32_t) * L.sparse.px.size());
+}
+
+void orc_tracker_get_saliency(void* t, int s, uint8_t* out) {
+  const level_state& L = static_cast<tracker*>(t)->levels[s];
+  if (L.sal.px.empty()) std::memset(out, 0, (size_t)L.nr * L.nc);
+  else std::memcpy(out, L.sal.px.data(), L.sal.px.size());
+}
+
+void orc_tracker_get_level(void* t, int s, uint8_t* out) {
+  const level_state& L = static_cast<tracker*>(t)->levels[s];
+  std::memcpy(out, L.in_u8.px.data(), L.in_u8.px.size());
+}
+
+void orc_tracker_get_matcher_images(void* t, int s, float* A, float* B, float* G2) {
+  const level_state& L = static_cast<tracker*>(t)->levels[s];
+  if (A) std::memcpy(A, L.A.px.data(), sizeof(float) * L.A.px.size());
+  if (B) std::memcpy(B, L.B.px.data(), sizeof(float) * L.B.px.size());
+  if (G2) std::memcpy(G2, L.G.px.data(), sizeof(f2) * L.G.px.size());
+}
+
+static int copy_decisions(const std::vector<lk_decision>& d, int32_t* accept, float* tr2,
+                          float* dist, int cap) {
+  int n = (int)d.size();
+  if (n > cap) return -n;
+  for (int i = 0; i < n; i++) {
+    accept[i] = d[i].accept;
+    tr2[2 * i] = d[i].tr.r;
+    tr2[2 * i + 1] = d[i].tr.c;
+    dist[i] = d[i].dist;
+  }
+  return n;
+}
+
+int orc_tracker_get_decisions(void* t, int s, int32_t* accept, float* tr2, float* dist, int cap) {
+  return copy_decisions(static_cast<tracker*>(t)->levels[s].last_decisions, accept, tr2, dist, cap);
+}
+
+int orc_tracker_lk_decide(void* t, int s, int32_t* accept, float* tr2, float* dist, int cap) {
+  std::vector<lk_decision> d;
+  static_cast<tracker*>(t)->lk_decide(s, d);
+  return copy_decisions(d, accept, tr2, dist, cap);
+}
+
+double orc_lk_loss_grad_f64_d(const double* I0, const double* I1, int nr, int nc,
+                              const double* coef, int K, const double* pos2, const double* d2,
+                              int np, int ws, double* dL_dd2, double* dL_dI1) {
+  image<double> a = wrap(I0, nr, nc), b = wrap(I1, nr, nc), g;
+  double L = lk_loss_grad_f64(a, b, coef, K, pos2, d2, np, ws, dL_dd2, dL_dI1 ? &g : nullptr);
+  if (dL_dI1) std::memcpy(dL_dI1, g.px.data(), sizeof(double) * g.px.size());
+  return L;
+}
+
+double orc_lk_loss_grad_f64(const float* I0, const float* I1, int nr, int nc, const float* coef,
+                            int K, const float* pos2, const float* d2, int np, int ws,
+                            double* dL_dd2, double* dL_dI1) {
+  std::vector<double> a(I0, I0 + (size_t)nr * nc), b(I1, I1 + (size_t)nr * nc);
+  std::vector<double> cf(coef, coef + 2 * K + 1), p(pos2, pos2 + 2 * np), d(d2, d2 + 2 * np);
+  return orc_lk_loss_grad_f64_d(a.data(), b.data(), nr, nc, cf.data(), K, p.data(), d.data(), np,
+                                ws, dL_dd2, dL_dI1);
+}
+
+}  // extern "C"

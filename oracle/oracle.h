@@ -1,0 +1,97 @@
+/* C entry points of the CPU oracle (loaded with ctypes by tests/ and bench.py's
+ * cpu_baseline leg).  TEST INFRASTRUCTURE ONLY: the product never calls these.
+ * Images are dense row-major arrays without border; 2-vectors are (row, col). */
+#ifndef CUIMG_ORACLE_H_
+#define CUIMG_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  float speed_r, speed_c;
+  float pos_r, pos_c;
+  uint16_t age;
+  uint16_t pad;
+} orc_particle;
+
+typedef struct {
+  int32_t nscales;
+  int32_t profile;        /* 0: reference defaults (u8), 1: blurred FP32 */
+  int32_t lk_ws;
+  int32_t lk_max_iters;
+  float lk_eps;
+  int32_t lk_top_n;
+  int32_t fast_n;
+  float fast_th;
+  int32_t box_size;
+  int32_t detector_frequency;
+  int32_t filtering_frequency;
+  int32_t k;
+  int32_t with_merge;
+  int32_t blur_sigma;
+  int32_t blur_K;
+} orc_tracker_params;
+
+float orc_gauss_coef_runtime(float sigma, int x);
+float orc_gauss_coef_meta(int sigma, int x);
+void orc_gauss_coefs(int sigma, int K, int meta_rounding, float* out);
+
+/* Separable Gaussian, GPU meta-kernel semantics; ch interleaved channels. */
+void orc_gaussian_sep_f32(const float* in, int nr, int nc, int ch, const float* coef, int K,
+                          float* out);
+void orc_gaussian_sep_u8(const uint8_t* in, int nr, int nc, const float* coef, int K, float* out);
+void orc_gaussian_sep_f64(const double* in, int nr, int nc, int ch, const double* coef, int K,
+                          double* out);
+
+void orc_pyr_down_u8(const uint8_t* in, int nr, int nc, uint8_t* out);
+void orc_pyr_down_f32(const float* in, int nr, int nc, float* out);
+
+uint8_t orc_fast_score(const uint8_t* in, int nr, int nc, int r, int c, int n, int th);
+void orc_fast_saliency(const uint8_t* in, int nr, int nc, const uint8_t* mask, int n, int th,
+                       uint8_t* out);
+
+void orc_sobel_gradient(const float* prev, int nr, int nc, int integer_source, float* out2);
+
+/* One LK match on a single level (A, B float images, G = 2-channel gradient). */
+void orc_lk_match(const float* A, const float* B, const float* G, int nr, int nc, int ws,
+                  int max_iters, float eps, int top_n, int template_u8, float p_r, float p_c,
+                  float pred_r, float pred_c, float* out_tr2, float* out_dist);
+
+void* orc_tracker_create(int nr, int nc, const orc_tracker_params* p);
+void orc_tracker_destroy(void* t);
+void orc_tracker_clear(void* t);
+void orc_tracker_run(void* t, const uint8_t* frame);
+int orc_tracker_nscales(void* t);
+void orc_tracker_level_dims(void* t, int s, int* nr, int* nc);
+int orc_tracker_nparticles(void* t, int s);
+int orc_tracker_get_particles(void* t, int s, orc_particle* out, int cap);
+int orc_tracker_set_particles(void* t, int s, const orc_particle* in, int n);
+int orc_tracker_get_matches(void* t, int s, uint32_t* out, int cap);
+int orc_tracker_compact_has_run(void* t, int s);
+void orc_tracker_get_sparse(void* t, int s, int32_t* out);
+void orc_tracker_get_saliency(void* t, int s, uint8_t* out);
+void orc_tracker_get_level(void* t, int s, uint8_t* out);
+void orc_tracker_get_matcher_images(void* t, int s, float* A, float* B, float* G2);
+/* Last frame's per-particle LK decision (before move/remove) at scale s. */
+int orc_tracker_get_decisions(void* t, int s, int32_t* accept, float* tr2, float* dist, int cap);
+
+/* Per-particle decisions of one scale's matcher on the current state, without
+ * changing it: lets a test feed identical particles to the GPU. */
+int orc_tracker_lk_decide(void* t, int s, int32_t* accept, float* tr2, float* dist, int cap);
+
+/* FP64 photometric LK loss and its gradient (blurred frames, bilinear samples). */
+double orc_lk_loss_grad_f64(const float* I0, const float* I1, int nr, int nc, const float* coef,
+                            int K, const float* pos2, const float* d2, int np, int ws,
+                            double* dL_dd2, double* dL_dI1);
+/* Same in FP64 inputs (used for central finite differences). */
+double orc_lk_loss_grad_f64_d(const double* I0, const double* I1, int nr, int nc,
+                              const double* coef, int K, const double* pos2, const double* d2,
+                              int np, int ws, double* dL_dd2, double* dL_dI1);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,66 @@
+// Oracle tracker state.  TEST INFRASTRUCTURE ONLY (see oracle_core.hpp).
+#pragma once
+#include "oracle_core.hpp"
+
+namespace orc {
+
+struct tracker_params {
+  int nscales = 3;
+  int profile = 1;  // 0: reference defaults on u8 frames, 1: blurred FP32 frames
+  lk_params lk;
+  int fast_n = 9;
+  float fast_th = 20.f;
+  int box_size = 3;
+  int detector_frequency = 1;
+  int filtering_frequency = 1;
+  int k = 300;
+  int with_merge = 1;
+  int blur_sigma = 1;
+  int blur_K = 3;
+};
+
+struct lk_decision {
+  int accept;
+  f2 tr;
+  float dist;
+};
+
+struct level_state {
+  int nr = 0, nc = 0;
+  image<uint8_t> in_u8;            // tracker::input_ of this scale
+  image<uint8_t> prev_u8, cur_u8;  // strategy prev_input_ / input_
+  image<float> A, B;               // matcher images (previous / current)
+  image<f2> G;                     // gradient of A
+  image<uint8_t> mask, sal;
+  image<int32_t> sparse;
+  std::vector<particle> parts;
+  std::vector<uint32_t> matches;
+  std::vector<lk_decision> last_decisions;
+  bool compact_has_run = false;
+  int frame_cpt = 0;
+  box pdomain{0, 0, -1, -1};
+};
+
+struct tracker {
+  tracker_params P;
+  std::vector<level_state> levels;
+  std::vector<float> blur_coef;
+
+  tracker(int nr, int nc, const tracker_params& p);
+  void run(const uint8_t* frame);
+  void clear();
+
+  void update_input(const uint8_t* frame);
+  void update_level(int s);
+  void prepare_matcher_images(level_state& L);
+  f2 flow_at(int j, f2 p) const;
+  f2 prediction(int s, f2 pos, int age, f2 speed) const;
+  void lk_decide(int s, std::vector<lk_decision>& out) const;
+  void match_particles(int s);
+  void detector_mask(level_state& L);
+  void new_particles(level_state& L);
+  void compact(level_state& L);
+  void merge(level_state& L);
+};
+
+}  // namespace orc

@@ -1,0 +1,171 @@
+"""GPU parity of the image kernels against the CPU oracle (through the C ABI).
+
+Gaussian: within 1e-5 relative of the oracle (FP32, the oracle sums without
+FMA contraction, the GPU with FMA; both accumulate tap +K first).
+Pyramid, border fill and FAST saliency: bit-exact.
+"""
+import numpy as np
+import pytest
+
+import cuimg_amd as cu
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = cu.Context(0)
+    yield c
+    c.close()
+
+
+def rel_err(got, ref):
+    scale = np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max() + 1e-30)
+    return float((np.abs(got.astype(np.float64) - ref) / scale).max())
+
+
+GAUSS_CASES = [
+    ("u8", (240, 320), 1),
+    ("u8", (1080, 1920), 1),
+    ("u8", (135, 241), 2),
+    ("f32", (17, 13), 1),
+    ("f32", (5, 4), 2),
+    ("f32", (1, 9), 1),
+    ("f32", (9, 1), 1),
+    ("f32", (1024, 1024), 1),
+    ("f32", (1024, 1000), 2),
+    ("f32", (777, 1031), 4),
+    ("f32x4", (270, 480), 1),
+    ("f32x4", (135, 250), 4),
+]
+
+
+@pytest.mark.parametrize("kind,shape,sigma", GAUSS_CASES)
+def test_gaussian_matches_oracle(ctx, orc, kind, shape, sigma):
+    print(f"gaussian {kind} {shape} sigma={sigma}", flush=True)
+    rng = np.random.default_rng(hash((kind, shape, sigma)) & 0xFFFF)
+    K = 3 * sigma
+    coef = orc.coefs(sigma, K)
+    assert np.array_equal(coef, cu.gaussian_coefs(sigma, K))
+    if kind == "u8":
+        img = rng.integers(0, 256, shape, dtype=np.uint8)
+        out_dt = cu.F32
+    elif kind == "f32":
+        img = rng.uniform(0, 255, shape).astype(np.float32)
+        out_dt = cu.F32
+    else:
+        img = rng.uniform(0, 255, shape + (4,)).astype(np.float32)
+        out_dt = cu.F32X4
+    src = ctx.upload(img, border=3 if kind == "u8" else 0)
+    dst = ctx.image(shape[0], shape[1], out_dt)
+    ctx.gaussian(src, dst, sigma, K)
+    got = dst.download()
+    ref = orc.gaussian(img, coef, K)
+    assert rel_err(got, ref) < 1e-5
+
+
+def test_gaussian_multi_levels(ctx, orc):
+    rng = np.random.default_rng(5)
+    shapes = [(1080, 1920), (540, 960), (270, 480), (135, 240)]
+    imgs = [rng.integers(0, 256, s, dtype=np.uint8) for s in shapes]
+    srcs = [ctx.upload(i, border=3) for i in imgs]
+    dsts = [ctx.image(s[0], s[1], cu.F32) for s in shapes]
+    ctx.gaussian_multi(srcs, dsts, 1, 3)
+    coef = orc.coefs(1, 3)
+    for i, d in zip(imgs, dsts):
+        assert rel_err(d.download(), orc.gaussian(i, coef, 3)) < 1e-5
+
+
+@pytest.mark.parametrize("th", [16, 64, 128])
+def test_gaussian_tile_heights_agree(ctx, orc, th):
+    rng = np.random.default_rng(th)
+    img = rng.uniform(0, 255, (300, 700)).astype(np.float32)
+    coef = orc.coefs(2, 6)
+    src = ctx.upload(img)
+    dst = ctx.image(300, 700, cu.F32)
+    ctx.convolve(src, dst, coef, 6, th=th)
+    assert rel_err(dst.download(), orc.gaussian(img, coef, 6)) < 1e-5
+
+
+def clamp_pad(a, b):
+    return np.pad(a, b, mode="edge")
+
+
+@pytest.mark.parametrize("shape,nlev", [((1080, 1920), 4), ((240, 320), 3), ((135, 241), 3), ((67, 131), 5)])
+def test_pyramid_u8_bit_exact(ctx, orc, shape, nlev):
+    rng = np.random.default_rng(shape[1])
+    frame = rng.integers(0, 256, shape, dtype=np.uint8)
+    f = ctx.upload(frame)
+    levels, ref = [], [frame]
+    r, c = shape
+    for l in range(nlev):
+        levels.append(ctx.image(r, c, cu.U8, border=3))
+        r, c = r // 2, c // 2
+    for l in range(1, nlev):
+        ref.append(orc.pyr_down(ref[-1]))
+    ctx.pyramid(f, levels)
+    for l in range(nlev):
+        got = levels[l].download_with_border()
+        assert np.array_equal(got, clamp_pad(ref[l], 3)), l
+
+
+def test_pyramid_f32_in_place(ctx, orc):
+    rng = np.random.default_rng(2)
+    img = rng.uniform(0, 255, (512, 768)).astype(np.float32)
+    levels = [ctx.upload(img)]
+    ref = [img]
+    for l in range(1, 5):
+        r, c = levels[-1].shape
+        levels.append(ctx.image(r // 2, c // 2, cu.F32))
+        ref.append(orc.pyr_down(ref[-1]))
+    ctx.pyramid(levels[0], levels)
+    for l in range(5):
+        assert np.array_equal(levels[l].download(), ref[l]), l
+
+
+def test_pyr_down_leaves_borders(ctx, orc):
+    rng = np.random.default_rng(4)
+    img = rng.integers(0, 256, (50, 70), dtype=np.uint8)
+    src = ctx.upload(img, border=3)
+    dst = ctx.image(25, 35, cu.U8, border=3)
+    ctx.pyr_down(src, dst)
+    assert np.array_equal(dst.download(), orc.pyr_down(img))
+
+
+@pytest.mark.parametrize("dtype", [cu.U8, cu.U32, cu.F32X2, cu.F32X4])
+def test_fill_border_clamp(ctx, dtype):
+    np_t, tail = {cu.U8: (np.uint8, ()), cu.U32: (np.uint32, ()), cu.F32X2: (np.float32, (2,)),
+                  cu.F32X4: (np.float32, (4,))}[dtype]
+    rng = np.random.default_rng(dtype)
+    a = (rng.integers(0, 200, (13, 17) + tail)).astype(np_t)
+    im = ctx.image(13, 17, dtype, border=4)
+    im.upload(a)
+    ctx.fill_border(im)
+    pad = [(4, 4), (4, 4)] + [(0, 0)] * len(tail)
+    assert np.array_equal(im.download_with_border(), np.pad(a, pad, mode="edge"))
+
+
+@pytest.mark.parametrize("seed,shape", [(1, (240, 320)), (2, (1080, 1920)), (3, (61, 95))])
+def test_fast_saliency_bit_exact(ctx, orc, seed, shape):
+    frames = cu.synth_video(0xC0FFEE00 + seed, shape[0], shape[1], 1, 1)
+    img = frames[0]
+    rng = np.random.default_rng(seed)
+    mask = np.where(rng.random(shape) < 0.2, 0, 255).astype(np.uint8)
+    src = ctx.upload(img, border=3)
+    ctx.fill_border(src)
+    m = ctx.upload(mask)
+    out = ctx.image(shape[0], shape[1], cu.U8)
+    for th in (20, 5):
+        ctx.fast(src, m, out, 9, th)
+        assert np.array_equal(out.download(), orc.fast_saliency(img, mask, 9, th))
+        ctx.fast(src, None, out, 9, th)
+        assert np.array_equal(out.download(), orc.fast_saliency(img, None, 9, th))
+
+
+def test_fast_small_border_uses_clamp(ctx, orc):
+    rng = np.random.default_rng(8)
+    img = rng.integers(0, 256, (40, 37), dtype=np.uint8)
+    src = ctx.upload(img, border=0)
+    out = ctx.image(40, 37, cu.U8)
+    ctx.fast(src, None, out, 9, 10)
+    assert np.array_equal(out.download(), orc.fast_saliency(img, None, 9, 10))

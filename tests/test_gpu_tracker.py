@@ -1,0 +1,158 @@
+"""GPU tracker vs the CPU oracle, frame by frame, through the C ABI.
+
+Lockstep: before every frame the GPU tracker receives the oracle's particle
+vectors (all scales), both run the same frame, and the frame's outputs are
+compared:
+  * pyramid levels: bit-exact;
+  * matcher decisions: accept flags equal and |translation difference| <= 0.01 px
+    for >= 99.9 % of particles (the matcher's window sums are reduced in a
+    different order on the GPU, so FP32 results differ in the last bits);
+  * when every decision agrees and the truncated new positions agree, the rest
+    of the frame is integer work and must match exactly: saliency, the dense
+    particle vector (order, ages), matches and the sparse index image.
+"""
+import numpy as np
+import pytest
+
+import cuimg_amd as cu
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL_PX = 0.01
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = cu.Context(0)
+    yield c
+    c.close()
+
+
+def to_cu(parts):
+    out = np.zeros(len(parts), cu.PARTICLE_DTYPE)
+    for k in parts.dtype.names:
+        out[k] = parts[k]
+    return out
+
+
+def compare_frame(o, g, L, stats, exact_checks=True):
+    frame_exact = True
+    for s in range(L):
+        acc_o, tr_o, _ = o.last_decisions(s)
+        acc_g, tr_g, _ = g.decisions(s)
+        assert len(acc_o) == len(acc_g), (s, len(acc_o), len(acc_g))
+        same = acc_o == acc_g
+        stats["decisions"] += len(acc_o)
+        stats["decision_mismatch"] += int((~same).sum())
+        both = same & (acc_o == 1)
+        d = np.abs(tr_o[both] - tr_g[both]).max(axis=1) if both.any() else np.zeros(0)
+        stats["tracked"] += int(both.sum())
+        stats["within"] += int((d <= TOL_PX).sum())
+        if (~same).any() or (d > 1e-3).any():
+            frame_exact = False
+    for s in range(L):
+        assert np.array_equal(o.level(s), g.level(s)), f"pyramid level {s}"
+    if not (exact_checks and frame_exact):
+        stats["frames_inexact"] += 1
+        return
+    for s in range(L):
+        po, pg = o.particles(s), g.particles(s)
+        trunc_o = np.stack([po["pos_r"].astype(np.int32), po["pos_c"].astype(np.int32)], 1)
+        trunc_g = np.stack([pg["pos_r"].astype(np.int32), pg["pos_c"].astype(np.int32)], 1)
+        if len(po) != len(pg) or not np.array_equal(trunc_o, trunc_g):
+            stats["frames_inexact"] += 1
+            return
+    for s in range(L):
+        assert np.array_equal(o.saliency(s), g.saliency(s)), f"saliency {s}"
+        po, pg = o.particles(s), g.particles(s)
+        assert np.array_equal(po["age"], pg["age"]), f"ages {s}"
+        assert np.abs(po["pos_r"] - pg["pos_r"]).max(initial=0) <= 1e-3
+        assert np.abs(po["pos_c"] - pg["pos_c"]).max(initial=0) <= 1e-3
+        assert o.compact_has_run(s) == g.compact_has_run(s)
+        if o.compact_has_run(s):
+            assert np.array_equal(o.matches(s), g.matches(s)), f"matches {s}"
+        assert np.array_equal(o.sparse(s), g.sparse(s)), f"sparse {s}"
+    stats["frames_exact"] += 1
+
+
+def run_lockstep(ctx, orc, video, params_o, params_g, nframes):
+    nr, nc = video.shape[1:]
+    L = params_o.nscales
+    o = orc.tracker(nr, nc, params_o)
+    g = cu.Tracker(ctx, nr, nc, params_g)
+    stats = dict(decisions=0, decision_mismatch=0, tracked=0, within=0, frames_exact=0, frames_inexact=0)
+    for f in range(nframes):
+        print(f"frame {f}", flush=True)
+        for s in range(L):
+            g.set_particles(s, to_cu(o.particles(s)))
+        o.run(video[f])
+        g.run(video[f])
+        assert g.frame_count == f + 1
+        compare_frame(o, g, L, stats)
+    return stats
+
+
+@pytest.mark.parametrize("profile", ["B", "R"])
+def test_lockstep_config1_size(ctx, orc, profile):
+    video = cu.synth_video(0xC0FFEE01, 240, 320, 12, 0)
+    p_o = oracle.profile_params(profile, 3)
+    p_g = cu.profile_params(profile, 3)
+    st = run_lockstep(ctx, orc, video, p_o, p_g, 12)
+    print(st)
+    assert st["tracked"] > 1000
+    assert st["within"] >= 0.999 * st["tracked"]
+    assert st["decision_mismatch"] <= 0.001 * st["decisions"]
+    assert st["frames_exact"] >= 10
+
+
+def test_lockstep_affine_4_levels(ctx, orc):
+    video = cu.synth_video(0xC0FFEE02, 360, 640, 8, 1)
+    st = run_lockstep(ctx, orc, video, oracle.profile_params("B", 4), cu.profile_params("B", 4), 8)
+    print(st)
+    assert st["within"] >= 0.999 * st["tracked"]
+    assert st["frames_exact"] >= 6
+
+
+def test_detector_and_merge_frequencies(ctx, orc):
+    video = cu.synth_video(0xC0FFEE05, 200, 280, 10, 0)
+    kw = dict(detector_frequency=3, filtering_frequency=2)
+    st = run_lockstep(ctx, orc, video, oracle.profile_params("B", 3, **kw), cu.profile_params("B", 3, **kw), 10)
+    assert st["frames_exact"] >= 8
+
+
+def test_independent_run_statistics(ctx, orc):
+    """Both trackers run on their own for 30 frames: the particle populations
+    stay statistically identical (counts within 1 %)."""
+    video = cu.synth_video(0xC0FFEE01, 240, 320, 30, 0)
+    o = orc.tracker(240, 320, oracle.profile_params("B", 3))
+    g = cu.Tracker(ctx, 240, 320, cu.profile_params("B", 3))
+    for f in range(30):
+        o.run(video[f])
+        g.run(video[f])
+    for s in range(3):
+        no, ng = len(o.particles(s)), len(g.particles(s))
+        assert abs(no - ng) <= max(2, 0.01 * no), (s, no, ng)
+
+
+def test_clear_resets(ctx):
+    video = cu.synth_video(0xC0FFEE01, 120, 160, 3, 0)
+    g = cu.Tracker(ctx, 120, 160, cu.profile_params("B", 2))
+    for f in range(3):
+        g.run(video[f])
+    assert g.nparticles(0) > 0
+    g.clear()
+    assert g.nparticles(0) == 0 and g.frame_count == 0
+    assert (g.sparse(0) == -1).all()
+    g.run(video[0])
+    assert g.nparticles(0) > 0
+
+
+def test_bad_arguments_raise(ctx):
+    with pytest.raises(cu.CuimgError):
+        cu.Tracker(ctx, 100, 100, cu.profile_params("B", 9))
+    with pytest.raises(cu.CuimgError):
+        cu.Tracker(ctx, 100, 100, cu.profile_params("B", 2, lk_ws=8))
+    g = cu.Tracker(ctx, 64, 64, cu.profile_params("B", 2))
+    with pytest.raises(cu.CuimgError):
+        g.run(ctx.image(32, 32, cu.U8))
