@@ -1,0 +1,258 @@
+"""Benchmark of the tracking hot path on one MI355X.
+
+Workload (default): the plain method on a 1920x1080 uint8 synthetic video with
+per-frame random affine motion, 4 pyramid scales, sigma=1 / 7-tap FP32 blur,
+FAST threshold 20, 7x7 LK windows with <= 5 iterations.  One step = one
+tracker frame step (pyramid, blur, matcher on every scale, detector mask,
+saliency, box selection, insertion, compaction, merge) on one frame.  All
+frames are generated on the host and uploaded before timing, so the timed
+region starts with its inputs resident in HBM.
+
+value = particle-updates/s: particles (age > 0) advanced through the matcher
+at their own scale (each one includes its coarser ancestor solves), summed
+over scales and timed frames, divided by the timed wall time.
+
+Printed: one JSON line (rank 0).  Extra keys carry frames/s, per-kernel
+timings, the Gaussian / pyramid roofline microbench and the CPU baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "particle-updates/s & frames/s @1080p 4-lvl 100k pts; % HBM roofline"
+HBM_PEAK_GBS = 8000.0
+SEED = 0xC0FFEE02
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def lk_algorithmic_bytes(level_shapes, s, nparts):
+    """Compulsory bytes of one matcher launch at scale s: the previous and
+    current FP32 frames of scales s..L-1 once each, plus each particle read and
+    written (20 B each way)."""
+    img = sum(2 * 4 * r * c for (r, c) in level_shapes[s:])
+    return img + 40 * nparts
+
+
+def tracker_bench(cu, ctx, args):
+    nr, nc, L = 1080, 1920, 4
+    nframes = args.warmup + args.steps
+    t0 = time.time()
+    video = cu.synth_video(SEED, nr, nc, nframes, 1)
+    log(f"generated {nframes} frames in {time.time() - t0:.1f}s")
+    frames = []
+    for f in range(nframes):
+        im = ctx.image(nr, nc, cu.U8)
+        im.upload(video[f])
+        frames.append(im)
+    params = cu.profile_params("B", L, 20.0)
+    trk = cu.Tracker(ctx, nr, nc, params)
+    shapes = [(nr >> s, nc >> s) for s in range(L)]
+
+    for f in range(args.warmup):
+        trk.run(frames[f])
+    ctx.sync()
+    live0 = [trk.work(s)[0] for s in range(L)]
+    ctx.profile(args.kernel_events)
+    ctx.profile_reset()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for f in range(args.warmup, nframes):
+        trk.run(frames[f])
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    live1 = [trk.work(s)[0] for s in range(L)]
+    updates = sum(live1) - sum(live0)
+    per_scale = [b - a for a, b in zip(live0, live1)]
+    nparts = [trk.nparticles(s) for s in range(L)]
+
+    kernels = {}
+    for name in ("lk_track", "gaussian_sep", "pyramid_build", "fast_saliency", "fill_u32", "fill_u8",
+                 "particle_mask", "nms_boxes", "subrow_count", "compact_count", "compact_scan",
+                 "compact_scatter", "insert_new", "merge_init", "merge_resolve", "merge_apply"):
+        n, ms = ctx.profile_query(name)
+        if n:
+            kernels[name] = {"launches": n, "total_ms": round(ms, 4), "avg_us": round(ms / n * 1000, 3)}
+    ctx.profile(False)
+    res = dict(seconds=dt, frames=args.steps, updates=updates, per_scale=per_scale, nparts=nparts,
+               kernels=kernels, shapes=shapes)
+    trk.close()
+    for im in frames:
+        im.free()
+    return res, video
+
+
+def gaussian_microbench(cu, ctx, reps=20):
+    """Separable Gaussian on f32 4096x4096 and f32x4 3840x2160 (sigma 1, 2, 4;
+    2*ceil(3 sigma)+1 taps), 4 distinct buffer pairs rotated so the working set
+    (>= 512 MiB) exceeds the 256 MB Infinity Cache; and the 5-level 2x pyramid
+    of the 4096x4096 image."""
+    rng = np.random.default_rng(3)
+    out = []
+    for kind, shape in (("f32", (4096, 4096)), ("f32x4", (2160, 3840))):
+        if kind == "f32":
+            img = rng.uniform(0, 255, shape).astype(np.float32)
+            dt, es = cu.F32, 4
+        else:
+            img = rng.uniform(0, 255, shape + (4,)).astype(np.float32)
+            dt, es = cu.F32X4, 16
+        ins = [ctx.upload(img) for _ in range(4)]
+        outs = [ctx.image(shape[0], shape[1], dt) for _ in range(4)]
+        nbytes = shape[0] * shape[1] * es * 2
+        for sigma in (1, 2, 4):
+            K = 3 * sigma
+            for i in range(4):
+                ctx.gaussian(ins[i], outs[i], sigma, K)
+            ctx.sync()
+            ctx.profile(True)
+            ctx.profile_reset()
+            for i in range(reps):
+                ctx.gaussian(ins[i % 4], outs[i % 4], sigma, K)
+            n, ms = ctx.profile_query("gaussian_sep")
+            ctx.profile(False)
+            us = ms / n * 1000
+            gbs = nbytes / us / 1e3
+            out.append({"kernel": "gaussian_sep", "image": f"{kind} {shape[1]}x{shape[0]}", "sigma": sigma,
+                        "taps": 2 * K + 1, "avg_us": round(us, 2), "bytes": nbytes, "GBps": round(gbs, 1),
+                        "frac": round(gbs / HBM_PEAK_GBS, 4)})
+        for im in ins + outs:
+            im.free()
+    # 5-level pyramid, in place on level 0
+    img = rng.uniform(0, 255, (4096, 4096)).astype(np.float32)
+    pyrs = []
+    for _ in range(4):
+        lv = [ctx.upload(img)]
+        for l in range(1, 5):
+            r, c = lv[-1].shape
+            lv.append(ctx.image(r // 2, c // 2, cu.F32))
+        pyrs.append(lv)
+    nbytes = 4 * 4096 * 4096 * (1 + 1 / 4 + 1 / 16 + 1 / 64 + 1 / 256)
+    for i in range(4):
+        ctx.pyramid(pyrs[i][0], pyrs[i])
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    for i in range(reps):
+        ctx.pyramid(pyrs[i % 4][0], pyrs[i % 4])
+    n, ms = ctx.profile_query("pyramid_build")
+    ctx.profile(False)
+    us = ms / n * 1000
+    gbs = nbytes / us / 1e3
+    out.append({"kernel": "pyramid_build", "image": "f32 4096x4096, 5 levels", "avg_us": round(us, 2),
+                "bytes": int(nbytes), "GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
+    for lv in pyrs:
+        for im in lv:
+            im.free()
+    return out
+
+
+def cpu_baseline(video, budget_s):
+    """The CPU oracle (a restatement of the reference algorithms, OpenMP over
+    particles and rows like the reference) on the first frames of the same
+    video, until about `budget_s` seconds of CPU work."""
+    import oracle
+    orc = oracle.Oracle()
+    nr, nc = video.shape[1:]
+    t = orc.tracker(nr, nc, oracle.profile_params("B", 4, 20.0))
+    t.run(video[0])
+    frames, updates, spent = 0, 0, 0.0
+    while spent < budget_s and frames + 1 < len(video):
+        live = sum(int((t.particles(s)["age"] > 0).sum()) for s in range(4))
+        t0 = time.perf_counter()
+        t.run(video[frames + 1])
+        spent += time.perf_counter() - t0
+        frames += 1
+        updates += live
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": updates / spent, "unit": "particle-updates/s", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle, frames 1..{frames} of the same 1920x1080 video ({spent:.1f} s, "
+                      f"{updates} particle updates, {frames / spent:.2f} frames/s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=270)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-micro", action="store_true")
+    ap.add_argument("--kernel-events", type=int, default=1,
+                    help="record HIP events around every kernel in the timed region")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import cuimg_amd as cu
+    ctx = cu.Context(0)
+    log("tracker benchmark")
+    tr, video = tracker_bench(cu, ctx, args)
+    micro = [] if args.no_micro else gaussian_microbench(cu, ctx)
+    ctx.close()
+
+    value = tr["updates"] / tr["seconds"]
+    ms_per_step = tr["seconds"] / tr["frames"] * 1000
+    k = tr["kernels"]
+    # dominant kernel of the frame step
+    dom = max(k.items(), key=lambda kv: kv[1]["total_ms"])[0] if k else None
+    roofline = None
+    if dom == "lk_track":
+        L = len(tr["shapes"])
+        # algorithmic bytes of all matcher launches of the timed frames / their total time
+        avg_parts = [u / tr["frames"] for u in tr["per_scale"]]
+        per_frame = sum(lk_algorithmic_bytes(tr["shapes"], s, avg_parts[s]) for s in range(L))
+        bytes_per_launch = per_frame / L
+        achieved = bytes_per_launch / (k[dom]["avg_us"] * 1e-6) / 1e9
+        roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                    "note": "matcher is gather/latency bound; bytes = both FP32 frames of scales >= s "
+                            "+ 40 B per particle, averaged over the 4 per-frame launches"}
+    elif dom is not None:
+        roofline = {"kernel": dom, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": None, "traffic": None}
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline")
+        cpu = cpu_baseline(video, args.cpu_budget)
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "plain pyramid-LK tracker, 1920x1080 u8 affine-motion video, 4 scales, "
+                               "sigma=1 7-tap FP32 blur, FAST th 20, 7x7 LK <=5 it",
+                   "frames": args.warmup + args.steps, "parallelism": "1 GPU"},
+        "frames_per_s": round(tr["frames"] / tr["seconds"], 2),
+        "particles_per_scale": tr["nparts"],
+        "updates_per_frame": round(tr["updates"] / tr["frames"], 1),
+        "roofline": roofline,
+        "kernels": k,
+        "gaussian_pyramid_roofline": micro,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

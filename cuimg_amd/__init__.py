@@ -120,6 +120,10 @@ def lib():
         L.cuimg_hip_tracker_get_decisions.argtypes = [vp, c_int, vp, vp, vp, ctypes.c_uint32, P(ctypes.c_uint32)]
         L.cuimg_hip_tracker_matcher_images.argtypes = [vp, c_int, P(Img), P(Img), P(Img)]
         L.cuimg_hip_tracker_frame_count.argtypes = [vp, P(c_int)]
+        L.cuimg_hip_particles_merge.argtypes = [vp, vp, ctypes.c_uint32, P(Img)]
+        L.cuimg_hip_tracker_work.argtypes = [vp, c_int, P(ctypes.c_uint64), P(ctypes.c_uint64)]
+        L.cuimg_hip_lk_loss_adjoint.argtypes = [vp, P(Img), P(Img), vp, vp, ctypes.c_uint32, c_int, c_int, c_int,
+                                                vp, vp, vp, P(Img), vp, ctypes.c_size_t]
         _lib = L
     return _lib
 
@@ -411,8 +415,95 @@ class Tracker:
                            "get_decisions")
         return acc, tr, dist
 
+    def work(self, s):
+        """(particles run through the matcher, accepted) at scale s since creation."""
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        self.ctx.check(self.L.cuimg_hip_tracker_work(self.h, s, ctypes.byref(a), ctypes.byref(b)), "tracker_work")
+        return a.value, b.value
+
     @property
     def frame_count(self):
         v = ctypes.c_int()
         self.ctx.check(self.L.cuimg_hip_tracker_frame_count(self.h, ctypes.byref(v)), "frame_count")
         return v.value
+
+
+def particles_merge(ctx, parts, sparse):
+    """merge_trajectories over a container state on the device: returns the
+    particle vector after the merge.  `sparse` is the (nrows, ncols) index image
+    (-1 = empty)."""
+    L = ctx.L
+    parts = np.ascontiguousarray(parts, PARTICLE_DTYPE)
+    sp = np.ascontiguousarray(sparse).astype(np.int32).view(np.uint32)
+    img = ctx.image(sp.shape[0], sp.shape[1], U32)
+    img.upload(sp)
+    dev = ctypes.c_void_p()
+    nbytes = max(parts.nbytes, 1)
+    ctx.check(L.cuimg_hip_malloc(ctx.h, nbytes, ctypes.byref(dev)), "malloc")
+    try:
+        if len(parts):
+            ctx.check(L.cuimg_hip_memcpy_h2d(ctx.h, dev, parts.ctypes.data, parts.nbytes), "h2d")
+        ctx.check(L.cuimg_hip_particles_merge(ctx.h, dev, len(parts), ctypes.byref(img.d)), "particles_merge")
+        out = np.zeros_like(parts)
+        if len(parts):
+            ctx.check(L.cuimg_hip_memcpy_d2h(ctx.h, out.ctypes.data, dev, parts.nbytes), "d2h")
+    finally:
+        L.cuimg_hip_free(ctx.h, dev)
+    return out
+
+
+class DeviceBuffer:
+    """Plain device allocation (cuimg_hip_malloc)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.p = ctypes.c_void_p()
+        self.nbytes = max(int(nbytes), 1)
+        ctx.check(ctx.L.cuimg_hip_malloc(ctx.h, self.nbytes, ctypes.byref(self.p)), "malloc")
+
+    @classmethod
+    def from_array(cls, ctx, arr):
+        arr = np.ascontiguousarray(arr)
+        b = cls(ctx, arr.nbytes)
+        if arr.nbytes:
+            ctx.check(ctx.L.cuimg_hip_memcpy_h2d(ctx.h, b.p, arr.ctypes.data, arr.nbytes), "h2d")
+        return b
+
+    def download(self, dtype, shape):
+        out = np.zeros(shape, dtype)
+        if out.nbytes:
+            self.ctx.check(self.ctx.L.cuimg_hip_memcpy_d2h(self.ctx.h, out.ctypes.data, self.p, out.nbytes), "d2h")
+        return out
+
+    def free(self):
+        if self.p and self.ctx.h:
+            self.ctx.L.cuimg_hip_free(self.ctx.h, self.p)
+            self.p = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def lk_loss_adjoint(ctx, I0, I1, pos, d, sigma=1, K=3, ws=7, want_dI=True):
+    """Photometric LK loss and gradients on the device.  I0, I1: DeviceImage
+    (F32); pos, d: (n, 2) float32 host arrays.  Returns (L, loss_p, dL_dd,
+    dL_dI1 or None) as host arrays."""
+    L = ctx.L
+    n = len(pos)
+    bp = DeviceBuffer.from_array(ctx, np.asarray(pos, np.float32))
+    bd = DeviceBuffer.from_array(ctx, np.asarray(d, np.float32))
+    tot = DeviceBuffer(ctx, 4)
+    lp = DeviceBuffer(ctx, 4 * n)
+    gd = DeviceBuffer(ctx, 8 * n)
+    nr, nc = I1.shape
+    scratch = DeviceBuffer(ctx, 8 * nr * nc)
+    gI = ctx.image(nr, nc, F32) if want_dI else None
+    ctx.check(L.cuimg_hip_lk_loss_adjoint(ctx.h, ctypes.byref(I0.d), ctypes.byref(I1.d), bp.p, bd.p, n, sigma, K,
+                                          ws, tot.p, lp.p, gd.p, ctypes.byref(gI.d) if gI else None, scratch.p,
+                                          2 * nr * nc), "lk_loss_adjoint")
+    ctx.sync()
+    return (float(tot.download(np.float32, (1,))[0]), lp.download(np.float32, (n,)),
+            gd.download(np.float32, (n, 2)), gI.download() if gI else None)

@@ -271,6 +271,7 @@ __device__ float2 prediction(const lk_args& a, float2 pos, int age, float2 speed
 template <typename TI, int NPL, bool SEL>
 __global__ __launch_bounds__(256) void lk_kernel(lk_args a) {
   __shared__ float lds_all[4][2 * 64 * NPL];
+  __shared__ uint32_t blk_live, blk_acc;
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   float* lds = lds_all[wave];
@@ -278,6 +279,11 @@ __global__ __launch_bounds__(256) void lk_kernel(lk_args a) {
   const int s = a.s;
   const lk_level_view& lv = a.lv[s];
   if (blockIdx.x == 0 && threadIdx.x == 0) a.cnt->n_matched = n;
+  if (threadIdx.x == 0) {
+    blk_live = 0;
+    blk_acc = 0;
+  }
+  __syncthreads();
   for (uint32_t i = blockIdx.x * 4 + wave; i < n; i += gridDim.x * 4) {
     cuimg_particle_f part = a.parts[i];
     int accept = 0;
@@ -300,6 +306,8 @@ __global__ __launch_bounds__(256) void lk_kernel(lk_args a) {
       }
     }
     if (lane == 0) {
+      if (part.age > 0) atomicAdd(&blk_live, 1u);
+      if (accept) atomicAdd(&blk_acc, 1u);
       a.dec_accept[i] = accept;
       a.dec_tr[i] = tr;
       a.dec_dist[i] = dist;
@@ -317,6 +325,11 @@ __global__ __launch_bounds__(256) void lk_kernel(lk_args a) {
       }
       a.parts[i] = part;
     }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && (blk_live | blk_acc)) {
+    atomicAdd(&a.cnt->live_total, (unsigned long long)blk_live);
+    atomicAdd(&a.cnt->accepted_total, (unsigned long long)blk_acc);
   }
 }
 

@@ -364,7 +364,7 @@ __global__ __launch_bounds__(256) void merge_init_kernel(merge_args a) {
     uint32_t mask = 0;
     for (int k = 0; k < 8; k++) {
       const uint32_t b = sparse_get(a.sparse, a.sparse_pitch, a.nr, a.nc, r + kC8R[k], c + kC8C[k]);
-      if (b == kEmpty) continue;
+      if (b == kEmpty || b >= n) continue;  // indices past the vector are treated as empty
       const cuimg_particle_f q = a.parts[b];
       if ((int)q.age > (int)p.age + 2 &&
           norml2_rn(__fsub_rn(p.speed_r, q.speed_r), __fsub_rn(p.speed_c, q.speed_c)) < 2.f)

@@ -439,6 +439,51 @@ int cuimg_hip_tracker_matcher_images(cuimg_hip_tracker* t, int scale, cuimg_img*
   return 0;
 }
 
+int cuimg_hip_particles_merge(cuimg_hip_ctx* ctx, cuimg_particle_f* parts, uint32_t n, const cuimg_img* sparse) {
+  if (!ctx || (!parts && n) || !sparse || sparse->dtype != CUIMG_U32)
+    return set_error(ctx, CUIMG_E_INVALID, "particles_merge: bad arguments");
+  const uint32_t cap = std::max<uint32_t>(n, 1);
+  void *buf = nullptr, *cnt = nullptr;
+  const size_t bytes = (size_t)cap * (4 + 4 + 4 + 1);
+  CUIMG_HIP(ctx, hipMalloc(&buf, bytes));
+  if (hipMalloc(&cnt, sizeof(level_counters)) != hipSuccess) {
+    hipFree(buf);
+    return set_error(ctx, CUIMG_E_NOMEM, "particles_merge: allocation failed");
+  }
+  level_counters c{};
+  c.n = n;
+  int rc = 0;
+  if (hipMemcpyAsync(cnt, &c, sizeof c, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) rc = CUIMG_E_NOMEM;
+  if (!rc) {
+    merge_args m{};
+    m.nr = sparse->nrows;
+    m.nc = sparse->ncols;
+    m.parts = parts;
+    m.sparse = static_cast<const uint32_t*>(sparse->begin);
+    m.sparse_pitch = sparse->pitch;
+    m.cnt = static_cast<level_counters*>(cnt);
+    m.actors = static_cast<uint32_t*>(buf);
+    m.zeroer = reinterpret_cast<int32_t*>(m.actors + cap);
+    m.minunres = m.zeroer + cap;
+    m.resolved = reinterpret_cast<uint8_t*>(m.minunres + cap);
+    m.cap = cap;
+    rc = launch_merge(ctx, m);
+  }
+  hipStreamSynchronize(ctx->stream);
+  hipFree(buf);
+  hipFree(cnt);
+  return rc;
+}
+
+int cuimg_hip_tracker_work(cuimg_hip_tracker* t, int scale, uint64_t* live, uint64_t* accepted) {
+  CUIMG_TRY(check_scale(t, scale));
+  level_counters c;
+  CUIMG_TRY(read_counters(t, scale, &c));
+  if (live) *live = c.live_total;
+  if (accepted) *accepted = c.accepted_total;
+  return 0;
+}
+
 int cuimg_hip_tracker_frame_count(cuimg_hip_tracker* t, int* out) {
   if (!t || !out) return CUIMG_E_INVALID;
   *out = t->frame_cpt;

@@ -19,6 +19,8 @@ struct level_counters {
   uint32_t actors;   // merge: particles with an eligible neighbour
   uint32_t n_matched; // size of the dense vector seen by the last matcher launch
   uint32_t pad;
+  unsigned long long live_total;      // particles (age > 0) run through the matcher
+  unsigned long long accepted_total;  // of which accepted (moved)
 };
 
 // One pyramid level as the matcher sees it: A = previous frame, B = current.

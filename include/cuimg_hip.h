@@ -196,6 +196,34 @@ int cuimg_hip_tracker_get_decisions(cuimg_hip_tracker* t, int scale, int32_t* ac
 int cuimg_hip_tracker_matcher_images(cuimg_hip_tracker* t, int scale, cuimg_img* A, cuimg_img* B,
                                      cuimg_img* G);
 int cuimg_hip_tracker_frame_count(cuimg_hip_tracker* t, int* out);
+/* Totals since creation (or clear) at scale s: particles with age > 0 run
+ * through the matcher, and how many of them were accepted.  Synchronises. */
+int cuimg_hip_tracker_work(cuimg_hip_tracker* t, int scale, uint64_t* live, uint64_t* accepted);
+
+/* merge_trajectories_kernel over a dense vector (tracking2/merge_trajectories.h:10-35,
+ * launched at pyrlk_tracker.hpp:865-870), with the result of the serial
+ * ascending-index loop: `parts` (device, n entries) is updated in place,
+ * `sparse` is the U32 index image of the domain (0xFFFFFFFF = empty).
+ * Synchronises the context. */
+int cuimg_hip_particles_merge(cuimg_hip_ctx* ctx, cuimg_particle_f* parts, uint32_t n,
+                              const cuimg_img* sparse);
+
+/* ------------------------------------------------------------- adjoint */
+/* Photometric LK loss of the blurred frames and its gradient (no counterpart in
+ * the reference; it is the term iterative_lk_match_5x5 minimises,
+ * tracking2/pyrlk_tracker.hpp:283-317, written as a loss):
+ *   L = sum_p sum_w ((G*I1)(x_p + d_p + w) - (G*I0)(x_p + w))^2
+ * G = gaussian_sep(sigma, half_width), bilinear samples as the matcher's li
+ * (:130-146), w over a ws x ws window.  Device inputs: pos2, d2 (n float2,
+ * (row, col)).  Device outputs: *loss_total, loss_p[n], dL_dd2[n float2] and,
+ * if dL_dI1 is not NULL, dL/dI1 (F32 image of the frame domain).  Blurred
+ * values are recomputed per particle, never stored.  scratch: device buffer of
+ * at least 2 * nrows * ncols floats.  ws <= 7 (odd), half_width <= 6. */
+int cuimg_hip_lk_loss_adjoint(cuimg_hip_ctx* ctx, const cuimg_img* I0, const cuimg_img* I1,
+                              const float* pos2, const float* d2, uint32_t n, int sigma,
+                              int half_width, int ws, float* loss_total, float* loss_p,
+                              float* dL_dd2, const cuimg_img* dL_dI1, float* scratch,
+                              size_t scratch_floats);
 
 /* -------------------------------------------------------- synthetic input */
 /* Seeded texture video used by the benchmarks (host code, OpenMP).  motion:

@@ -124,6 +124,7 @@ class Oracle:
         L.orc_tracker_get_matcher_images.argtypes = [vp, c_int, vp, vp, vp]
         L.orc_tracker_get_decisions.argtypes = [vp, c_int, P(ctypes.c_int32), P(c_float), P(c_float), c_int]
         L.orc_tracker_lk_decide.argtypes = [vp, c_int, P(ctypes.c_int32), P(c_float), P(c_float), c_int]
+        L.orc_merge.argtypes = [vp, c_int, P(ctypes.c_int32), c_int, c_int]
         L.orc_lk_loss_grad_f64.restype = c_double
         L.orc_lk_loss_grad_f64.argtypes = [P(c_float), P(c_float), c_int, c_int, P(c_float), c_int,
                                            P(c_float), P(c_float), c_int, c_int, vp, vp]
@@ -212,6 +213,13 @@ class Oracle:
                             float(p[0]), float(p[1]), float(pred[0]), float(pred[1]),
                             _ptr(tr, ctypes.c_float), _ptr(dist, ctypes.c_float))
         return tr, float(dist[0])
+
+    def merge(self, parts, sparse):
+        parts = np.array(parts, PARTICLE_DTYPE, copy=True)
+        sparse = np.ascontiguousarray(sparse, np.int32)
+        self.L.orc_merge(parts.ctypes.data, len(parts), _ptr(sparse, ctypes.c_int32), sparse.shape[0],
+                         sparse.shape[1])
+        return parts
 
     # ---------------------------------------------------------------- tracker
     def tracker(self, nr, nc, params):

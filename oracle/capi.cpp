@@ -212,6 +212,13 @@ int orc_tracker_lk_decide(void* t, int s, int32_t* accept, float* tr2, float* di
   return copy_decisions(d, accept, tr2, dist, cap);
 }
 
+void orc_merge(orc_particle* parts, int n, const int32_t* sparse, int nr, int nc) {
+  std::vector<particle> v(n);
+  std::memcpy(v.data(), parts, sizeof(particle) * n);
+  merge_state(v, wrap(sparse, nr, nc));
+  std::memcpy(parts, v.data(), sizeof(particle) * n);
+}
+
 double orc_lk_loss_grad_f64_d(const double* I0, const double* I1, int nr, int nc,
                               const double* coef, int K, const double* pos2, const double* d2,
                               int np, int ws, double* dL_dd2, double* dL_dI1) {

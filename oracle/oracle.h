@@ -82,6 +82,9 @@ int orc_tracker_get_decisions(void* t, int s, int32_t* accept, float* tr2, float
  * changing it: lets a test feed identical particles to the GPU. */
 int orc_tracker_lk_decide(void* t, int s, int32_t* accept, float* tr2, float* dist, int cap);
 
+/* merge_trajectories over a container state (in place); sparse: nr*nc, -1 empty. */
+void orc_merge(orc_particle* parts, int n, const int32_t* sparse, int nr, int nc);
+
 /* FP64 photometric LK loss and its gradient (blurred frames, bilinear samples). */
 double orc_lk_loss_grad_f64(const float* I0, const float* I1, int nr, int nc, const float* coef,
                             int K, const float* pos2, const float* d2, int np, int ws,

@@ -60,7 +60,9 @@ struct tracker {
   void detector_mask(level_state& L);
   void new_particles(level_state& L);
   void compact(level_state& L);
-  void merge(level_state& L);
+  static void merge(level_state& L);
 };
+
+void merge_state(std::vector<particle>& parts, const image<int32_t>& sparse);
 
 }  // namespace orc

@@ -257,6 +257,17 @@ void tracker::merge(level_state& L) {
   }
 }
 
+// Standalone merge on a given container state (tests).
+void merge_state(std::vector<particle>& parts, const image<int32_t>& sparse) {
+  level_state L;
+  L.parts.swap(parts);
+  L.sparse = sparse;
+  L.nr = sparse.nr;
+  L.nc = sparse.nc;
+  tracker::merge(L);
+  parts.swap(L.parts);
+}
+
 // pyrlk_cpu::update (tracking2/pyrlk_tracker.hpp:831-875).  The strategy's
 // copy of its input is an exact copy (the reference's equal-border memcpy skips
 // the last `border` rows, copy.h:35, which leaves them uninitialised).

@@ -1,0 +1,198 @@
+"""Pins the oracle's Lucas-Kanade matcher.
+
+Mathematical checks: zero motion gives exactly zero, integer and sub-pixel
+translations of smooth textures are recovered, degenerate (flat / 1-D) patches
+fail with the reference's failure code, leaving the domain fails, the Sobel/8
+gradient matches its closed form.  An independent FP32 restatement in numpy
+(scalar float32 arithmetic in the same operation order) checks the window
+bookkeeping (selection with ties, the error's index and counter) bit-exactly.
+"""
+import numpy as np
+import pytest
+
+f32 = np.float32
+
+
+def smooth_texture(rng, shape, sigma=2.0):
+    from scipy.ndimage import gaussian_filter
+    t = gaussian_filter(rng.standard_normal(shape), sigma)
+    t = (t - t.min()) / (t.max() - t.min()) * 255.0
+    return t.astype(np.float32)
+
+
+def test_sobel_closed_form(orc):
+    rng = np.random.default_rng(0)
+    P = rng.integers(0, 256, (20, 30)).astype(np.float32)
+    G = orc.sobel(P, True)
+    Pi = P.astype(np.int64)
+    gr = (Pi[2:, :-2] + 2 * Pi[2:, 1:-1] + Pi[2:, 2:] - Pi[:-2, :-2] - 2 * Pi[:-2, 1:-1] - Pi[:-2, 2:]) / 8
+    gc = (Pi[:-2, 2:] + 2 * Pi[1:-1, 2:] + Pi[2:, 2:] - Pi[:-2, :-2] - 2 * Pi[1:-1, :-2] - Pi[2:, :-2]) / 8
+    assert np.array_equal(G[1:-1, 1:-1, 0], gr.astype(np.float32))
+    assert np.array_equal(G[1:-1, 1:-1, 1], gc.astype(np.float32))
+    assert (G[0] == 0).all() and (G[-1] == 0).all() and (G[:, 0] == 0).all() and (G[:, -1] == 0).all()
+
+
+@pytest.mark.parametrize("profile", ["R", "B"])
+def test_zero_motion_is_exact(orc, profile):
+    rng = np.random.default_rng(1)
+    A = rng.integers(0, 256, (40, 50)).astype(np.float32)
+    G = orc.sobel(A, True)
+    kw = dict(ws=11, max_iters=30, top_n=25, template_u8=1) if profile == "R" else dict(ws=7, max_iters=5, top_n=49)
+    ws = kw["ws"]
+    hws, n = ws // 2, ws * ws
+    nsel = min(kw["top_n"], n)
+    for p in [(20, 25), (10, 12)]:
+        tr, dist = orc.lk_match(A, A, G, (float(p[0]), float(p[1])), (0.0, 0.0), **kw)
+        assert tr[0] == 0.0 and tr[1] == 0.0
+        # the error pairs window pixel (r,c) with template index (r+hws)*hws + (c+hws)
+        # (tracking2/pyrlk_tracker.hpp:325): non-zero even without motion
+        err = 0.0
+        for r in range(-hws, hws + 1):
+            for c in range(-hws, hws + 1):
+                idx = (r + hws) * hws + (c + hws)
+                rr, cc = idx // ws - hws, idx % ws - hws
+                err += abs(float(A[p[0] + rr, p[1] + cc]) - float(A[p[0] + r, p[1] + c]))
+        assert abs(dist - err / (nsel + n)) <= 1e-4 * max(1.0, dist)
+
+
+@pytest.mark.parametrize("shift", [(2, -1), (-1, 1), (0, 2)])
+def test_integer_translation_recovered(orc, shift):
+    rng = np.random.default_rng(2)
+    A = smooth_texture(rng, (60, 60))
+    B = np.roll(A, shift, axis=(0, 1))
+    G = orc.sobel(A, False)
+    tr, _ = orc.lk_match(A, B, G, (30.0, 30.0), (0.0, 0.0), ws=11, max_iters=50, eps=1e-4, top_n=121)
+    assert np.abs(tr - np.array(shift, np.float32)).max() < 1e-2, tr
+
+
+@pytest.mark.parametrize("shift", [(0.4, -0.3), (1.25, 0.75)])
+def test_subpixel_translation_recovered(orc, shift):
+    rng = np.random.default_rng(3)
+    A = smooth_texture(rng, (80, 80), 3.0)
+    from scipy.ndimage import map_coordinates
+    rr, cc = np.mgrid[0:80, 0:80].astype(np.float64)
+    B = map_coordinates(A.astype(np.float64), [rr - shift[0], cc - shift[1]], order=1, mode="nearest").astype(np.float32)
+    G = orc.sobel(A, False)
+    tr, _ = orc.lk_match(A, B, G, (40.0, 40.0), (0.0, 0.0), ws=11, max_iters=50, eps=1e-4, top_n=121)
+    assert np.abs(tr - np.array(shift, np.float32)).max() < 0.05, tr
+
+
+def test_degenerate_patches_fail(orc):
+    flat = np.full((40, 40), 77, np.float32)
+    tr, dist = orc.lk_match(flat, flat, orc.sobel(flat, True), (20.0, 20.0), (0.0, 0.0))
+    assert tuple(tr) == (-1.0, -1.0) and dist == 1000.0
+    stripes = np.tile((np.arange(40) * 7 % 256).astype(np.float32), (40, 1))  # varies along columns only
+    tr, dist = orc.lk_match(stripes, stripes, orc.sobel(stripes, True), (20.0, 20.0), (0.0, 0.0))
+    assert tuple(tr) == (-1.0, -1.0) and dist == 1000.0
+
+
+def test_leaving_domain_fails(orc):
+    rng = np.random.default_rng(4)
+    A = smooth_texture(rng, (40, 40))
+    B = np.roll(A, (-6, 0), axis=(0, 1))  # content moved up past the border band
+    G = orc.sobel(A, False)
+    tr, dist = orc.lk_match(A, B, G, (5.0, 20.0), (0.0, 0.0), ws=7, max_iters=20)
+    assert tuple(tr) == (-1.0, -1.0) and dist == 1000.0
+
+
+# ----------------------------------------------------------- numpy restatement
+def np_li(img, qr, qc):
+    nr, nc = img.shape
+    xr, xc = int(qr), int(qc)
+    a0, a1 = f32(f32(qr) - f32(xr)), f32(f32(qc) - f32(xc))
+    g = lambda r, c: img[min(max(r, 0), nr - 1), min(max(c, 0), nc - 1)]
+    b0, b1 = f32(f32(1) - a0), f32(f32(1) - a1)
+    s = f32(f32(b0 * b1) * g(xr, xc))
+    s = f32(s + f32(f32(a0 * b1) * g(xr + 1, xc)))
+    s = f32(s + f32(f32(b0 * a1) * g(xr, xc + 1)))
+    s = f32(s + f32(f32(a0 * a1) * g(xr + 1, xc + 1)))
+    return s
+
+
+def np_match(A, B, G, p, pred, ws, max_iters, eps, top_n, template_u8):
+    nr, nc = A.shape
+    hws, n = ws // 2, ws * ws
+    p = (f32(p[0]), f32(p[1]))
+    prediction = (f32(p[0] + f32(pred[0])), f32(p[1] + f32(pred[1])))
+    v = list(prediction)
+    gs, as_, valid = [], [], []
+    for r in range(-hws, hws + 1):
+        for c in range(-hws, hws + 1):
+            qr, qc = f32(p[0] + f32(r)), f32(p[1] + f32(c))
+            if 0 <= int(qr) < nr and 0 <= int(qc) < nc:
+                g = (np_li(G[:, :, 0], qr, qc), np_li(G[:, :, 1], qr, qc))
+                a = np_li(A, qr, qc)
+                if template_u8:
+                    a = f32(np.uint8(int(a) & 0xFF))
+                gs.append(g), as_.append(a), valid.append(True)
+            else:
+                gs.append((f32(0), f32(0))), as_.append(f32(0)), valid.append(False)
+    mag = [np.sqrt(f32(f32(g[0] * g[0]) + f32(g[1] * g[1]))) for g in gs]
+    order = sorted([i for i in range(n) if valid[i]], key=lambda i: (-mag[i], i))
+    sel = set(order[:top_n])
+    G00 = G01 = G11 = f32(0)
+    for i in range(n):
+        if i in sel:
+            gx, gy = gs[i]
+            G00 = f32(G00 + f32(gx * gx))
+            G01 = f32(G01 + f32(gx * gy))
+            G11 = f32(G11 + f32(gy * gy))
+    h, q = f32(f32(0.5) * f32(G00 + G11)), f32(f32(0.5) * f32(G00 - G11))
+    rr = np.sqrt(f32(f32(q * q) + f32(G01 * G01)))
+    if min(abs(f32(h + rr)), abs(f32(h - rr))) < f32(1e-4):
+        return np.array([-1, -1], np.float32), f32(1000)
+    det = f32(f32(G00 * G11) - f32(G01 * G01))
+    inv = f32(f32(1) / det)
+    I00, I01, I11 = f32(G11 * inv), f32(-G01 * inv), f32(G00 * inv)
+    nk = (f32(1), f32(1))
+    k, cpt = 0, len(sel)
+    while k < max_iters and np.sqrt(f32(f32(nk[0] * nk[0]) + f32(nk[1] * nk[1]))) >= f32(eps):
+        b0 = b1 = f32(0)
+        cpt = 0
+        i = 0
+        for r in range(-hws, hws + 1):
+            for c in range(-hws, hws + 1):
+                if i in sel:
+                    dt = f32(as_[i] - np_li(B, f32(v[0] + f32(r)), f32(v[1] + f32(c))))
+                    b0 = f32(b0 + f32(gs[i][0] * dt))
+                    b1 = f32(b1 + f32(gs[i][1] * dt))
+                    cpt += 1
+                i += 1
+        nk = (f32(f32(I00 * b0) + f32(I01 * b1)), f32(f32(I01 * b0) + f32(I11 * b1)))
+        v = [f32(v[0] + nk[0]), f32(v[1] + nk[1])]
+        d0, d1 = f32(prediction[0] - v[0]), f32(prediction[1] - v[1])
+        if np.sqrt(f32(f32(d0 * d0) + f32(d1 * d1))) > ws * 4:
+            return np.array([-1, -1], np.float32), f32(1000)
+        if not (3 <= int(v[0]) <= nr - 4 and 3 <= int(v[1]) <= nc - 4):
+            return np.array([-1, -1], np.float32), f32(1000)
+        k += 1
+    err = f32(0)
+    for r in range(-hws, hws + 1):
+        for c in range(-hws, hws + 1):
+            idx = (r + hws) * hws + (c + hws)
+            err = f32(err + abs(f32(as_[idx] - np_li(B, f32(v[0] + f32(r)), f32(v[1] + f32(c))))))
+            cpt += 1
+    return np.array([f32(v[0] - p[0]), f32(v[1] - p[1])], np.float32), f32(err / f32(cpt))
+
+
+@pytest.mark.parametrize("profile", ["R", "B"])
+def test_matches_independent_restatement(orc, profile):
+    rng = np.random.default_rng(10 if profile == "R" else 11)
+    kw = (dict(ws=11, max_iters=30, eps=0.1, top_n=25, template_u8=1) if profile == "R"
+          else dict(ws=7, max_iters=5, eps=0.1, top_n=49, template_u8=0))
+    for trial in range(12):
+        if profile == "R":
+            A = rng.integers(0, 256, (36, 40)).astype(np.float32)
+            A[::3, ::2] = 128  # plateaus: equal gradients, exercising the tie rule
+            G = orc.sobel(A, True)
+            B = np.clip(np.roll(A, (1, -1), axis=(0, 1)) + rng.integers(-3, 4, A.shape), 0, 255).astype(np.float32)
+        else:
+            A = smooth_texture(rng, (36, 40))
+            G = orc.sobel(A, False)
+            B = np.roll(A, (1, 0), axis=(0, 1)) + rng.normal(0, 1, A.shape).astype(np.float32)
+        p = (float(rng.uniform(2, 33)), float(rng.uniform(2, 37)))
+        pred = (float(rng.uniform(-1.5, 1.5)), float(rng.uniform(-1.5, 1.5)))
+        tr, dist = orc.lk_match(A, B, G, p, pred, **kw)
+        tr2, dist2 = np_match(A, B, G, p, pred, **kw)
+        assert np.array_equal(tr, tr2), (trial, tr, tr2)
+        assert np.float32(dist) == dist2, (trial, dist, dist2)
