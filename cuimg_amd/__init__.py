@@ -495,7 +495,7 @@ def lk_loss_adjoint(ctx, I0, I1, pos, d, sigma=1, K=3, ws=7, want_dI=True):
     n = len(pos)
     bp = DeviceBuffer.from_array(ctx, np.asarray(pos, np.float32))
     bd = DeviceBuffer.from_array(ctx, np.asarray(d, np.float32))
-    tot = DeviceBuffer(ctx, 4)
+    tot = DeviceBuffer(ctx, 8)
     lp = DeviceBuffer(ctx, 4 * n)
     gd = DeviceBuffer(ctx, 8 * n)
     nr, nc = I1.shape
@@ -505,5 +505,5 @@ def lk_loss_adjoint(ctx, I0, I1, pos, d, sigma=1, K=3, ws=7, want_dI=True):
                                           ws, tot.p, lp.p, gd.p, ctypes.byref(gI.d) if gI else None, scratch.p,
                                           2 * nr * nc), "lk_loss_adjoint")
     ctx.sync()
-    return (float(tot.download(np.float32, (1,))[0]), lp.download(np.float32, (n,)),
+    return (float(tot.download(np.float64, (1,))[0]), lp.download(np.float32, (n,)),
             gd.download(np.float32, (n, 2)), gI.download() if gI else None)

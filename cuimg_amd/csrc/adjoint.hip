@@ -42,7 +42,7 @@ struct adj_args {
   float* S;            // nr*nc dense accumulator (zeroed by the host code)
   float* loss_p;       // per particle
   float2* dL_dd;
-  float* loss_total;
+  double* loss_total;   // FP64 accumulation of the per-particle sums
 };
 
 __device__ __forceinline__ float ldI(const float* I, int pitch, int nr, int nc, int r, int c) {
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void lk_adjoint_kernel(adj_args a) {
       if (lane == 0) {
         a.loss_p[p] = lp;
         a.dL_dd[p] = make_float2(gr, gc);
-        atomicAdd(a.loss_total, lp);
+        atomicAdd(a.loss_total, (double)lp);
       }
       continue;
     }
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void lk_adjoint_kernel(adj_args a) {
     if (lane == 0) {
       a.loss_p[p] = lp;
       a.dL_dd[p] = make_float2(gr, gc);
-      atomicAdd(a.loss_total, lp);
+      atomicAdd(a.loss_total, (double)lp);
     }
     // splat 2r with the bilinear weights onto the blurred pixels
     const float w00 = (1 - a1r) * (1 - a1c), w10 = a1r * (1 - a1c), w01 = (1 - a1r) * a1c, w11 = a1r * a1c;
@@ -239,7 +239,7 @@ __global__ void zero_kernel(float* p, size_t n) {
 }
 
 int lk_loss_adjoint(cuimg_hip_ctx* ctx, const cuimg_img* I0, const cuimg_img* I1, const float* pos2,
-                    const float* d2, uint32_t n, int sigma, int K, int ws, float* loss_total, float* loss_p,
+                    const float* d2, uint32_t n, int sigma, int K, int ws, double* loss_total, float* loss_p,
                     float* dL_dd2, const cuimg_img* dL_dI1, float* scratch, size_t scratch_floats) {
   if (!I0 || !I1 || I0->dtype != CUIMG_F32 || I1->dtype != CUIMG_F32 || I0->nrows != I1->nrows ||
       I0->ncols != I1->ncols)
@@ -276,7 +276,7 @@ int lk_loss_adjoint(cuimg_hip_ctx* ctx, const cuimg_img* I0, const cuimg_img* I1
   const int zb = (int)std::min<size_t>((npx + 255) / 256, 4096);
   CUIMG_TRY(launch(ctx, "adjoint_zero", [&] {
     zero_kernel<<<zb, 256, 0, ctx->stream>>>(scratch, npx);
-    zero_kernel<<<1, 64, 0, ctx->stream>>>(loss_total, 1);
+    zero_kernel<<<1, 64, 0, ctx->stream>>>(reinterpret_cast<float*>(loss_total), 2);
   }));
   const int blocks = (int)std::min<uint32_t>((n + 3) / 4, 4096);
   if (n) CUIMG_TRY(launch(ctx, "lk_adjoint", [&] { lk_adjoint_kernel<<<blocks, 256, 0, ctx->stream>>>(a); }));
@@ -297,7 +297,7 @@ using namespace cuimg;
 
 extern "C" int cuimg_hip_lk_loss_adjoint(cuimg_hip_ctx* ctx, const cuimg_img* I0, const cuimg_img* I1,
                                          const float* pos2, const float* d2, uint32_t n, int sigma, int half_width,
-                                         int ws, float* loss_total, float* loss_p, float* dL_dd2,
+                                         int ws, double* loss_total, float* loss_p, float* dL_dd2,
                                          const cuimg_img* dL_dI1, float* scratch, size_t scratch_floats) {
   return lk_loss_adjoint(ctx, I0, I1, pos2, d2, n, sigma, half_width, ws, loss_total, loss_p, dL_dd2, dL_dI1,
                          scratch, scratch_floats);

@@ -37,21 +37,21 @@ launch_scope::launch_scope(cuimg_hip_ctx* c, const char* n) : ctx(c), name(n) {
   if (ctx->profiling) {
     a = take_event(ctx);
     b = take_event(ctx);
-    if (a) hipEventRecord(a, ctx->stream);
+    if (a) (void)hipEventRecord(a, ctx->stream);
   }
 }
 
 int launch_scope::done() {
   hipError_t e = hipGetLastError();
   if (ctx->profiling && a && b) {
-    hipEventRecord(b, ctx->stream);
+    (void)hipEventRecord(b, ctx->stream);
     ctx->pending.push_back({name, a, b});
     // harvest periodically so the pending list stays bounded
     if (ctx->pending.size() > 4096) {
-      hipStreamSynchronize(ctx->stream);
+      (void)hipStreamSynchronize(ctx->stream);
       for (auto& t : ctx->pending) {
         float ms = 0.f;
-        hipEventElapsedTime(&ms, t.a, t.b);
+        (void)hipEventElapsedTime(&ms, t.a, t.b);
         auto& s = ctx->stats[t.name];
         s.first += 1;
         s.second += ms;
@@ -175,14 +175,14 @@ int cuimg_hip_ctx_create(int device, cuimg_hip_ctx** out) {
 
 int cuimg_hip_ctx_destroy(cuimg_hip_ctx* ctx) {
   if (!ctx) return 0;
-  hipStreamSynchronize(ctx->stream);
-  for (auto& kv : ctx->image_base) hipFree(kv.second);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->image_base) (void)hipFree(kv.second);
   for (auto& t : ctx->pending) {
-    hipEventDestroy(t.a);
-    hipEventDestroy(t.b);
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
   }
-  for (auto e : ctx->free_events) hipEventDestroy(e);
-  hipStreamDestroy(ctx->stream);
+  for (auto e : ctx->free_events) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return 0;
 }
@@ -219,7 +219,7 @@ int cuimg_hip_profile_query(cuimg_hip_ctx* ctx, const char* kernel, int64_t* lau
   CUIMG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   for (auto& t : ctx->pending) {
     float ms = 0.f;
-    hipEventElapsedTime(&ms, t.a, t.b);
+    (void)hipEventElapsedTime(&ms, t.a, t.b);
     auto& s = ctx->stats[t.name];
     s.first += 1;
     s.second += ms;

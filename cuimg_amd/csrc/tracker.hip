@@ -44,6 +44,7 @@ struct level_res {
   uint32_t cap = 0;
   int nblocks = 0, nbr = 0, nbc = 0;
   bool compact_has_run = false;
+  cuimg_scale_params sp{};
 };
 
 }  // namespace cuimg
@@ -82,7 +83,7 @@ static int dev_alloc_n(cuimg_hip_tracker* t, size_t n, T** out) {
 
 static void destroy(cuimg_hip_tracker* t) {
   if (!t) return;
-  hipStreamSynchronize(t->ctx->stream);
+  (void)hipStreamSynchronize(t->ctx->stream);
   for (int s = 0; s < t->L; s++) {
     level_res& R = t->lv[s];
     for (int k = 0; k < 2; k++) {
@@ -94,7 +95,7 @@ static void destroy(cuimg_hip_tracker* t) {
     image_free(t->ctx, &R.sparse);
   }
   image_free(t->ctx, &t->frame_buf);
-  for (void* p : t->allocs) hipFree(p);
+  for (void* p : t->allocs) (void)hipFree(p);
   delete t;
 }
 
@@ -146,6 +147,8 @@ static int create(cuimg_hip_ctx* ctx, int nrows, int ncols, const cuimg_tracker_
     level_res& R = t->lv[s];
     R.nr = r;
     R.nc = c;
+    R.sp = cuimg_scale_params{P->fast_n, P->fast_th, P->detector_frequency, P->filtering_frequency, P->k,
+                              P->with_merge};
     for (int k = 0; k < 2; k++) {
       if ((rc = image_alloc(ctx, r, c, 3, CUIMG_U8, &R.u8[k]))) return fail(rc);
       if (P->profile == 1 && (rc = image_alloc(ctx, r, c, 0, CUIMG_F32, &R.blur[k]))) return fail(rc);
@@ -215,18 +218,18 @@ static int update_level(cuimg_hip_tracker* t, int s, int slot) {
     a.pd_c1 = 3;
     a.pd_r2 = R.nr - 4;
     a.pd_c2 = R.nc - 4;
-    a.P = lk_params_dev{P.lk_ws, P.lk_max_iters, P.lk_top_n, P.lk_eps, 1e-4f, P.k, P.profile == 0 ? 1 : 0};
+    a.P = lk_params_dev{P.lk_ws, P.lk_max_iters, P.lk_top_n, P.lk_eps, 1e-4f, R.sp.k, P.profile == 0 ? 1 : 0};
     CUIMG_TRY(launch_lk(ctx, a, P.profile == 0));
   }
 
-  if (!(t->frame_cpt % P.detector_frequency)) {
+  if (!(t->frame_cpt % R.sp.detector_frequency)) {
     detect_args d{};
     d.nr = R.nr;
     d.nc = R.nc;
     d.box = P.box_size;
     d.nbr = R.nbr;
     d.nbc = R.nbc;
-    d.fast_th = P.fast_th;
+    d.fast_th = R.sp.fast_th;
     d.sal = static_cast<const uint8_t*>(R.sal.begin);
     d.sal_pitch = R.sal.pitch;
     d.mask = static_cast<uint8_t*>(R.mask.begin);
@@ -244,13 +247,13 @@ static int update_level(cuimg_hip_tracker* t, int s, int slot) {
     d.cap = R.cap;
     d.nblocks = R.nblocks;
     CUIMG_TRY(launch_particle_mask(ctx, d));
-    CUIMG_TRY(fast_saliency(ctx, &R.u8[slot], &R.mask, &R.sal, P.fast_n, (int)P.fast_th));
+    CUIMG_TRY(fast_saliency(ctx, &R.u8[slot], &R.mask, &R.sal, R.sp.fast_n, (int)R.sp.fast_th));
     CUIMG_TRY(launch_detect_and_compact(ctx, d));
     R.cur ^= 1;
     R.compact_has_run = true;
   }
 
-  if (!(t->frame_cpt % P.filtering_frequency) && P.with_merge) {
+  if (!(t->frame_cpt % R.sp.filtering_frequency) && R.sp.with_merge) {
     merge_args m{};
     m.nr = R.nr;
     m.nc = R.nc;
@@ -322,6 +325,21 @@ int cuimg_hip_tracker_run_host(cuimg_hip_tracker* t, const uint8_t* frame, size_
   CUIMG_HIP(t->ctx, hipMemcpy2DAsync(t->frame_buf.begin, t->frame_buf.pitch, frame, pitch, t->frame_buf.ncols,
                                      t->frame_buf.nrows, hipMemcpyHostToDevice, t->ctx->stream));
   return run(t, &t->frame_buf);
+}
+
+int cuimg_hip_tracker_set_scale_params(cuimg_hip_tracker* t, int scale, const cuimg_scale_params* in) {
+  CUIMG_TRY(check_scale(t, scale));
+  if (!in || in->fast_n < 1 || in->fast_th < 0.f || in->detector_frequency < 1 || in->filtering_frequency < 1)
+    return set_error(t->ctx, CUIMG_E_INVALID, "set_scale_params: bad settings");
+  t->lv[scale].sp = *in;
+  return 0;
+}
+
+int cuimg_hip_tracker_get_scale_params(cuimg_hip_tracker* t, int scale, cuimg_scale_params* out) {
+  CUIMG_TRY(check_scale(t, scale));
+  if (!out) return CUIMG_E_INVALID;
+  *out = t->lv[scale].sp;
+  return 0;
 }
 
 int cuimg_hip_tracker_clear(cuimg_hip_tracker* t) {
@@ -447,7 +465,7 @@ int cuimg_hip_particles_merge(cuimg_hip_ctx* ctx, cuimg_particle_f* parts, uint3
   const size_t bytes = (size_t)cap * (4 + 4 + 4 + 1);
   CUIMG_HIP(ctx, hipMalloc(&buf, bytes));
   if (hipMalloc(&cnt, sizeof(level_counters)) != hipSuccess) {
-    hipFree(buf);
+    (void)hipFree(buf);
     return set_error(ctx, CUIMG_E_NOMEM, "particles_merge: allocation failed");
   }
   level_counters c{};
@@ -469,9 +487,9 @@ int cuimg_hip_particles_merge(cuimg_hip_ctx* ctx, cuimg_particle_f* parts, uint3
     m.cap = cap;
     rc = launch_merge(ctx, m);
   }
-  hipStreamSynchronize(ctx->stream);
-  hipFree(buf);
-  hipFree(cnt);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(buf);
+  (void)hipFree(cnt);
   return rc;
 }
 

@@ -156,6 +156,19 @@ typedef struct {
 
 typedef struct cuimg_hip_tracker cuimg_hip_tracker;
 
+/* Settings the reference keeps per scale (one pyrlk_cpu strategy and one
+ * fast_detector per scale): fast_detector::set_n / set_fast_threshold
+ * (tracking2/fast_detector.hpp:170-193), pyrlk_cpu::set_detector_frequency /
+ * set_filtering_frequency / set_k / set_with_merge (pyrlk_tracker.h:45-58). */
+typedef struct {
+  int32_t fast_n;
+  float fast_th;
+  int32_t detector_frequency;
+  int32_t filtering_frequency;
+  int32_t k;
+  int32_t with_merge;
+} cuimg_scale_params;
+
 /* tracker<pyrlk_cpu<fast_detector<cpu>, iterative_lk_match_5x5>>(domain, nscales)
  * (tracking2/tracker.hpp:12-26, pyrlk_tracker.hpp:771-793). */
 int cuimg_hip_tracker_create(cuimg_hip_ctx* ctx, int nrows, int ncols,
@@ -167,6 +180,9 @@ int cuimg_hip_tracker_destroy(cuimg_hip_tracker* t);
 int cuimg_hip_tracker_run(cuimg_hip_tracker* t, const cuimg_img* frame);
 /* Same from a host buffer (includes the host-to-device copy). */
 int cuimg_hip_tracker_run_host(cuimg_hip_tracker* t, const uint8_t* frame, size_t pitch);
+/* Per-scale settings; take effect from the next frame.  Reading fills `out`. */
+int cuimg_hip_tracker_set_scale_params(cuimg_hip_tracker* t, int scale, const cuimg_scale_params* in);
+int cuimg_hip_tracker_get_scale_params(cuimg_hip_tracker* t, int scale, cuimg_scale_params* out);
 /* tracker::clear (tracking2/tracker.hpp:95-102). */
 int cuimg_hip_tracker_clear(cuimg_hip_tracker* t);
 /* Number of entries of scale(s).pset().dense_particles(); synchronises. */
@@ -215,13 +231,13 @@ int cuimg_hip_particles_merge(cuimg_hip_ctx* ctx, cuimg_particle_f* parts, uint3
  *   L = sum_p sum_w ((G*I1)(x_p + d_p + w) - (G*I0)(x_p + w))^2
  * G = gaussian_sep(sigma, half_width), bilinear samples as the matcher's li
  * (:130-146), w over a ws x ws window.  Device inputs: pos2, d2 (n float2,
- * (row, col)).  Device outputs: *loss_total, loss_p[n], dL_dd2[n float2] and,
+ * (row, col)).  Device outputs: *loss_total (FP64), loss_p[n], dL_dd2[n float2] and,
  * if dL_dI1 is not NULL, dL/dI1 (F32 image of the frame domain).  Blurred
  * values are recomputed per particle, never stored.  scratch: device buffer of
  * at least 2 * nrows * ncols floats.  ws <= 7 (odd), half_width <= 6. */
 int cuimg_hip_lk_loss_adjoint(cuimg_hip_ctx* ctx, const cuimg_img* I0, const cuimg_img* I1,
                               const float* pos2, const float* d2, uint32_t n, int sigma,
-                              int half_width, int ws, float* loss_total, float* loss_p,
+                              int half_width, int ws, double* loss_total, float* loss_p,
                               float* dL_dd2, const cuimg_img* dL_dI1, float* scratch,
                               size_t scratch_floats);
 

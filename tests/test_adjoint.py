@@ -85,6 +85,11 @@ def test_gpu_adjoint_matches_fp64(orc):
     assert abs(L32 - L64) <= 1e-4 * L64
     assert rel_norm(gd, gd64) <= 1e-4
     assert rel_norm(gI, gI64) <= 1e-4
+    # per-particle losses against the FP64 forward
+    _, _, _ = orc.loss_grad(I0, I1, coef, 3, pos[:1], d[:1], want_dI=False)
+    lp64 = np.array([orc.loss_grad(I0, I1, coef, 3, pos[i:i + 1], d[i:i + 1], want_dI=False)[0]
+                     for i in range(0, 50000, 997)])
+    assert np.abs(lp[::997] - lp64).max() <= 1e-4 * lp64.max()
     # per particle, away from cancellation
     mag = np.abs(gd64).max(1)
     big = mag > 1e-3 * mag.max()
