@@ -2,18 +2,21 @@
 
 Workload (default): the plain method on a 1920x1080 uint8 synthetic video with
 per-frame random affine motion, 4 pyramid scales, sigma=1 / 7-tap FP32 blur,
-FAST threshold 20, 7x7 LK windows with <= 5 iterations.  One step = one
-tracker frame step (pyramid, blur, matcher on every scale, detector mask,
-saliency, box selection, insertion, compaction, merge) on one frame.  All
-frames are generated on the host and uploaded before timing, so the timed
-region starts with its inputs resident in HBM.
+7x7 LK windows with <= 5 iterations, FAST threshold 8, which keeps about 100k
+particles per frame on this video.  One step = one tracker frame step
+(pyramid, blur, matcher on every scale, detector mask, saliency, box
+selection, insertion, compaction, merge) on one frame.  All frames are
+generated on the host and uploaded before timing, so the timed region starts
+with its inputs resident in HBM.  The same video is also run at the
+reference's default threshold 20 (about 14k particles) and reported beside it.
 
 value = particle-updates/s: particles (age > 0) advanced through the matcher
 at their own scale (each one includes its coarser ancestor solves), summed
 over scales and timed frames, divided by the timed wall time.
 
 Printed: one JSON line (rank 0).  Extra keys carry frames/s, per-kernel
-timings, the Gaussian / pyramid roofline microbench and the CPU baseline.
+timings, the threshold-20 run, the Gaussian / pyramid roofline microbench and
+the CPU baseline.
 """
 import argparse
 import json
@@ -29,6 +32,9 @@ import numpy as np  # noqa: E402
 METRIC = "particle-updates/s & frames/s @1080p 4-lvl 100k pts; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
 SEED = 0xC0FFEE02
+NR, NC, NSCALES = 1080, 1920, 4
+FAST_TH = 8.0          # about 100k particles on the synthetic video
+FAST_TH_SPARSE = 20.0  # the reference's default threshold: about 14k particles
 
 
 def log(msg):
@@ -43,52 +49,60 @@ def lk_algorithmic_bytes(level_shapes, s, nparts):
     return img + 40 * nparts
 
 
-def tracker_bench(cu, ctx, args):
-    nr, nc, L = 1080, 1920, 4
-    nframes = args.warmup + args.steps
+KERNELS = ("lk_track", "gaussian_sep", "pyramid_build", "fast_saliency", "fill_u32", "fill_u8",
+           "particle_mask", "nms_boxes", "subrow_count", "compact_count", "compact_scan",
+           "compact_scatter", "insert_new", "merge_init", "merge_resolve", "merge_apply")
+
+
+def upload_video(cu, ctx, nframes):
     t0 = time.time()
-    video = cu.synth_video(SEED, nr, nc, nframes, 1)
+    video = cu.synth_video(SEED, NR, NC, nframes, 1)
     log(f"generated {nframes} frames in {time.time() - t0:.1f}s")
     frames = []
     for f in range(nframes):
-        im = ctx.image(nr, nc, cu.U8)
+        im = ctx.image(NR, NC, cu.U8)
         im.upload(video[f])
         frames.append(im)
-    params = cu.profile_params("B", L, 20.0)
-    trk = cu.Tracker(ctx, nr, nc, params)
-    shapes = [(nr >> s, nc >> s) for s in range(L)]
+    return video, frames
 
-    for f in range(args.warmup):
+
+def tracker_bench(cu, ctx, frames, fast_th, warmup, steps, kernel_events):
+    """warmup untimed frames, then `steps` timed frame steps of one tracker."""
+    params = cu.profile_params("B", NSCALES, fast_th)
+    trk = cu.Tracker(ctx, NR, NC, params)
+    shapes = [(NR >> s, NC >> s) for s in range(NSCALES)]
+    for f in range(warmup):
         trk.run(frames[f])
     ctx.sync()
-    live0 = [trk.work(s)[0] for s in range(L)]
-    ctx.profile(args.kernel_events)
+    live0 = [trk.work(s)[0] for s in range(NSCALES)]
+    ctx.profile(kernel_events)
     ctx.profile_reset()
     ctx.sync()
     t0 = time.perf_counter()
-    for f in range(args.warmup, nframes):
+    for f in range(warmup, warmup + steps):
         trk.run(frames[f])
     ctx.sync()
     dt = time.perf_counter() - t0
-    live1 = [trk.work(s)[0] for s in range(L)]
-    updates = sum(live1) - sum(live0)
+    live1 = [trk.work(s)[0] for s in range(NSCALES)]
     per_scale = [b - a for a, b in zip(live0, live1)]
-    nparts = [trk.nparticles(s) for s in range(L)]
-
+    nparts = [trk.nparticles(s) for s in range(NSCALES)]
     kernels = {}
-    for name in ("lk_track", "gaussian_sep", "pyramid_build", "fast_saliency", "fill_u32", "fill_u8",
-                 "particle_mask", "nms_boxes", "subrow_count", "compact_count", "compact_scan",
-                 "compact_scatter", "insert_new", "merge_init", "merge_resolve", "merge_apply"):
+    for name in KERNELS:
         n, ms = ctx.profile_query(name)
         if n:
             kernels[name] = {"launches": n, "total_ms": round(ms, 4), "avg_us": round(ms / n * 1000, 3)}
     ctx.profile(False)
-    res = dict(seconds=dt, frames=args.steps, updates=updates, per_scale=per_scale, nparts=nparts,
-               kernels=kernels, shapes=shapes)
     trk.close()
-    for im in frames:
-        im.free()
-    return res, video
+    return dict(seconds=dt, frames=steps, updates=sum(per_scale), per_scale=per_scale, nparts=nparts,
+                kernels=kernels, shapes=shapes, fast_th=fast_th)
+
+
+def summary(tr):
+    return {"fast_th": tr["fast_th"], "particle_updates_per_s": round(tr["updates"] / tr["seconds"], 1),
+            "frames_per_s": round(tr["frames"] / tr["seconds"], 2),
+            "ms_per_step": round(tr["seconds"] / tr["frames"] * 1000, 4),
+            "updates_per_frame": round(tr["updates"] / tr["frames"], 1),
+            "particles_per_scale": tr["nparts"], "kernels": tr["kernels"]}
 
 
 def gaussian_microbench(cu, ctx, reps=20):
@@ -155,27 +169,45 @@ def gaussian_microbench(cu, ctx, reps=20):
     return out
 
 
-def cpu_baseline(video, budget_s):
+def cpu_baseline(video, budget_s, fast_th):
     """The CPU oracle (a restatement of the reference algorithms, OpenMP over
     particles and rows like the reference) on the first frames of the same
-    video, until about `budget_s` seconds of CPU work."""
+    video and threshold, until about `budget_s` seconds of CPU work."""
     import oracle
     orc = oracle.Oracle()
     nr, nc = video.shape[1:]
-    t = orc.tracker(nr, nc, oracle.profile_params("B", 4, 20.0))
+    t = orc.tracker(nr, nc, oracle.profile_params("B", NSCALES, fast_th))
     t.run(video[0])
     frames, updates, spent = 0, 0, 0.0
     while spent < budget_s and frames + 1 < len(video):
-        live = sum(int((t.particles(s)["age"] > 0).sum()) for s in range(4))
+        live = sum(int((t.particles(s)["age"] > 0).sum()) for s in range(NSCALES))
         t0 = time.perf_counter()
         t.run(video[frames + 1])
         spent += time.perf_counter() - t0
         frames += 1
         updates += live
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": updates / spent, "unit": "particle-updates/s", "cores": threads, "kind": "port",
-            "sample": f"CPU oracle, frames 1..{frames} of the same 1920x1080 video ({spent:.1f} s, "
-                      f"{updates} particle updates, {frames / spent:.2f} frames/s)"}
+    return {"value": round(updates / spent, 1), "unit": "particle-updates/s", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle, frames 1..{frames} of the same 1920x1080 video at FAST th {fast_th:g} "
+                      f"({spent:.1f} s, {updates} particle updates, {frames / spent:.2f} frames/s)"}
+
+
+def lk_roofline(tr):
+    """Roofline entry of the matcher: algorithmic bytes of the timed frames'
+    matcher launches divided by their measured time."""
+    k = tr["kernels"].get("lk_track")
+    if not k:
+        return None
+    L = len(tr["shapes"])
+    avg_parts = [u / tr["frames"] for u in tr["per_scale"]]
+    per_frame = sum(lk_algorithmic_bytes(tr["shapes"], s, avg_parts[s]) for s in range(L))
+    bytes_per_launch = per_frame / L
+    achieved = bytes_per_launch / (k["avg_us"] * 1e-6) / 1e9
+    return {"kernel": "lk_track", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "bytes_per_launch": int(bytes_per_launch), "avg_us": k["avg_us"],
+            "note": "matcher is gather/latency bound; bytes = both FP32 frames of scales >= s "
+                    "+ 40 B per particle, averaged over the 4 per-frame launches"}
 
 
 def main():
@@ -186,6 +218,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-micro", action="store_true")
+    ap.add_argument("--no-sparse", action="store_true", help="skip the FAST th 20 run")
+    ap.add_argument("--fast-th", type=float, default=FAST_TH)
     ap.add_argument("--kernel-events", type=int, default=1,
                     help="record HIP events around every kernel in the timed region")
     args = ap.parse_args()
@@ -196,58 +230,51 @@ def main():
 
     import cuimg_amd as cu
     ctx = cu.Context(0)
-    log("tracker benchmark")
-    tr, video = tracker_bench(cu, ctx, args)
+    video, frames = upload_video(cu, ctx, args.warmup + args.steps)
+    log(f"tracker benchmark, FAST th {args.fast_th:g}")
+    tr = tracker_bench(cu, ctx, frames, args.fast_th, args.warmup, args.steps, args.kernel_events)
+    log(f"  {tr['updates'] / tr['seconds']:.4g} particle-updates/s, {tr['frames'] / tr['seconds']:.1f} frames/s")
+    sparse = None
+    if not args.no_sparse:
+        steps = min(args.steps, 120)
+        log(f"tracker benchmark, FAST th {FAST_TH_SPARSE:g}")
+        sp = tracker_bench(cu, ctx, frames, FAST_TH_SPARSE, min(args.warmup, 30), steps, args.kernel_events)
+        sparse = summary(sp)
+        log(f"  {sparse['particle_updates_per_s']:.4g} particle-updates/s, {sparse['frames_per_s']} frames/s")
+    for im in frames:
+        im.free()
     micro = [] if args.no_micro else gaussian_microbench(cu, ctx)
     ctx.close()
-
-    value = tr["updates"] / tr["seconds"]
-    ms_per_step = tr["seconds"] / tr["frames"] * 1000
-    k = tr["kernels"]
-    # dominant kernel of the frame step
-    dom = max(k.items(), key=lambda kv: kv[1]["total_ms"])[0] if k else None
-    roofline = None
-    if dom == "lk_track":
-        L = len(tr["shapes"])
-        # algorithmic bytes of all matcher launches of the timed frames / their total time
-        avg_parts = [u / tr["frames"] for u in tr["per_scale"]]
-        per_frame = sum(lk_algorithmic_bytes(tr["shapes"], s, avg_parts[s]) for s in range(L))
-        bytes_per_launch = per_frame / L
-        achieved = bytes_per_launch / (k[dom]["avg_us"] * 1e-6) / 1e9
-        roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                    "note": "matcher is gather/latency bound; bytes = both FP32 frames of scales >= s "
-                            "+ 40 B per particle, averaged over the 4 per-frame launches"}
-    elif dom is not None:
-        roofline = {"kernel": dom, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": None, "traffic": None}
 
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
-        cpu = cpu_baseline(video, args.cpu_budget)
+        cpu = cpu_baseline(video, args.cpu_budget, args.fast_th)
 
+    s = summary(tr)
     line = {
         "metric": METRIC,
-        "value": round(value, 1),
+        "value": s["particle_updates_per_s"],
         "unit": "particle-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": s["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "plain pyramid-LK tracker, 1920x1080 u8 affine-motion video, 4 scales, "
-                               "sigma=1 7-tap FP32 blur, FAST th 20, 7x7 LK <=5 it",
+        "config": {"workload": f"plain pyramid-LK tracker, 1920x1080 u8 affine-motion video, 4 scales, "
+                               f"sigma=1 7-tap FP32 blur, FAST th {args.fast_th:g} "
+                               f"(~{s['updates_per_frame'] / 1000:.0f}k particle updates/frame), 7x7 LK <=5 it",
                    "frames": args.warmup + args.steps, "parallelism": "1 GPU"},
-        "frames_per_s": round(tr["frames"] / tr["seconds"], 2),
-        "particles_per_scale": tr["nparts"],
-        "updates_per_frame": round(tr["updates"] / tr["frames"], 1),
-        "roofline": roofline,
-        "kernels": k,
+        "frames_per_s": s["frames_per_s"],
+        "particles_per_scale": s["particles_per_scale"],
+        "updates_per_frame": s["updates_per_frame"],
+        "roofline": lk_roofline(tr),
+        "kernels": s["kernels"],
+        "fast_th_20": sparse,
         "gaussian_pyramid_roofline": micro,
         "cpu_baseline": cpu,
     }
