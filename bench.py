@@ -51,7 +51,7 @@ def lk_algorithmic_bytes(level_shapes, s, nparts):
 
 KERNELS = ("lk_track", "gaussian_sep", "pyramid_build", "fast_saliency", "fill_u32", "fill_u8",
            "particle_mask", "nms_boxes", "subrow_count", "compact_count", "compact_scan",
-           "compact_scatter", "insert_new", "merge_init", "merge_resolve", "merge_apply")
+           "compact_scatter", "insert_new", "merge_scan", "merge_resolve")
 
 
 def upload_video(cu, ctx, nframes):

@@ -23,8 +23,8 @@
 // (box row, row offset) "sub-row" is counted by one wave, one scan over
 // sub-rows gives the offsets, and one wave per sub-row writes its points with
 // a ballot prefix.  Compaction is a classic count / scan / scatter over
-// 1024-particle blocks.  The serial merge is resolved exactly in parallel rounds
-// (see merge_round below).
+// 1024-particle blocks.  The serial merge is resolved exactly in parallel
+// rounds and a serial replay of the few actors they leave open.
 #include "tracker_state.hpp"
 
 #include <climits>
@@ -332,136 +332,346 @@ int launch_detect_and_compact(cuimg_hip_ctx* ctx, const detect_args& a) {
 
 // ------------------------------------------------------------------ merge
 //
-// Serial semantics, resolved in parallel.  With the ages before the merge, the
-// edge i -> b ("i may remove b") is static: b is a c8 neighbour (through the
-// sparse image) with age_b > age_i + 2 and |speed_i - speed_b| < 2.  In index
-// order, i acts iff nobody earlier removed it, and then removes the first of
-// its targets that nobody earlier removed.  A round resolves every pending
-// actor i whose outcome no longer depends on a pending actor j < i (j < i
-// targeting i, or j < i targeting one of i's targets up to the first live
-// one); the smallest pending actor always resolves, so the rounds terminate,
-// and each decision equals the serial one.
+// Serial semantics, resolved in parallel rounds.  With the ages before the
+// merge, the edge i -> b ("i may remove b") is static: b is a c8 neighbour
+// (through the sparse image) with age_b > age_i + 2 and |speed_i - speed_b| < 2.
+// In index order, i acts iff nobody earlier removed it, and then removes the
+// first of its targets that nobody earlier removed.  Only actors targeting a
+// particle can remove it.  A round resolves every open actor i whose outcome
+// no open actor j < i can change: no open j < i targets i, and no open j < i
+// targets one of i's targets up to the first one still alive at i's turn
+// (zeroer[b] = index of b's remover; b is dead at i's turn iff zeroer[b] < i).
+// The smallest open actor always resolves, and on tracking data each round
+// leaves about a quarter of the open actors.  Once at most 64 remain (or after
+// kMaxRounds rounds) the rest are sorted by index and replayed serially by one
+// wave: a removal is visible to a later actor through zeroer (removals by
+// resolved actors and by earlier chunks) or through a ballot over the victims
+// of the earlier lanes of its chunk.
+//
+// minunres[b] = (255 - round) << 24 | smallest open actor targeting b, written
+// by atomicMin: a later round's tag is smaller, so stale entries never win and
+// no reset pass is needed.  Particle indices stay below 2^24.  Between calls
+// zeroer and minunres hold 0xFFFFFFFF; the resolve kernel restores the entries
+// it touched.
 
-__device__ __forceinline__ int32_t ld_acq(const int32_t* p) {
+constexpr uint32_t kNoIdx = 0xFFFFFFFFu;
+constexpr uint32_t kPendLds = 4096;
+constexpr uint32_t kMaxRounds = 250;
+constexpr uint32_t kSerialSwitch = 64;
+
+__device__ __forceinline__ uint32_t ld32(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_rel(int32_t* p, int32_t v) {
+__device__ __forceinline__ void st32(uint32_t* p, uint32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ unsigned long long ld64(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st64(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t round_tag(uint32_t r) { return (255u - r) << 24; }
 
 __device__ __forceinline__ float norml2_rn(float a, float b) {
   return sqrtf(__fadd_rn(__fmul_rn(a, a), __fmul_rn(b, b)));
 }
 
-__global__ __launch_bounds__(256) void merge_init_kernel(merge_args a) {
+__host__ __device__ static uint32_t pow2_at_least(uint32_t n) {
+  uint32_t p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+size_t merge_scratch_bytes(uint32_t cap) {
+  return (size_t)pow2_at_least(cap) * 8 + (size_t)cap * sizeof(merge_actor) + 2 * (size_t)cap * 4 + 256;
+}
+
+merge_args merge_scratch_carve(void* base, uint32_t cap) {
+  const uint32_t pc = pow2_at_least(cap);
+  merge_args m{};
+  char* p = static_cast<char*>(base);
+  m.pend = reinterpret_cast<unsigned long long*>(p);
+  p += (size_t)pc * 8;
+  m.actors = reinterpret_cast<merge_actor*>(p);
+  p += (size_t)cap * sizeof(merge_actor);
+  m.zeroer = reinterpret_cast<uint32_t*>(p);
+  p += (size_t)cap * 4;
+  m.minunres = reinterpret_cast<uint32_t*>(p);
+  m.cap = cap;
+  m.pend_cap = pc;
+  return m;
+}
+
+int launch_merge_reset(cuimg_hip_ctx* ctx, const merge_args& a) {
+  // zeroer and minunres are adjacent: one fill of 2*cap words
+  return launch_fill_u32(ctx, a.zeroer, 0, 0, 1, 0, (int)(2 * a.cap), kNoIdx);
+}
+
+// One thread per particle: targets in c8 order, actor records appended with
+// one atomic per wave, round-1 minunres by atomicMin.
+__global__ __launch_bounds__(256) void merge_scan_kernel(merge_args a) {
   const uint32_t n = a.cnt->n;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    st_rel(&a.zeroer[i], INT_MAX);
-    st_rel(&a.minunres[i], INT_MAX);
-    const cuimg_particle_f p = a.parts[i];
-    if (p.age == 0) continue;
-    const int r = tzf(p.pos_r), c = tzf(p.pos_c);
-    uint32_t mask = 0;
-    for (int k = 0; k < 8; k++) {
-      const uint32_t b = sparse_get(a.sparse, a.sparse_pitch, a.nr, a.nc, r + kC8R[k], c + kC8C[k]);
-      if (b == kEmpty || b >= n) continue;  // indices past the vector are treated as empty
-      const cuimg_particle_f q = a.parts[b];
-      if ((int)q.age > (int)p.age + 2 &&
-          norml2_rn(__fsub_rn(p.speed_r, q.speed_r), __fsub_rn(p.speed_c, q.speed_c)) < 2.f)
-        mask |= 1u << k;
+  const int lane = threadIdx.x & 63;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
+    const uint32_t i = i0 + lane;
+    merge_actor rec;
+    rec.i = (int32_t)i;
+    rec.mask = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) rec.b[k] = -1;
+    if (i < n) {
+      const cuimg_particle_f p = a.parts[i];
+      if (p.age != 0) {
+        const int r = tzf(p.pos_r), c = tzf(p.pos_c);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          const uint32_t b = sparse_get(a.sparse, a.sparse_pitch, a.nr, a.nc, r + kC8R[k], c + kC8C[k]);
+          if (b == kEmpty || b >= n) continue;  // indices past the vector are treated as empty
+          const cuimg_particle_f q = a.parts[b];
+          if ((int)q.age > (int)p.age + 2 &&
+              norml2_rn(__fsub_rn(p.speed_r, q.speed_r), __fsub_rn(p.speed_c, q.speed_c)) < 2.f) {
+            rec.mask |= 1u << k;
+            rec.b[k] = (int32_t)b;
+          }
+        }
+      }
     }
-    if (mask) {
-      const uint32_t slot = atomicAdd(&a.cnt->actors, 1u);
-      a.actors[slot] = i | (mask << 24);
+    const bool act = rec.mask != 0;
+    const uint64_t m = __ballot(act);
+    if (m == 0) continue;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&a.cnt->actors, (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (act) {
+      const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+      a.actors[base + __popcll(m & below)] = rec;
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (rec.mask & (1u << k)) atomicMin(&a.minunres[rec.b[k]], round_tag(1) | i);
     }
   }
 }
 
-__device__ __forceinline__ int32_t target_of(const merge_args& a, const cuimg_particle_f& p, int k) {
-  return (int32_t)sparse_get(a.sparse, a.sparse_pitch, a.nr, a.nc, tzf(p.pos_r) + kC8R[k], tzf(p.pos_c) + kC8C[k]);
+// Round-r decision for an open actor: false = still open.  A concurrent
+// resolver j of this round writes zeroer[b] only where no open actor below j
+// targets b, so whatever this thread reads there gives the same answer.
+__device__ bool merge_decide(const merge_args& a, const merge_actor& A, uint32_t r) {
+  const uint32_t tg = 255u - r, i = (uint32_t)A.i;
+  auto blocked = [&](uint32_t x) {
+    const uint32_t v = ld32(&a.minunres[x]);
+    return (v >> 24) == tg && (v & 0xFFFFFFu) < i;
+  };
+  if (blocked(i)) return false;
+  if (ld32(&a.zeroer[i]) < i) return true;  // removed before its turn
+  int32_t victim = -1;
+#pragma unroll
+  for (int j = 0; j < 8; j++)
+    if (victim < 0 && (A.mask & (1u << j))) {
+      const uint32_t b = (uint32_t)A.b[j];
+      if (blocked(b)) return false;
+      if (ld32(&a.zeroer[b]) < i) continue;
+      victim = (int32_t)b;
+    }
+  if (victim >= 0) {
+    st32(&a.zeroer[victim], i);
+    a.parts[victim].age = 0;
+  }
+  return true;
 }
 
-__global__ __launch_bounds__(1024) void merge_resolve_kernel(merge_args a) {
-  __shared__ uint32_t pending;
-  const uint32_t m = a.cnt->actors;
-  for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) a.resolved[k] = 0;
-  __syncthreads();
-  for (;;) {
-    // (a) clear the pending-minimum of pending actors and their targets
-    for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
-      if (a.resolved[k]) continue;
-      const uint32_t e = a.actors[k];
-      const int32_t i = (int32_t)(e & 0xFFFFFFu);
-      const uint32_t mask = e >> 24;
-      st_rel(&a.minunres[i], INT_MAX);
-      const cuimg_particle_f p = a.parts[i];
-      for (int t = 0; t < 8; t++)
-        if (mask & (1u << t)) st_rel(&a.minunres[target_of(a, p, t)], INT_MAX);
-    }
-    __threadfence();
-    __syncthreads();
-    // (b) smallest pending actor targeting each particle
-    for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
-      if (a.resolved[k]) continue;
-      const uint32_t e = a.actors[k];
-      const int32_t i = (int32_t)(e & 0xFFFFFFu);
-      const uint32_t mask = e >> 24;
-      const cuimg_particle_f p = a.parts[i];
-      for (int t = 0; t < 8; t++)
-        if (mask & (1u << t)) atomicMin(&a.minunres[target_of(a, p, t)], i);
-    }
-    __threadfence();
-    if (threadIdx.x == 0) pending = 0;
-    __syncthreads();
-    // (c) resolve what no smaller pending actor can change
-    for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
-      if (a.resolved[k]) continue;
-      const uint32_t e = a.actors[k];
-      const int32_t i = (int32_t)(e & 0xFFFFFFu);
-      const uint32_t mask = e >> 24;
-      bool done = false;
-      if (ld_acq(&a.minunres[i]) < i) {
-        done = false;  // an earlier pending actor may remove i
-      } else if (ld_acq(&a.zeroer[i]) < i) {
-        done = true;   // removed before its turn: no action
-      } else {
-        const cuimg_particle_f p = a.parts[i];
-        bool blocked = false;
-        int32_t victim = -1;
-        for (int t = 0; t < 8 && victim < 0 && !blocked; t++) {
-          if (!(mask & (1u << t))) continue;
-          const int32_t b = target_of(a, p, t);
-          if (ld_acq(&a.minunres[b]) < i) blocked = true;
-          else if (ld_acq(&a.zeroer[b]) < i) continue;
-          else victim = b;
+// Bitonic sort of keys[0, np) ascending, np a power of two, by one workgroup.
+template <bool GLOBAL>
+__device__ void bitonic_sort(unsigned long long* keys, uint32_t np) {
+  for (uint32_t size = 2; size <= np; size <<= 1)
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (uint32_t t = threadIdx.x; t < np / 2; t += blockDim.x) {
+        const uint32_t lo = 2 * t - (t & (stride - 1)), hi = lo + stride;
+        const bool asc = (lo & size) == 0;
+        const unsigned long long x = GLOBAL ? ld64(&keys[lo]) : keys[lo];
+        const unsigned long long y = GLOBAL ? ld64(&keys[hi]) : keys[hi];
+        if ((x > y) == asc) {
+          if (GLOBAL) {
+            st64(&keys[lo], y);
+            st64(&keys[hi], x);
+          } else {
+            keys[lo] = y;
+            keys[hi] = x;
+          }
         }
-        if (!blocked) {
-          if (victim >= 0) atomicMin(&a.zeroer[victim], i);
+      }
+      if (GLOBAL) __threadfence();
+      __syncthreads();
+    }
+}
+
+// Ascending sort of one key per lane across the wave.
+__device__ unsigned long long wave_sort(unsigned long long key) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1)
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const unsigned long long o = __shfl_xor(key, j, 64);
+      const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+      key = keep_min ? (o < key ? o : key) : (o > key ? o : key);
+    }
+  return key;
+}
+
+__device__ __forceinline__ uint32_t rdl(uint32_t v, uint32_t l) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+
+// Serial replay of up to 64 open actors in index order (one wave; `key` of
+// lane l = the l-th actor, kNoIdx keys past the end).
+__device__ void replay_chunk(const merge_args& a, unsigned long long key) {
+  const int lane = threadIdx.x & 63;
+  const bool valid = key != ~0ull;
+  uint32_t ai = kNoIdx, zs = kNoIdx, amask = 0;
+  uint32_t ab[8], zb[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) ab[j] = zb[j] = kNoIdx;
+  if (valid) {
+    const merge_actor A = a.actors[(uint32_t)key];
+    ai = (uint32_t)A.i;
+    amask = A.mask;
+    zs = ld32(&a.zeroer[ai]);
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      if (amask & (1u << j)) {
+        ab[j] = (uint32_t)A.b[j];
+        zb[j] = ld32(&a.zeroer[ab[j]]);
+      }
+  }
+  const uint32_t cnt = (uint32_t)__popcll(__ballot(valid));  // valid keys come first
+  uint32_t victim = kNoIdx;
+  for (uint32_t l = 0; l < cnt; l++) {
+    const uint32_t i = rdl(ai, l);
+    const uint64_t below = l == 0 ? 0ull : (~0ull >> (64 - l));
+    if (rdl(zs, l) < i || (__ballot(victim == i) & below)) continue;  // removed before its turn
+    const uint32_t msk = rdl(amask, l);
+    bool done = false;
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      if (!done && (msk & (1u << j))) {
+        const uint32_t b = rdl(ab[j], l);
+        if (!(rdl(zb[j], l) < i || (__ballot(victim == b) & below))) {
+          if ((uint32_t)lane == l) victim = b;
           done = true;
         }
       }
-      if (done) a.resolved[k] = 1;
-      else atomicAdd(&pending, 1u);
+  }
+  if (valid && victim != kNoIdx) {
+    st32(&a.zeroer[victim], ai);
+    a.parts[victim].age = 0;
+  }
+  __threadfence();
+}
+
+// One workgroup: round 1 over all actors, later rounds over the open ones,
+// serial replay of the rest, restoration of the scratch.
+__global__ __launch_bounds__(1024) void merge_resolve_kernel(merge_args a) {
+  __shared__ unsigned long long keys_lds[kPendLds];
+  __shared__ unsigned long long small[64];
+  __shared__ uint32_t npend, nopen[2], nsmall;
+  const uint32_t m = a.cnt->actors;
+  if (threadIdx.x == 0) npend = 0;
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
+    const merge_actor A = a.actors[k];
+    if (!merge_decide(a, A, 1)) {
+      const uint32_t slot = atomicAdd(&npend, 1u);
+      const unsigned long long key = ((unsigned long long)(uint32_t)A.i << 32) | k;
+      if (slot < kPendLds) keys_lds[slot] = key;
+      else st64(&a.pend[slot], key);
+    }
+  }
+  __threadfence();
+  __syncthreads();
+  const uint32_t np = npend;
+  const bool in_lds = np <= kPendLds;
+  if (!in_lds) {
+    for (uint32_t k = threadIdx.x; k < kPendLds; k += blockDim.x) st64(&a.pend[k], keys_lds[k]);
+    __threadfence();
+    __syncthreads();
+  }
+  unsigned long long* keys = in_lds ? keys_lds : a.pend;
+  auto kget = [&](uint32_t s) { return in_lds ? keys_lds[s] : ld64(&a.pend[s]); };
+  auto kset = [&](uint32_t s, unsigned long long v) {
+    if (in_lds) keys_lds[s] = v;
+    else st64(&a.pend[s], v);
+  };
+
+  uint32_t open = np, r = 1;
+  while (open > kSerialSwitch && r < kMaxRounds) {
+    r++;
+    if (threadIdx.x == 0) nopen[r & 1] = 0;
+    for (uint32_t s = threadIdx.x; s < np; s += blockDim.x) {
+      const unsigned long long key = kget(s);
+      if (key == ~0ull) continue;
+      const merge_actor A = a.actors[(uint32_t)key];
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        if (A.mask & (1u << j)) atomicMin(&a.minunres[A.b[j]], round_tag(r) | (uint32_t)A.i);
     }
     __threadfence();
     __syncthreads();
-    if (pending == 0) break;
+    for (uint32_t s = threadIdx.x; s < np; s += blockDim.x) {
+      const unsigned long long key = kget(s);
+      if (key == ~0ull) continue;
+      if (merge_decide(a, a.actors[(uint32_t)key], r)) kset(s, ~0ull);
+      else atomicAdd(&nopen[r & 1], 1u);
+    }
+    __threadfence();
     __syncthreads();
+    open = nopen[r & 1];
+  }
+
+  if (open > 0 && open <= 64) {
+    // gather the open keys into one wave, sort them there, replay
+    if (threadIdx.x == 0) nsmall = 0;
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < np; s += blockDim.x) {
+      const unsigned long long key = kget(s);
+      if (key != ~0ull) small[atomicAdd(&nsmall, 1u)] = key;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) replay_chunk(a, wave_sort(threadIdx.x < open ? small[threadIdx.x] : ~0ull));
+  } else if (open > 0) {
+    // after kMaxRounds: full sort (open keys first), replay in chunks of 64
+    const uint32_t np2 = pow2_at_least(np);
+    for (uint32_t k = np + threadIdx.x; k < np2; k += blockDim.x) kset(k, ~0ull);
+    __threadfence();
+    __syncthreads();
+    if (np2 <= kPendLds) bitonic_sort<false>(keys, np2);
+    else bitonic_sort<true>(keys, np2);
+    if (threadIdx.x < 64)
+      for (uint32_t c0 = 0; c0 < open; c0 += 64) {
+        const uint32_t idx = c0 + threadIdx.x;
+        replay_chunk(a, idx < open ? kget(idx) : ~0ull);
+      }
+  }
+  __syncthreads();
+  // restore the scratch for the next call
+  for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
+    const merge_actor A = a.actors[k];
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      if (A.mask & (1u << j)) {
+        st32(&a.minunres[A.b[j]], kNoIdx);
+        st32(&a.zeroer[A.b[j]], kNoIdx);
+      }
   }
   if (threadIdx.x == 0) a.cnt->actors = 0;
 }
 
-__global__ __launch_bounds__(256) void merge_apply_kernel(merge_args a) {
-  const uint32_t n = a.cnt->n;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    if (ld_acq(&a.zeroer[i]) != INT_MAX) a.parts[i].age = 0;
-}
-
 int launch_merge(cuimg_hip_ctx* ctx, const merge_args& a) {
   const int blocks = (int)std::min<uint32_t>((a.cap + 255) / 256, 2048);
-  CUIMG_TRY(launch(ctx, "merge_init", [&] { merge_init_kernel<<<blocks, 256, 0, ctx->stream>>>(a); }));
-  CUIMG_TRY(launch(ctx, "merge_resolve", [&] { merge_resolve_kernel<<<1, 1024, 0, ctx->stream>>>(a); }));
-  return launch(ctx, "merge_apply", [&] { merge_apply_kernel<<<blocks, 256, 0, ctx->stream>>>(a); });
+  CUIMG_TRY(launch(ctx, "merge_scan", [&] { merge_scan_kernel<<<blocks, 256, 0, ctx->stream>>>(a); }));
+  return launch(ctx, "merge_resolve", [&] { merge_resolve_kernel<<<1, 1024, 0, ctx->stream>>>(a); });
 }
 
 }  // namespace cuimg

@@ -37,10 +37,7 @@ struct level_res {
   int32_t* dec_accept = nullptr;
   float2* dec_tr = nullptr;
   float* dec_dist = nullptr;
-  uint32_t* actors = nullptr;
-  int32_t* zeroer = nullptr;
-  int32_t* minunres = nullptr;
-  uint8_t* resolved = nullptr;
+  merge_args merge{};  // merge scratch (one allocation)
   uint32_t cap = 0;
   int nblocks = 0, nbr = 0, nbc = 0;
   bool compact_has_run = false;
@@ -160,6 +157,8 @@ static int create(cuimg_hip_ctx* ctx, int nrows, int ncols, const cuimg_tracker_
     R.nbc = (int)((float)c / (float)P->box_size);
     const size_t nboxes = (size_t)R.nbr * R.nbc;
     R.cap = (uint32_t)std::max<size_t>((size_t)r * c / 4, 4096);
+    if (R.cap > kMergeMaxCap)
+      return fail(set_error(ctx, CUIMG_E_UNSUPPORTED, "tracker: %dx%d exceeds the particle capacity limit", r, c));
     R.nblocks = ceil_div((int)R.cap, 1024);
     if ((rc = dev_alloc_n(t, std::max<size_t>(nboxes, 1), &R.boxcode))) return fail(rc);
     if ((rc = dev_alloc_n(t, std::max<size_t>((size_t)R.nbr * P->box_size, 1), &R.subcnt))) return fail(rc);
@@ -171,10 +170,12 @@ static int create(cuimg_hip_ctx* ctx, int nrows, int ncols, const cuimg_tracker_
     if ((rc = dev_alloc_n(t, R.cap, &R.dec_accept))) return fail(rc);
     if ((rc = dev_alloc_n(t, R.cap, &R.dec_tr))) return fail(rc);
     if ((rc = dev_alloc_n(t, R.cap, &R.dec_dist))) return fail(rc);
-    if ((rc = dev_alloc_n(t, R.cap, &R.actors))) return fail(rc);
-    if ((rc = dev_alloc_n(t, R.cap, &R.zeroer))) return fail(rc);
-    if ((rc = dev_alloc_n(t, R.cap, &R.minunres))) return fail(rc);
-    if ((rc = dev_alloc_n(t, R.cap, &R.resolved))) return fail(rc);
+    {
+      void* m = nullptr;
+      if ((rc = dev_alloc(t, merge_scratch_bytes(R.cap), &m))) return fail(rc);
+      R.merge = merge_scratch_carve(m, R.cap);
+      if ((rc = launch_merge_reset(ctx, R.merge))) return fail(rc);
+    }
     // particle_container ctor: sparse <- -1 (tracking2/particle_container.hpp:23-31)
     if ((rc = fill_sparse(t, R))) return fail(rc);
     r /= 2;
@@ -254,18 +255,13 @@ static int update_level(cuimg_hip_tracker* t, int s, int slot) {
   }
 
   if (!(t->frame_cpt % R.sp.filtering_frequency) && R.sp.with_merge) {
-    merge_args m{};
+    merge_args m = R.merge;
     m.nr = R.nr;
     m.nc = R.nc;
     m.parts = R.parts[R.cur];
     m.sparse = sparse;
     m.sparse_pitch = R.sparse.pitch;
     m.cnt = t->cnt + s;
-    m.actors = R.actors;
-    m.zeroer = R.zeroer;
-    m.minunres = R.minunres;
-    m.resolved = R.resolved;
-    m.cap = R.cap;
     CUIMG_TRY(launch_merge(ctx, m));
   }
   return 0;
@@ -347,6 +343,7 @@ int cuimg_hip_tracker_clear(cuimg_hip_tracker* t) {
   t->frame_cpt = 0;
   for (int s = 0; s < t->L; s++) {
     CUIMG_TRY(fill_sparse(t, t->lv[s]));
+    CUIMG_TRY(launch_merge_reset(t->ctx, t->lv[s].merge));
     t->lv[s].compact_has_run = false;
   }
   CUIMG_HIP(t->ctx, hipMemsetAsync(t->cnt, 0, sizeof(level_counters) * t->L, t->ctx->stream));
@@ -460,10 +457,11 @@ int cuimg_hip_tracker_matcher_images(cuimg_hip_tracker* t, int scale, cuimg_img*
 int cuimg_hip_particles_merge(cuimg_hip_ctx* ctx, cuimg_particle_f* parts, uint32_t n, const cuimg_img* sparse) {
   if (!ctx || (!parts && n) || !sparse || sparse->dtype != CUIMG_U32)
     return set_error(ctx, CUIMG_E_INVALID, "particles_merge: bad arguments");
+  if (n > kMergeMaxCap)
+    return set_error(ctx, CUIMG_E_UNSUPPORTED, "particles_merge: %u particles exceed %u", n, kMergeMaxCap);
   const uint32_t cap = std::max<uint32_t>(n, 1);
   void *buf = nullptr, *cnt = nullptr;
-  const size_t bytes = (size_t)cap * (4 + 4 + 4 + 1);
-  CUIMG_HIP(ctx, hipMalloc(&buf, bytes));
+  CUIMG_HIP(ctx, hipMalloc(&buf, merge_scratch_bytes(cap)));
   if (hipMalloc(&cnt, sizeof(level_counters)) != hipSuccess) {
     (void)hipFree(buf);
     return set_error(ctx, CUIMG_E_NOMEM, "particles_merge: allocation failed");
@@ -473,19 +471,15 @@ int cuimg_hip_particles_merge(cuimg_hip_ctx* ctx, cuimg_particle_f* parts, uint3
   int rc = 0;
   if (hipMemcpyAsync(cnt, &c, sizeof c, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) rc = CUIMG_E_NOMEM;
   if (!rc) {
-    merge_args m{};
+    merge_args m = merge_scratch_carve(buf, cap);
     m.nr = sparse->nrows;
     m.nc = sparse->ncols;
     m.parts = parts;
     m.sparse = static_cast<const uint32_t*>(sparse->begin);
     m.sparse_pitch = sparse->pitch;
     m.cnt = static_cast<level_counters*>(cnt);
-    m.actors = static_cast<uint32_t*>(buf);
-    m.zeroer = reinterpret_cast<int32_t*>(m.actors + cap);
-    m.minunres = m.zeroer + cap;
-    m.resolved = reinterpret_cast<uint8_t*>(m.minunres + cap);
-    m.cap = cap;
-    rc = launch_merge(ctx, m);
+    rc = launch_merge_reset(ctx, m);
+    if (!rc) rc = launch_merge(ctx, m);
   }
   (void)hipStreamSynchronize(ctx->stream);
   (void)hipFree(buf);

@@ -85,17 +85,30 @@ int launch_fill_u8(cuimg_hip_ctx* ctx, uint8_t* begin, int pitch, int nr, int nc
 int launch_particle_mask(cuimg_hip_ctx* ctx, const detect_args& a);
 int launch_detect_and_compact(cuimg_hip_ctx* ctx, const detect_args& a);
 
+// A particle with at least one eligible merge target: b[k] is the c8
+// neighbour k (valid where bit k of mask is set).
+struct merge_actor {
+  int32_t i;
+  uint32_t mask;
+  int32_t b[8];
+};
+
 struct merge_args {
   int nr, nc;
   cuimg_particle_f* parts;
   const uint32_t* sparse; int sparse_pitch;
   level_counters* cnt;
-  uint32_t* actors;     // cap
-  int32_t* zeroer;      // cap
-  int32_t* minunres;    // cap
-  uint8_t* resolved;    // cap
-  uint32_t cap;
+  merge_actor* actors;       // cap
+  uint32_t* zeroer;          // cap; 0xFFFFFFFF between calls
+  uint32_t* minunres;        // cap; 0xFFFFFFFF between calls
+  unsigned long long* pend;  // pend_cap: open actors beyond the LDS list
+  uint32_t cap, pend_cap;    // pend_cap: power of two >= cap; cap < 2^24
 };
+constexpr uint32_t kMergeMaxCap = (1u << 24) - 1;
+// Bytes of merge scratch for a capacity (pend, actors, zeroer, minunres).
+size_t merge_scratch_bytes(uint32_t cap);
+merge_args merge_scratch_carve(void* base, uint32_t cap);
+int launch_merge_reset(cuimg_hip_ctx* ctx, const merge_args& a);
 int launch_merge(cuimg_hip_ctx* ctx, const merge_args& a);
 
 }  // namespace cuimg
