@@ -41,17 +41,16 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def lk_algorithmic_bytes(level_shapes, s, nparts):
-    """Compulsory bytes of one matcher launch at scale s: the previous and
-    current FP32 frames of scales s..L-1 once each, plus each particle read and
-    written (20 B each way)."""
-    img = sum(2 * 4 * r * c for (r, c) in level_shapes[s:])
-    return img + 40 * nparts
+def lk_algorithmic_bytes(level_shapes, nparts):
+    """Compulsory bytes of one matcher launch (all scales at once): the
+    previous and current FP32 frames of every scale read once, plus each
+    particle read and written (20 B each way)."""
+    img = sum(2 * 4 * r * c for (r, c) in level_shapes)
+    return img + 40 * sum(nparts)
 
 
-KERNELS = ("lk_track", "gaussian_sep", "pyramid_build", "fast_saliency", "fill_u32", "fill_u8",
-           "particle_mask", "nms_boxes", "subrow_count", "compact_count", "compact_scan",
-           "compact_scatter", "insert_new", "merge_scan", "merge_resolve")
+KERNELS = ("pyramid_build", "gaussian_sep", "fills", "lk_track", "particle_mask", "fast_saliency",
+           "detect_count", "compact_scan", "scatter_insert", "merge_scan", "merge_resolve")
 
 
 def upload_video(cu, ctx, nframes):
@@ -66,8 +65,10 @@ def upload_video(cu, ctx, nframes):
     return video, frames
 
 
-def tracker_bench(cu, ctx, frames, fast_th, warmup, steps, kernel_events):
-    """warmup untimed frames, then `steps` timed frame steps of one tracker."""
+def tracker_bench(cu, ctx, frames, fast_th, warmup, steps, prof_steps):
+    """warmup untimed frames, `steps` timed frame steps replayed from the
+    captured graph, then `prof_steps` more steps with plain launches and HIP
+    events around every kernel (the per-kernel breakdown)."""
     params = cu.profile_params("B", NSCALES, fast_th)
     trk = cu.Tracker(ctx, NR, NC, params)
     shapes = [(NR >> s, NC >> s) for s in range(NSCALES)]
@@ -75,9 +76,6 @@ def tracker_bench(cu, ctx, frames, fast_th, warmup, steps, kernel_events):
         trk.run(frames[f])
     ctx.sync()
     live0 = [trk.work(s)[0] for s in range(NSCALES)]
-    ctx.profile(kernel_events)
-    ctx.profile_reset()
-    ctx.sync()
     t0 = time.perf_counter()
     for f in range(warmup, warmup + steps):
         trk.run(frames[f])
@@ -86,15 +84,26 @@ def tracker_bench(cu, ctx, frames, fast_th, warmup, steps, kernel_events):
     live1 = [trk.work(s)[0] for s in range(NSCALES)]
     per_scale = [b - a for a, b in zip(live0, live1)]
     nparts = [trk.nparticles(s) for s in range(NSCALES)]
-    kernels = {}
-    for name in KERNELS:
-        n, ms = ctx.profile_query(name)
-        if n:
-            kernels[name] = {"launches": n, "total_ms": round(ms, 4), "avg_us": round(ms / n * 1000, 3)}
-    ctx.profile(False)
+    kernels, prof = {}, None
+    if prof_steps:
+        trk.set_graph(False)
+        ctx.profile(True)
+        ctx.profile_reset()
+        t1 = time.perf_counter()
+        for f in range(warmup + steps, warmup + steps + prof_steps):
+            trk.run(frames[f])
+        ctx.sync()
+        pdt = time.perf_counter() - t1
+        live2 = [trk.work(s)[0] for s in range(NSCALES)]
+        for name in KERNELS:
+            n, ms = ctx.profile_query(name)
+            if n:
+                kernels[name] = {"launches": n, "total_ms": round(ms, 4), "avg_us": round(ms / n * 1000, 3)}
+        ctx.profile(False)
+        prof = dict(frames=prof_steps, seconds=pdt, per_scale=[b - a for a, b in zip(live1, live2)])
     trk.close()
     return dict(seconds=dt, frames=steps, updates=sum(per_scale), per_scale=per_scale, nparts=nparts,
-                kernels=kernels, shapes=shapes, fast_th=fast_th)
+                kernels=kernels, shapes=shapes, fast_th=fast_th, prof=prof)
 
 
 def summary(tr):
@@ -193,21 +202,19 @@ def cpu_baseline(video, budget_s, fast_th):
 
 
 def lk_roofline(tr):
-    """Roofline entry of the matcher: algorithmic bytes of the timed frames'
-    matcher launches divided by their measured time."""
-    k = tr["kernels"].get("lk_track")
-    if not k:
+    """Roofline entry of the matcher: algorithmic bytes per launch (one launch
+    per frame step covers every scale) over its average measured duration."""
+    k, prof = tr["kernels"].get("lk_track"), tr["prof"]
+    if not k or not prof:
         return None
-    L = len(tr["shapes"])
-    avg_parts = [u / tr["frames"] for u in tr["per_scale"]]
-    per_frame = sum(lk_algorithmic_bytes(tr["shapes"], s, avg_parts[s]) for s in range(L))
-    bytes_per_launch = per_frame / L
+    avg_parts = [u / prof["frames"] for u in prof["per_scale"]]
+    bytes_per_launch = lk_algorithmic_bytes(tr["shapes"], avg_parts)
     achieved = bytes_per_launch / (k["avg_us"] * 1e-6) / 1e9
     return {"kernel": "lk_track", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
             "bytes_per_launch": int(bytes_per_launch), "avg_us": k["avg_us"],
-            "note": "matcher is gather/latency bound; bytes = both FP32 frames of scales >= s "
-                    "+ 40 B per particle, averaged over the 4 per-frame launches"}
+            "note": "matcher is gather/latency bound; bytes = both FP32 frames of every scale once "
+                    "+ 40 B per particle matched"}
 
 
 def main():
@@ -220,8 +227,8 @@ def main():
     ap.add_argument("--no-micro", action="store_true")
     ap.add_argument("--no-sparse", action="store_true", help="skip the FAST th 20 run")
     ap.add_argument("--fast-th", type=float, default=FAST_TH)
-    ap.add_argument("--kernel-events", type=int, default=1,
-                    help="record HIP events around every kernel in the timed region")
+    ap.add_argument("--prof-steps", type=int, default=30,
+                    help="frame steps after the timed region run with per-kernel HIP events")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -230,15 +237,15 @@ def main():
 
     import cuimg_amd as cu
     ctx = cu.Context(0)
-    video, frames = upload_video(cu, ctx, args.warmup + args.steps)
+    video, frames = upload_video(cu, ctx, args.warmup + args.steps + args.prof_steps)
     log(f"tracker benchmark, FAST th {args.fast_th:g}")
-    tr = tracker_bench(cu, ctx, frames, args.fast_th, args.warmup, args.steps, args.kernel_events)
+    tr = tracker_bench(cu, ctx, frames, args.fast_th, args.warmup, args.steps, args.prof_steps)
     log(f"  {tr['updates'] / tr['seconds']:.4g} particle-updates/s, {tr['frames'] / tr['seconds']:.1f} frames/s")
     sparse = None
     if not args.no_sparse:
         steps = min(args.steps, 120)
         log(f"tracker benchmark, FAST th {FAST_TH_SPARSE:g}")
-        sp = tracker_bench(cu, ctx, frames, FAST_TH_SPARSE, min(args.warmup, 30), steps, args.kernel_events)
+        sp = tracker_bench(cu, ctx, frames, FAST_TH_SPARSE, min(args.warmup, 30), steps, 0)
         sparse = summary(sp)
         log(f"  {sparse['particle_updates_per_s']:.4g} particle-updates/s, {sparse['frames_per_s']} frames/s")
     for im in frames:

@@ -35,6 +35,11 @@ class Img(ctypes.Structure):
                 ("pitch", ctypes.c_int32), ("border", ctypes.c_int32), ("dtype", ctypes.c_int32)]
 
 
+class LkParams(ctypes.Structure):
+    _fields_ = [("ws", ctypes.c_int32), ("max_iters", ctypes.c_int32), ("eps", ctypes.c_float),
+                ("top_n", ctypes.c_int32), ("k", ctypes.c_int32), ("template_u8", ctypes.c_int32)]
+
+
 class TrackerParams(ctypes.Structure):
     _fields_ = [("nscales", ctypes.c_int32), ("profile", ctypes.c_int32),
                 ("lk_ws", ctypes.c_int32), ("lk_max_iters", ctypes.c_int32),
@@ -122,6 +127,17 @@ def lib():
         L.cuimg_hip_tracker_frame_count.argtypes = [vp, P(c_int)]
         L.cuimg_hip_particles_merge.argtypes = [vp, vp, ctypes.c_uint32, P(Img)]
         L.cuimg_hip_tracker_work.argtypes = [vp, c_int, P(ctypes.c_uint64), P(ctypes.c_uint64)]
+        L.cuimg_hip_tracker_set_graph.argtypes = [vp, c_int]
+        L.cuimg_hip_lk_track.argtypes = [vp, P(Img), P(Img), c_int, c_int, vp, ctypes.c_uint32, P(LkParams),
+                                         P(Img), vp, vp, vp]
+        L.cuimg_hip_batch_create.argtypes = [vp, c_int, P(c_int), P(c_int), P(TrackerParams), P(vp)]
+        L.cuimg_hip_batch_destroy.argtypes = [vp]
+        L.cuimg_hip_batch_step.argtypes = [vp, P(Img)]
+        L.cuimg_hip_batch_set_graph.argtypes = [vp, c_int]
+        L.cuimg_hip_batch_tracker.argtypes = [vp, c_int, P(vp)]
+        L.cuimg_hip_batch_set_tuning.argtypes = [vp, c_int, c_int]
+        L.cuimg_hip_batch_get_tuning.argtypes = [vp, P(c_int), P(c_int)]
+        L.cuimg_hip_batch_clear.argtypes = [vp]
         L.cuimg_hip_lk_loss_adjoint.argtypes = [vp, P(Img), P(Img), vp, vp, ctypes.c_uint32, c_int, c_int, c_int,
                                                 vp, vp, vp, P(Img), vp, ctypes.c_size_t]
         _lib = L
@@ -315,22 +331,31 @@ class Tracker:
     """tracker<pyrlk>(domain, nscales) on the GPU (cuimg_hip_tracker).
 
     run(frame) is tracker::run; particles(s) / matches(s) / sparse(s) are
-    scale(s).pset().dense_particles() / matches() / sparse_particles()."""
+    scale(s).pset().dense_particles() / matches() / sparse_particles().  A
+    tracker made by Batch.tracker(i) is a view of one stream of a batch."""
 
-    def __init__(self, ctx, nrows, ncols, params):
+    def __init__(self, ctx, nrows, ncols, params, _view=None):
         self.ctx = ctx
         self.L = ctx.L
         self.params = params
+        self.nscales = params.nscales
+        if _view is not None:
+            self.h, self._owner = _view, False
+            return
         h = ctypes.c_void_p()
         ctx.check(self.L.cuimg_hip_tracker_create(ctx.h, nrows, ncols, ctypes.byref(params), ctypes.byref(h)),
                   "tracker_create")
-        self.h = h
-        self.nscales = params.nscales
+        self.h, self._owner = h, True
 
     def close(self):
         if getattr(self, "h", None):
-            self.L.cuimg_hip_tracker_destroy(self.h)
+            if self._owner:
+                self.L.cuimg_hip_tracker_destroy(self.h)
             self.h = None
+
+    def set_graph(self, on):
+        """Replay frame steps from a captured hipGraph (default) or launch them."""
+        self.ctx.check(self.L.cuimg_hip_tracker_set_graph(self.h, 1 if on else 0), "set_graph")
 
     def __del__(self):
         try:
@@ -426,6 +451,90 @@ class Tracker:
         v = ctypes.c_int()
         self.ctx.check(self.L.cuimg_hip_tracker_frame_count(self.h, ctypes.byref(v)), "frame_count")
         return v.value
+
+
+class Batch:
+    """Independent trackers stepped together (cuimg_hip_batch): shapes is a list
+    of (nrows, ncols), params a TrackerParams or one per stream."""
+
+    def __init__(self, ctx, shapes, params):
+        self.ctx = ctx
+        self.L = ctx.L
+        n = len(shapes)
+        plist = params if isinstance(params, (list, tuple)) else [params] * n
+        self.params = plist
+        rows = (ctypes.c_int * n)(*[r for r, _ in shapes])
+        cols = (ctypes.c_int * n)(*[c for _, c in shapes])
+        parr = (TrackerParams * n)(*plist)
+        h = ctypes.c_void_p()
+        ctx.check(self.L.cuimg_hip_batch_create(ctx.h, n, rows, cols, parr, ctypes.byref(h)), "batch_create")
+        self.h = h
+        self.n = n
+        self.shapes = list(shapes)
+        self._views = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cuimg_hip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, frames):
+        arr = (Img * self.n)(*[f.d for f in frames])
+        self.ctx.check(self.L.cuimg_hip_batch_step(self.h, arr), "batch_step")
+
+    def set_graph(self, on):
+        self.ctx.check(self.L.cuimg_hip_batch_set_graph(self.h, 1 if on else 0), "batch_set_graph")
+
+    def set_tuning(self, lk_div, blur_th=0):
+        self.ctx.check(self.L.cuimg_hip_batch_set_tuning(self.h, lk_div, blur_th), "batch_set_tuning")
+
+    def tuning(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self.ctx.check(self.L.cuimg_hip_batch_get_tuning(self.h, ctypes.byref(a), ctypes.byref(b)), "batch_get_tuning")
+        return a.value, b.value
+
+    def clear(self):
+        self.ctx.check(self.L.cuimg_hip_batch_clear(self.h), "batch_clear")
+
+    def tracker(self, i):
+        if i not in self._views:
+            h = ctypes.c_void_p()
+            self.ctx.check(self.L.cuimg_hip_batch_tracker(self.h, i, ctypes.byref(h)), "batch_tracker")
+            r, c = self.shapes[i]
+            self._views[i] = Tracker(self.ctx, r, c, self.params[i], _view=h)
+        return self._views[i]
+
+
+def lk_track(ctx, prev_levels, cur_levels, scale, parts, ws=7, max_iters=5, eps=0.1, top_n=49, k=300,
+             template_u8=0, sparse=None):
+    """pyrlk_match_particles_kernel for the particles of one scale: returns
+    (particles after the move, accept, translation, dist)."""
+    L = ctx.L
+    n = len(parts)
+    parts = np.ascontiguousarray(parts, PARTICLE_DTYPE)
+    prm = LkParams(ws, max_iters, eps, top_n, k, template_u8)
+    a = (Img * len(prev_levels))(*[x.d for x in prev_levels])
+    b = (Img * len(cur_levels))(*[x.d for x in cur_levels])
+    buf = DeviceBuffer.from_array(ctx, parts) if n else None
+    acc, tr, dist = (DeviceBuffer(ctx, max(4 * n, 4)), DeviceBuffer(ctx, max(8 * n, 8)),
+                     DeviceBuffer(ctx, max(4 * n, 4)))
+    try:
+        ctx.check(L.cuimg_hip_lk_track(ctx.h, a, b, len(prev_levels), scale, buf.p if buf else None, n,
+                                       ctypes.byref(prm), ctypes.byref(sparse.d) if sparse is not None else None,
+                                       acc.p, tr.p, dist.p), "lk_track")
+        out = buf.download(PARTICLE_DTYPE, n) if n else parts.copy()
+        return (out, acc.download(np.int32, n), tr.download(np.float32, 2 * n).reshape(n, 2),
+                dist.download(np.float32, n))
+    finally:
+        for x in (buf, acc, tr, dist):
+            if x is not None:
+                x.free()
 
 
 def particles_merge(ctx, parts, sparse):

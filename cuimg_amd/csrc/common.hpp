@@ -98,4 +98,16 @@ __device__ __forceinline__ const T* px(const void* begin, int pitch, int r, int 
 }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// A wave-uniform value moved to scalar registers (all lanes hold the same bits).
+template <typename T>
+__device__ __forceinline__ T uni(const T& v) {
+  static_assert(sizeof(T) % 4 == 0, "whole dwords");
+  T r;
+  const int* src = reinterpret_cast<const int*>(&v);
+  int* dst = reinterpret_cast<int*>(&r);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(T) / 4); k++) dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
+  return r;
+}
+
 }  // namespace cuimg

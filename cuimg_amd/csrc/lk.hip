@@ -1,456 +1,80 @@
-// Pyramid Lucas-Kanade particle matcher for gfx950.
-//
-// Reference: pyrlk_match_particles_kernel (tracking2/pyrlk_tracker.hpp:676-769)
-// with iterative_lk_match_5x5::match (:165-335), the bilinear sampler `li`
-// (:130-146), the Sobel/8 gradient of the previous frame (:909-938) and the
-// recursive coarse-to-fine prediction (predictions.h:60-74,
-// pyrlk_tracker.hpp:1025-1063, transformations.h:77-80).
-//
-// Design: one wavefront per particle; lane l owns window pixels l, l+64, ...
-// (raster order, NPL per lane).  A level solve is latency-bound, so it is built
-// around few memory round trips:
-//  * the previous frame's (ws+5)^2 patch around the particle and the current
-//    frame's (ws+1+2*kMB)^2 patch around the prediction are loaded into LDS in
-//    one batch of clamped reads;
-//  * the Sobel/8 gradient of the previous frame is computed for the (ws+3)^2
-//    inner pixels of its patch in LDS (zero on the image's outer ring), so no
-//    gradient image exists; template samples and gradients come from LDS;
-//  * Gauss-Newton steps sample the current-frame patch; it is reloaded only when
-//    the estimate leaves its kMB-pixel margin;
-//  * window sums are wave reductions (DPP within rows, two cross-row shuffles),
-//    bit-identical in every lane; the 2x2 solve runs redundantly per lane in
-//    FP32 registers, so all control flow is wave-uniform.
-// Sample rows/cols satisfy |tz(x + r) - (tz(x) + r)| <= 1 for float x and
-// integer r, which sizes the patches: every corner a lane reads lies inside.
-// No MFMA: nothing here is a dense contraction.  A particle at scale s first
-// re-solves its synthetic ancestors at scales L-1 .. s+1, as the reference's
-// recursion does, each solve feeding the next one's prediction.
-//
-// Numerics: per-sample arithmetic (bilinear weights, samples, Sobel) is written
-// without FMA contraction in the oracle's operation order, so template and
-// image samples are bit-identical to the CPU oracle; only the window sums are
-// reduced in tree order instead of raster order.
-#include "tracker_state.hpp"
+// Pyramid LK matcher: the standalone launch of one scale's particles
+// (pyrlk_match_particles_kernel, tracking2/pyrlk_tracker.hpp:676-769).  The
+// device code and its reference notes are in lk_dev.hpp; the tracker's frame
+// step runs the same body as one job of a batched launch.
+#include "lk_dev.hpp"
+
+#include <algorithm>
 
 namespace cuimg {
 
-__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
-__device__ __forceinline__ float add_rn(float a, float b) { return __fadd_rn(a, b); }
-__device__ __forceinline__ float sub_rn(float a, float b) { return __fsub_rn(a, b); }
-
-template <int CTRL>
-__device__ __forceinline__ float mov_dpp(float x) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, false));
-}
-
-// Sum over the 64 lanes, the same bits in every lane: every step pairs lanes
-// symmetrically (xor-1 and xor-2 quad permutes, half-row and row mirrors, then
-// xor 16 and xor 32), so both partners add the same two values.
-__device__ __forceinline__ float wave_sum(float x) {
-  x += mov_dpp<0xB1>(x);   // quad_perm [1,0,3,2]
-  x += mov_dpp<0x4E>(x);   // quad_perm [2,3,0,1]
-  x += mov_dpp<0x141>(x);  // row_half_mirror
-  x += mov_dpp<0x140>(x);  // row_mirror
-  x += __shfl_xor(x, 16, 64);
-  x += __shfl_xor(x, 32, 64);
-  return x;
-}
-
-__device__ __forceinline__ int tz(float v) { return (int)v; }
-
-__device__ __forceinline__ float norml2(float a, float b) {
-  return sqrtf(add_rn(mul_rn(a, a), mul_rn(b, b)));
-}
-
-// k / q for 0 <= k < 4096, q <= 64: (k + 0.5) / q is at least 0.5/q away from
-// an integer, far more than the float error of the product.
-__device__ __forceinline__ int divq(int k, float invq) { return (int)(((float)k + 0.5f) * invq); }
-
-template <typename TI>
-__device__ __forceinline__ float ld(const void* img, int pitch, int r, int c) {
-  return (float)*px<TI>(img, pitch, r, c);
-}
-
-// Bilinear weights of li: base = coordinates truncated toward zero.
-struct bw {
-  int xr, xc;
-  float w00, w10, w01, w11;
-};
-
-__device__ __forceinline__ bw weights(float qr, float qc) {
-  bw w;
-  w.xr = tz(qr);
-  w.xc = tz(qc);
-  const float a0 = sub_rn(qr, (float)w.xr), a1 = sub_rn(qc, (float)w.xc);
-  const float b0 = sub_rn(1.f, a0), b1 = sub_rn(1.f, a1);
-  w.w00 = mul_rn(b0, b1);
-  w.w10 = mul_rn(a0, b1);
-  w.w01 = mul_rn(b0, a1);
-  w.w11 = mul_rn(a0, a1);
-  return w;
-}
-
-// li's left-to-right sum (((w00 v00 + w10 v10) + w01 v01) + w11 v11).
-__device__ __forceinline__ float blend(const bw& w, float v00, float v10, float v01, float v11) {
-  float s = mul_rn(w.w00, v00);
-  s = add_rn(s, mul_rn(w.w10, v10));
-  s = add_rn(s, mul_rn(w.w01, v01));
-  s = add_rn(s, mul_rn(w.w11, v11));
-  return s;
-}
-
-struct match_out {
-  float2 tr;
-  float dist;
-};
-
-constexpr int kMB = 4;  // motion margin of the current-frame patch
-
-// LDS of one wave, sized for the largest window of its NPL class.
-template <int NPL>
-struct lk_lds {
-  static constexpr int WSMAX = NPL == 1 ? 7 : (NPL == 2 ? 11 : 15);
-  static constexpr int QA = WSMAX + 5;            // previous-frame patch side
-  static constexpr int QG = WSMAX + 3;            // gradient patch side
-  static constexpr int QB = WSMAX + 1 + 2 * kMB;  // current-frame patch side
-  float2 G[QG * QG];
-  float A[QA * QA];
-  float B[QB * QB];
-  float tpl[64 * NPL];
-  float mag[64 * NPL];
-};
-
-// dst[i*q + j] = img[clamp(r0 + i), clamp(c0 + j)]: indexing the patch with an
-// unclamped coordinate reproduces the reference's clamped read.
-template <typename TI>
-__device__ __forceinline__ void load_patch(float* dst, int q, float invq, const void* img, int pitch, int nr, int nc,
-                                           int r0, int c0, int lane) {
-  for (int k = lane; k < q * q; k += 64) {
-    const int i = divq(k, invq), j = k - i * q;
-    dst[k] = ld<TI>(img, pitch, clampi(r0 + i, 0, nr - 1), clampi(c0 + j, 0, nc - 1));
-  }
-}
-
-// Orders the wave's LDS writes before its later LDS reads (one wave, in-order
-// LDS queue: only the compiler has to be kept from moving accesses).
-__device__ __forceinline__ void wave_sync_lds() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Per-lane window geometry, fixed for a launch.
-template <int NPL>
-struct lane_win {
-  int r[NPL], c[NPL];
-  bool in[NPL];
-};
-
-// iterative_lk_match_5x5::match for one wave.
-template <typename TI, int NPL, bool SEL>
-__device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk_params_dev& P,
-                                                const lane_win<NPL>& W, float2 p, float2 pred_tr, lk_lds<NPL>& S) {
-  const match_out fail{make_float2(-1.f, -1.f), 1000.f};
-  const int lane = threadIdx.x & 63;
-  const int ws = P.ws, hws = ws / 2, n = ws * ws;
-  const int QA = ws + 5, QG = ws + 3, QB = ws + 1 + 2 * kMB;
-  const float iQA = 1.f / (float)QA, iQG = 1.f / (float)QG, iQB = 1.f / (float)QB;
-  const float2 prediction = make_float2(add_rn(p.x, pred_tr.x), add_rn(p.y, pred_tr.y));
-  float2 v = prediction;
-
-  // ---- both patches in one batch of loads
-  const int oAr = tz(p.x) - hws - 2, oAc = tz(p.y) - hws - 2;
-  int oBr = tz(v.x) - hws - kMB, oBc = tz(v.y) - hws - kMB;
-  wave_sync_lds();
-  load_patch<TI>(S.A, QA, iQA, L.A, L.pitchA, L.nr, L.nc, oAr, oAc, lane);
-  load_patch<TI>(S.B, QB, iQB, L.B, L.pitchB, L.nr, L.nc, oBr, oBc, lane);
-  wave_sync_lds();
-
-  // ---- Sobel/8 of the previous frame at global pixel (oA + 1 + (i, j));
-  // zero off the image's interior (the outer ring, and clamped reads of it)
-  for (int k = lane; k < QG * QG; k += 64) {
-    const int i = divq(k, iQG), j = k - i * QG;
-    const int yr = oAr + 1 + i, yc = oAc + 1 + j;
-    float2 g = make_float2(0.f, 0.f);
-    if (yr >= 1 && yr <= L.nr - 2 && yc >= 1 && yc <= L.nc - 2) {
-      const float* a = &S.A[(i + 1) * QA + (j + 1)];
-      const float pmm = a[-QA - 1], pm0 = a[-QA], pmp = a[-QA + 1];
-      const float p0m = a[-1], p0p = a[1];
-      const float ppm = a[QA - 1], pp0 = a[QA], ppp = a[QA + 1];
-      float gr = add_rn(ppm, mul_rn(2.f, pp0));
-      gr = add_rn(gr, ppp);
-      gr = sub_rn(gr, pmm);
-      gr = sub_rn(gr, mul_rn(2.f, pm0));
-      gr = sub_rn(gr, pmp);
-      float gc = add_rn(pmp, mul_rn(2.f, p0p));
-      gc = add_rn(gc, ppp);
-      gc = sub_rn(gc, pmm);
-      gc = sub_rn(gc, mul_rn(2.f, p0m));
-      gc = sub_rn(gc, ppm);
-      g = make_float2(mul_rn(gr, 0.125f), mul_rn(gc, 0.125f));
-    }
-    S.G[k] = g;
-  }
-  wave_sync_lds();
-
-  // ---- template: bilinear samples of A and of its gradient (pixels whose
-  // truncated coordinate is inside the image)
-  float2 gs[NPL];
-  float as[NPL];
-  bool sel[NPL];
-#pragma unroll
-  for (int k = 0; k < NPL; k++) {
-    const int i = lane + 64 * k;
-    gs[k] = make_float2(0.f, 0.f);
-    as[k] = 0.f;
-    sel[k] = false;
-    if (W.in[k]) {
-      const bw w = weights(add_rn(p.x, (float)W.r[k]), add_rn(p.y, (float)W.c[k]));
-      if (w.xr >= 0 && w.xr < L.nr && w.xc >= 0 && w.xc < L.nc) {
-        sel[k] = true;
-        const int ar = w.xr - oAr, ac = w.xc - oAc;  // in [1, ws+2]
-        const float* a = &S.A[ar * QA + ac];
-        const float av = blend(w, a[0], a[QA], a[1], a[QA + 1]);
-        as[k] = P.template_u8 ? (float)(uint8_t)(int)av : av;  // float -> uint8 through int
-        const float2* g = &S.G[(ar - 1) * QG + (ac - 1)];
-        const float2 g00 = g[0], g10 = g[QG], g01 = g[1], g11 = g[QG + 1];
-        gs[k] = make_float2(blend(w, g00.x, g10.x, g01.x, g11.x), blend(w, g00.y, g10.y, g01.y, g11.y));
-      }
-      S.tpl[i] = as[k];
-      if (SEL) S.mag[i] = sel[k] ? norml2(gs[k].x, gs[k].y) : -1.f;
-    }
-  }
-  wave_sync_lds();
-  if (SEL) {
-    // rank among valid pixels by (|g| desc, raster index asc); keep the top_n
-#pragma unroll
-    for (int k = 0; k < NPL; k++) {
-      const int i = lane + 64 * k;
-      if (sel[k]) {
-        const float m = S.mag[i];
-        int rank = 0;
-        for (int j = 0; j < n; j++) {
-          const float mj = S.mag[j];
-          rank += (mj > m) || (mj == m && j < i);
-        }
-        sel[k] = rank < P.top_n;
-      }
-    }
-  }
-
-  float g00 = 0.f, g01 = 0.f, g11 = 0.f, cnt = 0.f;
-#pragma unroll
-  for (int k = 0; k < NPL; k++)
-    if (sel[k]) {
-      g00 += gs[k].x * gs[k].x;
-      g01 += gs[k].x * gs[k].y;
-      g11 += gs[k].y * gs[k].y;
-      cnt += 1.f;
-    }
-  const float G00 = wave_sum(g00), G01 = wave_sum(g01), G11 = wave_sum(g11);
-  const float nsel = wave_sum(cnt);
-  const float G10 = G01;
-  {
-    const float h = mul_rn(0.5f, add_rn(G00, G11));
-    const float q = mul_rn(0.5f, sub_rn(G00, G11));
-    const float rr = sqrtf(add_rn(mul_rn(q, q), mul_rn(G01, G10)));
-    const float min_ev = fminf(fabsf(add_rn(h, rr)), fabsf(sub_rn(h, rr)));
-    if (min_ev < P.min_eig) return fail;
-  }
-  const float det = sub_rn(mul_rn(G00, G11), mul_rn(G10, G01));
-  const float invdet = 1.f / det;
-  const float I00 = mul_rn(G11, invdet), I01 = mul_rn(-G01, invdet);
-  const float I10 = mul_rn(-G10, invdet), I11 = mul_rn(G00, invdet);
-
-  // current-frame sample at v + (r, c) from the patch
-  auto sampleB = [&](int k) -> float {
-    const bw w = weights(add_rn(v.x, (float)W.r[k]), add_rn(v.y, (float)W.c[k]));
-    const float* b = &S.B[(w.xr - oBr) * QB + (w.xc - oBc)];
-    return blend(w, b[0], b[QB], b[1], b[QB + 1]);
-  };
-  // the window at v has all its corners inside the patch
-  auto ensure_covered = [&]() {
-    const int vr = tz(v.x), vc = tz(v.y);
-    if (vr - hws - 1 - oBr >= 0 && vr + hws + 2 - oBr <= QB - 1 && vc - hws - 1 - oBc >= 0 &&
-        vc + hws + 2 - oBc <= QB - 1)
-      return;
-    oBr = vr - hws - kMB;
-    oBc = vc - hws - kMB;
-    wave_sync_lds();
-    load_patch<TI>(S.B, QB, iQB, L.B, L.pitchB, L.nr, L.nc, oBr, oBc, lane);
-    wave_sync_lds();
-  };
-
-  float nk0 = 1.f, nk1 = 1.f;
-  const int rlo = 3, rhi = L.nr - 4, clo = 3, chi = L.nc - 4;
-  for (int it = 0; it < P.max_iters && sqrtf(add_rn(mul_rn(nk0, nk0), mul_rn(nk1, nk1))) >= P.eps; it++) {
-    ensure_covered();
-    float b0 = 0.f, b1 = 0.f;
-#pragma unroll
-    for (int k = 0; k < NPL; k++)
-      if (sel[k]) {
-        const float dt = sub_rn(as[k], sampleB(k));
-        b0 += gs[k].x * dt;
-        b1 += gs[k].y * dt;
-      }
-    b0 = wave_sum(b0);
-    b1 = wave_sum(b1);
-    nk0 = add_rn(mul_rn(I00, b0), mul_rn(I01, b1));
-    nk1 = add_rn(mul_rn(I10, b0), mul_rn(I11, b1));
-    v.x = add_rn(v.x, nk0);
-    v.y = add_rn(v.y, nk1);
-    if (norml2(sub_rn(prediction.x, v.x), sub_rn(prediction.y, v.y)) > (float)(ws * 4)) return fail;
-    const int vr = tz(v.x), vc = tz(v.y);
-    if (!(vr >= rlo && vr <= rhi && vc >= clo && vc <= chi)) return fail;
-  }
-
-  // error with the reference's index (r+hws)*hws + (c+hws) and counter
-  ensure_covered();
-  float e = 0.f;
-#pragma unroll
-  for (int k = 0; k < NPL; k++)
-    if (W.in[k]) {
-      const int idx = (W.r[k] + hws) * hws + (W.c[k] + hws);
-      e += fabsf(sub_rn(S.tpl[idx], sampleB(k)));
-    }
-  const float err = wave_sum(e);
-  match_out o;
-  o.tr = make_float2(sub_rn(v.x, p.x), sub_rn(v.y, p.y));
-  o.dist = err / (nsel + (float)n);
-  return o;
-}
-
-// One particle per wave.  The particle at scale s and its synthetic age-1
-// ancestors at p/2, p/4, ... are solved in one loop from the coarsest scale
-// down (recursive_prediction / get_flow_at): an ancestor whose prediction
-// leaves domain - border(3) contributes the identity, so does a match ending
-// outside its level's domain, and a failed match (-1,-1) is kept; without a
-// coarser scale the prediction is the speed (zero at age 1).
 template <typename TI, int NPL, bool SEL>
 __global__ __launch_bounds__(256) void lk_kernel(lk_args a) {
-  __shared__ lk_lds<NPL> lds_all[4];
-  __shared__ uint32_t blk_live, blk_acc;
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  lk_lds<NPL>& S = lds_all[wave];
-  const uint32_t n = a.cnt->n;
-  const int s = a.s, nlev = a.nlevels;
-  if (blockIdx.x == 0 && threadIdx.x == 0) a.cnt->n_matched = n;
-  if (threadIdx.x == 0) {
-    blk_live = 0;
-    blk_acc = 0;
-  }
-  lane_win<NPL> W;
-  {
-    const int ws = a.P.ws, hws = ws / 2;
-    const float iws = 1.f / (float)ws;
-#pragma unroll
-    for (int k = 0; k < NPL; k++) {
-      const int i = lane + 64 * k;
-      const int q = divq(i, iws);
-      W.in[k] = i < ws * ws;
-      W.r[k] = q - hws;
-      W.c[k] = i - q * ws - hws;
-    }
-  }
-  __syncthreads();
-  for (uint32_t i = blockIdx.x * 4 + wave; i < n; i += gridDim.x * 4) {
-    cuimg_particle_f part = a.parts[i];
-    int accept = 0;
-    float2 tr = make_float2(0.f, 0.f);
-    float dist = 0.f;
-    const float2 pos = make_float2(part.pos_r, part.pos_c);
-    if (part.age > 0 && tz(pos.x) >= a.pd_r1 && tz(pos.x) <= a.pd_r2 && tz(pos.y) >= a.pd_c1 &&
-        tz(pos.y) <= a.pd_c2) {
-      float2 flow = make_float2(0.f, 0.f);
-      for (int j = s + 1 < nlev ? nlev - 1 : s; j >= s; j--) {
-        const lk_level_view& lv = a.lv[j];
-        float2 p, ptr;
-        bool run;
-        if (j > s) {
-          // pos / 2^(j-s): exact halvings
-          p = make_float2(ldexpf(pos.x, s - j), ldexpf(pos.y, s - j));
-          ptr = j == nlev - 1 ? make_float2(0.f, 0.f) : make_float2(2.f * flow.x, 2.f * flow.y);
-          const int pr = tz(add_rn(p.x, ptr.x)), pc = tz(add_rn(p.y, ptr.y));
-          run = pr >= 3 && pr <= lv.nr - 4 && pc >= 3 && pc <= lv.nc - 4;
-        } else {
-          p = pos;
-          ptr = s + 1 < nlev ? make_float2(2.f * flow.x, 2.f * flow.y)
-                             : (part.age == 1 ? make_float2(0.f, 0.f) : make_float2(part.speed_r, part.speed_c));
-          const int pr = tz(add_rn(p.x, ptr.x)), pc = tz(add_rn(p.y, ptr.y));
-          run = pr >= a.pd_r1 && pr <= a.pd_r2 && pc >= a.pd_c1 && pc <= a.pd_c2;
-        }
-        if (!run) {
-          flow = make_float2(0.f, 0.f);
-          continue;
-        }
-        const match_out m = wave_match<TI, NPL, SEL>(lv, a.P, W, p, ptr, S);
-        const int mr = tz(add_rn(p.x, m.tr.x)), mc = tz(add_rn(p.y, m.tr.y));
-        if (j > s) {
-          flow = (mr >= 0 && mr < lv.nr && mc >= 0 && mc < lv.nc) ? m.tr : make_float2(0.f, 0.f);
-        } else {
-          const float2 pred = make_float2(add_rn(p.x, ptr.x), add_rn(p.y, ptr.y));
-          tr = m.tr;
-          dist = m.dist;
-          if (mr >= a.pd_r1 && mr <= a.pd_r2 && mc >= a.pd_c1 && mc <= a.pd_c2 && dist < (float)a.P.k &&
-              norml2(sub_rn((float)mr, pred.x), sub_rn((float)mc, pred.y)) < 160.f)
-            accept = 1;
-        }
-      }
-    }
-    if (lane == 0) {
-      if (part.age > 0) atomicAdd(&blk_live, 1u);
-      if (accept) atomicAdd(&blk_acc, 1u);
-      a.dec_accept[i] = accept;
-      a.dec_tr[i] = tr;
-      a.dec_dist[i] = dist;
-      if (accept) {
-        part.age++;
-        part.speed_r = tr.x;
-        part.speed_c = tr.y;
-        part.pos_r = add_rn(part.pos_r, tr.x);
-        part.pos_c = add_rn(part.pos_c, tr.y);
-        // kernel_particle_container::move: the last writer in index order wins
-        atomicMax(reinterpret_cast<int*>(px<uint32_t>(a.sparse, a.sparse_pitch, tz(part.pos_r), tz(part.pos_c))),
-                  (int)i);
-      } else {
-        part.age = 0;
-      }
-      a.parts[i] = part;
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0 && (blk_live | blk_acc)) {
-    atomicAdd(&a.cnt->live_total, (unsigned long long)blk_live);
-    atomicAdd(&a.cnt->accepted_total, (unsigned long long)blk_acc);
-  }
-}
-
-template <typename TI>
-static int launch_ti(cuimg_hip_ctx* ctx, const lk_args& a, int blocks) {
-  const int n = a.P.ws * a.P.ws;
-  const bool sel = a.P.top_n < n;
-  if (a.P.ws <= lk_lds<1>::WSMAX) {
-    if (sel) lk_kernel<TI, 1, true><<<blocks, 256, 0, ctx->stream>>>(a);
-    else lk_kernel<TI, 1, false><<<blocks, 256, 0, ctx->stream>>>(a);
-  } else if (a.P.ws <= lk_lds<2>::WSMAX) {
-    if (sel) lk_kernel<TI, 2, true><<<blocks, 256, 0, ctx->stream>>>(a);
-    else lk_kernel<TI, 2, false><<<blocks, 256, 0, ctx->stream>>>(a);
-  } else {
-    if (sel) lk_kernel<TI, 4, true><<<blocks, 256, 0, ctx->stream>>>(a);
-    else lk_kernel<TI, 4, false><<<blocks, 256, 0, ctx->stream>>>(a);
-  }
-  return 0;
-}
-
-int launch_lk(cuimg_hip_ctx* ctx, const lk_args& a, int image_u8) {
-  if (a.P.ws < 1 || a.P.ws > lk_lds<4>::WSMAX || (a.P.ws & 1) == 0)
-    return set_error(ctx, CUIMG_E_UNSUPPORTED, "lk: window %d not supported (odd, <= 15)", a.P.ws);
-  const int blocks = 2048;  // 8192 waves, grid-stride over the device-side count
-  return launch(ctx, "lk_track", [&] {
-    if (image_u8) launch_ti<uint8_t>(ctx, a, blocks);
-    else launch_ti<float>(ctx, a, blocks);
-  });
+  lk_body<TI, NPL, SEL>(a, blockIdx.x, gridDim.x);
 }
 
 }  // namespace cuimg
+
+using namespace cuimg;
+
+extern "C" int cuimg_hip_lk_track(cuimg_hip_ctx* ctx, const cuimg_img* prev, const cuimg_img* cur, int nlevels,
+                                  int scale, cuimg_particle_f* parts, uint32_t n, const cuimg_lk_params* p,
+                                  const cuimg_img* sparse, int32_t* accept, float* tr2, float* dist) {
+  if (!ctx || !prev || !cur || !p || (n && !parts)) return CUIMG_E_INVALID;
+  if (nlevels < 1 || nlevels > kMaxLevels || scale < 0 || scale >= nlevels)
+    return set_error(ctx, CUIMG_E_INVALID, "lk_track: %d levels, scale %d", nlevels, scale);
+  if (p->ws < 1 || p->ws > 15 || !(p->ws & 1) || p->max_iters < 0 || p->top_n < 1)
+    return set_error(ctx, CUIMG_E_UNSUPPORTED, "lk_track: window %d must be odd and <= 15", p->ws);
+  const int dt = prev[0].dtype;
+  if (dt != CUIMG_U8 && dt != CUIMG_F32) return set_error(ctx, CUIMG_E_INVALID, "lk_track: images must be U8 or F32");
+  lk_args a{};
+  for (int j = 0; j < nlevels; j++) {
+    const cuimg_img& A = prev[j];
+    const cuimg_img& B = cur[j];
+    if (A.dtype != dt || B.dtype != dt || A.nrows != B.nrows || A.ncols != B.ncols || !A.begin || !B.begin ||
+        A.nrows < 1 || A.ncols < 1)
+      return set_error(ctx, CUIMG_E_INVALID, "lk_track: level %d images differ or are empty", j);
+    a.lv[j] = lk_level_view{A.begin, B.begin, A.pitch, B.pitch, A.nrows, A.ncols};
+  }
+  const cuimg_img& S = prev[scale];
+  if (sparse && (sparse->dtype != CUIMG_U32 || sparse->nrows != S.nrows || sparse->ncols != S.ncols))
+    return set_error(ctx, CUIMG_E_INVALID, "lk_track: sparse image must be U32 of the scale's domain");
+  if (n == 0) return 0;
+  a.nlevels = nlevels;
+  a.s = scale;
+  a.parts = parts;
+  a.sparse = sparse ? static_cast<uint32_t*>(sparse->begin) : nullptr;
+  a.sparse_pitch = sparse ? sparse->pitch : 0;
+  a.pd_r1 = 3;
+  a.pd_c1 = 3;
+  a.pd_r2 = S.nrows - 4;
+  a.pd_c2 = S.ncols - 4;
+  a.P = lk_params_dev{p->ws, p->max_iters, std::min(p->top_n, p->ws * p->ws), p->eps, 1e-4f, p->k, p->template_u8};
+  // counters and, when not given, decision buffers
+  const size_t need = sizeof(level_counters) + (accept ? 0 : (size_t)n * 16);
+  void* tmp = nullptr;
+  CUIMG_HIP(ctx, hipMalloc(&tmp, need));
+  level_counters c{};
+  c.n = n;
+  a.cnt = static_cast<level_counters*>(tmp);
+  char* q = static_cast<char*>(tmp) + sizeof(level_counters);
+  a.dec_tr = accept ? reinterpret_cast<float2*>(tr2) : reinterpret_cast<float2*>(q);
+  a.dec_accept = accept ? accept : reinterpret_cast<int32_t*>(q + 8 * (size_t)n);
+  a.dec_dist = accept ? dist : reinterpret_cast<float*>(q + 12 * (size_t)n);
+  int rc = 0;
+  if (accept && (!tr2 || !dist)) rc = set_error(ctx, CUIMG_E_INVALID, "lk_track: accept, tr2 and dist go together");
+  if (!rc && hipMemcpyAsync(a.cnt, &c, sizeof c, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+    rc = set_error(ctx, CUIMG_E_NOMEM, "lk_track: counter upload failed");
+  if (!rc) {
+    const int blocks = (int)std::max<uint32_t>(1, std::min<uint32_t>(2048, (n + 15) / 16));
+    rc = launch(ctx, "lk_track", [&] {
+      lk_variant(a.P.ws, a.P.top_n, dt == CUIMG_U8, [&](auto ti, auto npl, auto sel) {
+        using TI = typename decltype(ti)::type;
+        lk_kernel<TI, decltype(npl)::value, decltype(sel)::value><<<blocks, 256, 0, ctx->stream>>>(a);
+      });
+    });
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(tmp);
+  return rc;
+}

@@ -1,78 +1,11 @@
-// Particle container for gfx950: detector mask, box non-maximum selection,
-// raster-order insertion, stream compaction and the trajectory merge.
-//
-// Reference semantics (tracking2/):
-//  * create_detector_mask + particle_mask_kernel (pyrlk_tracker.hpp:879-888,
-//    tracking_strategies.hpp:201-225): mask <- 255, then 0 on the 3x3 block of
-//    every particle of the dense vector, dead ones included;
-//  * fast_detector::new_particles(cpu) (fast_detector.hpp:217-262): one box per
-//    cell of domain/box at p*box + (1,1); skipped if any pixel holds a particle;
-//    first strict maximum in raster order; rejected below the float threshold;
-//    particles appended in raster order over domain - border(3)
-//    (particle_container::add, particle_container.hpp:370-387);
-//  * particle_container::compact(cpu) (particle_container.hpp:216-252): keep
-//    age != 0 in order, matches[old] = new (or -1), sparse(pos) = new index with
-//    the last particle of a pixel winning;
-//  * merge_trajectories (merge_trajectories.h:10-35): in ascending index order a
-//    live particle removes the first c8 neighbour (CPU table order,
-//    neighb2d_data.h:80-85) more than 2 frames older whose speed differs by
-//    less than 2 px.
-//
-// Raster order on the GPU: a box row covers `box` image rows, so the raster
-// order of new points is (box row, row inside the box, box column).  Each
-// (box row, row offset) "sub-row" is counted by one wave, one scan over
-// sub-rows gives the offsets, and one wave per sub-row writes its points with
-// a ballot prefix.  Compaction is a classic count / scan / scatter over
-// 1024-particle blocks.  The serial merge is resolved exactly in parallel
-// rounds and a serial replay of the few actors they leave open.
-#include "tracker_state.hpp"
-
-#include <climits>
+// Particle container for gfx950: standalone launches of the container and
+// merge kernels (the batched frame step lives in tracker.hip).
+#include "particles_dev.hpp"
 
 namespace cuimg {
 
-__constant__ int8_t kC8R[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
-__constant__ int8_t kC8C[8] = {1, 1, 1, 0, 0, -1, -1, -1};
-
-constexpr uint32_t kEmpty = 0xFFFFFFFFu;
-
-__device__ __forceinline__ int tzf(float v) { return (int)v; }
-
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    uint32_t y = __shfl_up(x, d, 64);
-    if (lane >= d) x += y;
-  }
-  return x;
-}
-
-// Exclusive scan of one value per thread over the workgroup (blockDim <= 1024).
-__device__ uint32_t block_excl_scan(uint32_t x, uint32_t* tmp, uint32_t* total) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-  uint32_t inc = wave_incl_scan(x);
-  if (lane == 63) tmp[wave] = inc;
-  __syncthreads();
-  if (wave == 0) {
-    uint32_t w = lane < nw ? tmp[lane] : 0;
-    uint32_t wi = wave_incl_scan(w);
-    if (lane < nw) tmp[32 + lane] = wi - w;
-    if (lane == nw - 1) tmp[31] = wi;
-  }
-  __syncthreads();
-  uint32_t r = inc - x + tmp[32 + wave];
-  if (total) *total = tmp[31];
-  __syncthreads();
-  return r;
-}
-
-// ------------------------------------------------------------------ fills
 __global__ void fill_u32_kernel(uint32_t* begin, int pitch, int r0, int r1, int c0, int c1, uint32_t v) {
-  const int w = c1 - c0;
-  const long total = (long)(r1 - r0) * w;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x)
-    *px<uint32_t>(begin, pitch, r0 + (int)(i / w), c0 + (int)(i % w)) = v;
+  fill_u32_body(begin, pitch, r0, r1, c0, c1, v, blockIdx.x, gridDim.x);
 }
 
 int launch_fill_u32(cuimg_hip_ctx* ctx, uint32_t* begin, int pitch, int r0, int r1, int c0, int c1, uint32_t v) {
@@ -82,305 +15,6 @@ int launch_fill_u32(cuimg_hip_ctx* ctx, uint32_t* begin, int pitch, int r0, int 
   return launch(ctx, "fill_u32", [&] {
     fill_u32_kernel<<<blocks, 256, 0, ctx->stream>>>(begin, pitch, r0, r1, c0, c1, v);
   });
-}
-
-// u8 domain fill with 4-byte stores (rows are 4-byte aligned at column 0).
-__global__ void fill_u8_kernel(uint8_t* begin, int pitch, int nr, int nc, uint32_t v4) {
-  const int wpr = (nc + 3) / 4;
-  const long total = (long)nr * wpr;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
-    const int r = (int)(i / wpr), w = (int)(i % wpr);
-    uint8_t* p = px<uint8_t>(begin, pitch, r, 4 * w);
-    if (4 * w + 4 <= nc) {
-      *reinterpret_cast<uint32_t*>(p) = v4;
-    } else {
-      for (int k = 0; 4 * w + k < nc; k++) p[k] = (uint8_t)v4;
-    }
-  }
-}
-
-int launch_fill_u8(cuimg_hip_ctx* ctx, uint8_t* begin, int pitch, int nr, int nc, uint8_t v) {
-  const long total = (long)nr * ((nc + 3) / 4);
-  if (total <= 0) return 0;
-  const int blocks = (int)std::min<long>((total + 255) / 256, 8192);
-  const uint32_t v4 = 0x01010101u * v;
-  return launch(ctx, "fill_u8", [&] { fill_u8_kernel<<<blocks, 256, 0, ctx->stream>>>(begin, pitch, nr, nc, v4); });
-}
-
-// ------------------------------------------------------------ detector mask
-__global__ void particle_mask_kernel(detect_args a) {
-  const uint32_t n = a.cnt->n;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int r = tzf(a.parts[i].pos_r), c = tzf(a.parts[i].pos_c);
-    for (int dr = -1; dr <= 1; dr++)
-      for (int dc = -1; dc <= 1; dc++) {
-        const int rr = r + dr, cc = c + dc;
-        if (rr >= 0 && rr < a.nr && cc >= 0 && cc < a.nc) *px<uint8_t>(a.mask, a.mask_pitch, rr, cc) = 0;
-      }
-  }
-}
-
-int launch_particle_mask(cuimg_hip_ctx* ctx, const detect_args& a) {
-  CUIMG_TRY(launch_fill_u8(ctx, a.mask, a.mask_pitch, a.nr, a.nc, 255));
-  const int blocks = (int)std::min<uint32_t>((a.cap + 255) / 256, 2048);
-  return launch(ctx, "particle_mask", [&] { particle_mask_kernel<<<blocks, 256, 0, ctx->stream>>>(a); });
-}
-
-// ---------------------------------------------------------- box selection
-__device__ __forceinline__ uint32_t sparse_get(const uint32_t* sp, int pitch, int nr, int nc, int r, int c) {
-  if (r < 0 || r >= nr || c < 0 || c >= nc) return kEmpty;
-  return *px<uint32_t>(sp, pitch, r, c);
-}
-
-// One workgroup per box row; each thread one box.  code = dr*box + dc of the
-// selected point, 0xFF when the box yields nothing.
-__global__ __launch_bounds__(256) void nms_kernel(detect_args a) {
-  const int bi = blockIdx.x;
-  const int B = a.box;
-  for (int bj = threadIdx.x; bj < a.nbc; bj += blockDim.x) {
-    const int pr = bi * B + 1, pc = bj * B + 1;
-    uint8_t code = 0xFF;
-    bool skip = false;
-    for (int r = 0; r < B && !skip; r++)
-      for (int c = 0; c < B; c++)
-        if (pr + r < a.nr && pc + c < a.nc &&
-            *px<uint32_t>(a.sparse, a.sparse_pitch, pr + r, pc + c) != kEmpty) {
-          skip = true;
-          break;
-        }
-    if (!skip) {
-      // the saliency border is 0 (its memset), reached only when box == 1
-      float vmax = (pr < a.nr && pc < a.nc) ? (float)*px<uint8_t>(a.sal, a.sal_pitch, pr, pc) : 0.f;
-      int mr = pr, mc = pc;
-      for (int r = 0; r < B; r++)
-        for (int c = 0; c < B; c++)
-          if (pr + r < a.nr && pc + c < a.nc) {
-            const float sv = (float)*px<uint8_t>(a.sal, a.sal_pitch, pr + r, pc + c);
-            if (vmax < sv) {
-              vmax = sv;
-              mr = pr + r;
-              mc = pc + c;
-            }
-          }
-      if (!(vmax < a.fast_th) && mr >= 3 && mr <= a.nr - 4 && mc >= 3 && mc <= a.nc - 4)
-        code = (uint8_t)((mr - pr) * B + (mc - pc));
-    }
-    a.boxcode[(size_t)bi * a.nbc + bj] = code;
-  }
-}
-
-// One wave per sub-row (box row bi, row offset dr): number of points.
-__global__ __launch_bounds__(256) void subrow_count_kernel(detect_args a) {
-  const int sub = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (sub >= a.nbr * a.box) return;
-  const int bi = sub / a.box, dr = sub % a.box;
-  uint32_t count = 0;
-  for (int b0 = 0; b0 < a.nbc; b0 += 64) {
-    const int bj = b0 + lane;
-    const uint8_t code = bj < a.nbc ? a.boxcode[(size_t)bi * a.nbc + bj] : 0xFF;
-    const bool hit = code != 0xFF && code / a.box == dr;
-    count += __popcll(__ballot(hit));
-  }
-  if (lane == 0) a.subcnt[sub] = count;
-}
-
-// Compaction count: flags (alive, owner of its sparse pixel) and alive count
-// per 1024-particle block.  Ownership is read before any index is rewritten.
-__global__ __launch_bounds__(256) void compact_count_kernel(detect_args a) {
-  __shared__ uint32_t tmp[64];
-  const uint32_t n = a.cnt->n;
-  const uint32_t base = blockIdx.x * 1024 + threadIdx.x * 4;
-  uint32_t c = 0;
-  for (int k = 0; k < 4; k++) {
-    const uint32_t i = base + k;
-    if (i < n) {
-      const cuimg_particle_f p = a.parts[i];
-      uint8_t f = 0;
-      if (p.age != 0) {
-        f = 1;
-        const uint32_t s = sparse_get(a.sparse, a.sparse_pitch, a.nr, a.nc, tzf(p.pos_r), tzf(p.pos_c));
-        if (s == i) f |= 2;
-        c++;
-      }
-      a.flags[i] = f;
-    }
-  }
-  uint32_t total;
-  block_excl_scan(c, tmp, &total);
-  if (threadIdx.x == 0) a.blockcnt[blockIdx.x] = total;
-}
-
-// Single workgroup: exclusive scans of the block counts and sub-row counts,
-// then the container counters.
-__global__ __launch_bounds__(1024) void compact_scan_kernel(detect_args a) {
-  __shared__ uint32_t tmp[64];
-  __shared__ uint32_t carry;
-  uint32_t sums[2];
-  for (int pass = 0; pass < 2; pass++) {
-    uint32_t* arr = pass == 0 ? a.blockcnt : a.subcnt;
-    const int len = pass == 0 ? a.nblocks : a.nbr * a.box;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int b0 = 0; b0 < len; b0 += blockDim.x) {
-      const int i = b0 + threadIdx.x;
-      const uint32_t x = i < len ? arr[i] : 0;
-      uint32_t total;
-      const uint32_t ex = block_excl_scan(x, tmp, &total);
-      if (i < len) arr[i] = carry + ex;
-      __syncthreads();
-      if (threadIdx.x == 0) carry += total;
-      __syncthreads();
-    }
-    sums[pass] = carry;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    level_counters& C = *a.cnt;
-    const uint32_t n_old = C.n, n_live = sums[0], n_new = sums[1];
-    C.n_old = n_old;
-    C.n_live = n_live;
-    C.n_new = n_new;
-    uint32_t total = n_live + n_new;
-    if (total > a.cap) {
-      C.overflow = 1;
-      total = a.cap;
-    }
-    C.n = total;
-  }
-}
-
-__global__ __launch_bounds__(256) void compact_scatter_kernel(detect_args a) {
-  __shared__ uint32_t tmp[64];
-  const uint32_t n_old = a.cnt->n_old;
-  const uint32_t base = blockIdx.x * 1024 + threadIdx.x * 4;
-  uint32_t c = 0;
-  uint8_t f[4];
-  for (int k = 0; k < 4; k++) {
-    const uint32_t i = base + k;
-    f[k] = i < n_old ? a.flags[i] : 0;
-    c += f[k] & 1;
-  }
-  uint32_t off = a.blockcnt[blockIdx.x] + block_excl_scan(c, tmp, nullptr);
-  for (int k = 0; k < 4; k++) {
-    const uint32_t i = base + k;
-    if (i >= n_old) break;
-    if (f[k] & 1) {
-      const cuimg_particle_f p = a.parts[i];
-      a.parts_out[off] = p;
-      a.matches[i] = off;
-      if (f[k] & 2) *px<uint32_t>(a.sparse, a.sparse_pitch, tzf(p.pos_r), tzf(p.pos_c)) = off;
-      off++;
-    } else {
-      a.matches[i] = kEmpty;
-    }
-  }
-}
-
-// One wave per sub-row: appends its points in column order.
-__global__ __launch_bounds__(256) void insert_new_kernel(detect_args a) {
-  const int sub = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (sub >= a.nbr * a.box) return;
-  const level_counters& C = *a.cnt;
-  const uint32_t n_live = C.n_live, n_old = C.n_old, n = C.n;
-  const int bi = sub / a.box, dr = sub % a.box;
-  uint32_t k = a.subcnt[sub];  // rank among new points
-  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (int b0 = 0; b0 < a.nbc; b0 += 64) {
-    const int bj = b0 + lane;
-    const uint8_t code = bj < a.nbc ? a.boxcode[(size_t)bi * a.nbc + bj] : 0xFF;
-    const bool hit = code != 0xFF && code / a.box == dr;
-    const uint64_t m = __ballot(hit);
-    if (hit) {
-      const uint32_t kk = k + (uint32_t)__popcll(m & below);
-      const uint32_t j = n_live + kk;
-      if (j < n) {
-        const int r = bi * a.box + 1 + dr, c = bj * a.box + 1 + code % a.box;
-        cuimg_particle_f p;
-        p.speed_r = 0.f;
-        p.speed_c = 0.f;
-        p.pos_r = (float)r;
-        p.pos_c = (float)c;
-        p.age = 1;
-        p.pad = 0;
-        a.parts_out[j] = p;
-        *px<uint32_t>(a.sparse, a.sparse_pitch, r, c) = j;
-        a.matches[n_old + kk] = j;
-      } else {
-        a.matches[n_old + kk] = kEmpty;
-      }
-    }
-    k += (uint32_t)__popcll(m);
-  }
-}
-
-int launch_detect_and_compact(cuimg_hip_ctx* ctx, const detect_args& a) {
-  const int nsub = a.nbr * a.box;
-  if (a.nbr > 0 && a.nbc > 0) {
-    CUIMG_TRY(launch(ctx, "nms_boxes", [&] { nms_kernel<<<a.nbr, 256, 0, ctx->stream>>>(a); }));
-    CUIMG_TRY(launch(ctx, "subrow_count",
-                     [&] { subrow_count_kernel<<<ceil_div(nsub, 4), 256, 0, ctx->stream>>>(a); }));
-  }
-  CUIMG_TRY(launch(ctx, "compact_count", [&] { compact_count_kernel<<<a.nblocks, 256, 0, ctx->stream>>>(a); }));
-  CUIMG_TRY(launch(ctx, "compact_scan", [&] { compact_scan_kernel<<<1, 1024, 0, ctx->stream>>>(a); }));
-  CUIMG_TRY(launch(ctx, "compact_scatter", [&] { compact_scatter_kernel<<<a.nblocks, 256, 0, ctx->stream>>>(a); }));
-  if (nsub > 0)
-    CUIMG_TRY(launch(ctx, "insert_new", [&] { insert_new_kernel<<<ceil_div(nsub, 4), 256, 0, ctx->stream>>>(a); }));
-  return 0;
-}
-
-// ------------------------------------------------------------------ merge
-//
-// Serial semantics, resolved in parallel rounds.  With the ages before the
-// merge, the edge i -> b ("i may remove b") is static: b is a c8 neighbour
-// (through the sparse image) with age_b > age_i + 2 and |speed_i - speed_b| < 2.
-// In index order, i acts iff nobody earlier removed it, and then removes the
-// first of its targets that nobody earlier removed.  Only actors targeting a
-// particle can remove it.  A round resolves every open actor i whose outcome
-// no open actor j < i can change: no open j < i targets i, and no open j < i
-// targets one of i's targets up to the first one still alive at i's turn
-// (zeroer[b] = index of b's remover; b is dead at i's turn iff zeroer[b] < i).
-// The smallest open actor always resolves, and on tracking data each round
-// leaves about a quarter of the open actors.  Once at most 64 remain (or after
-// kMaxRounds rounds) the rest are sorted by index and replayed serially by one
-// wave: a removal is visible to a later actor through zeroer (removals by
-// resolved actors and by earlier chunks) or through a ballot over the victims
-// of the earlier lanes of its chunk.
-//
-// minunres[b] = (255 - round) << 24 | smallest open actor targeting b, written
-// by atomicMin: a later round's tag is smaller, so stale entries never win and
-// no reset pass is needed.  Particle indices stay below 2^24.  Between calls
-// zeroer and minunres hold 0xFFFFFFFF; the resolve kernel restores the entries
-// it touched.
-
-constexpr uint32_t kNoIdx = 0xFFFFFFFFu;
-constexpr uint32_t kPendLds = 4096;
-constexpr uint32_t kMaxRounds = 250;
-constexpr uint32_t kSerialSwitch = 64;
-
-__device__ __forceinline__ uint32_t ld32(const uint32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st32(uint32_t* p, uint32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned long long ld64(const unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st64(unsigned long long* p, unsigned long long v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint32_t round_tag(uint32_t r) { return (255u - r) << 24; }
-
-__device__ __forceinline__ float norml2_rn(float a, float b) {
-  return sqrtf(__fadd_rn(__fmul_rn(a, a), __fmul_rn(b, b)));
-}
-
-__host__ __device__ static uint32_t pow2_at_least(uint32_t n) {
-  uint32_t p = 1;
-  while (p < n) p <<= 1;
-  return p;
 }
 
 size_t merge_scratch_bytes(uint32_t cap) {
@@ -408,265 +42,8 @@ int launch_merge_reset(cuimg_hip_ctx* ctx, const merge_args& a) {
   return launch_fill_u32(ctx, a.zeroer, 0, 0, 1, 0, (int)(2 * a.cap), kNoIdx);
 }
 
-// One thread per particle: targets in c8 order, actor records appended with
-// one atomic per wave, round-1 minunres by atomicMin.
-__global__ __launch_bounds__(256) void merge_scan_kernel(merge_args a) {
-  const uint32_t n = a.cnt->n;
-  const int lane = threadIdx.x & 63;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
-    const uint32_t i = i0 + lane;
-    merge_actor rec;
-    rec.i = (int32_t)i;
-    rec.mask = 0;
-#pragma unroll
-    for (int k = 0; k < 8; k++) rec.b[k] = -1;
-    if (i < n) {
-      const cuimg_particle_f p = a.parts[i];
-      if (p.age != 0) {
-        const int r = tzf(p.pos_r), c = tzf(p.pos_c);
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-          const uint32_t b = sparse_get(a.sparse, a.sparse_pitch, a.nr, a.nc, r + kC8R[k], c + kC8C[k]);
-          if (b == kEmpty || b >= n) continue;  // indices past the vector are treated as empty
-          const cuimg_particle_f q = a.parts[b];
-          if ((int)q.age > (int)p.age + 2 &&
-              norml2_rn(__fsub_rn(p.speed_r, q.speed_r), __fsub_rn(p.speed_c, q.speed_c)) < 2.f) {
-            rec.mask |= 1u << k;
-            rec.b[k] = (int32_t)b;
-          }
-        }
-      }
-    }
-    const bool act = rec.mask != 0;
-    const uint64_t m = __ballot(act);
-    if (m == 0) continue;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&a.cnt->actors, (uint32_t)__popcll(m));
-    base = __shfl(base, leader, 64);
-    if (act) {
-      const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-      a.actors[base + __popcll(m & below)] = rec;
-#pragma unroll
-      for (int k = 0; k < 8; k++)
-        if (rec.mask & (1u << k)) atomicMin(&a.minunres[rec.b[k]], round_tag(1) | i);
-    }
-  }
-}
-
-// Round-r decision for an open actor: false = still open.  A concurrent
-// resolver j of this round writes zeroer[b] only where no open actor below j
-// targets b, so whatever this thread reads there gives the same answer.
-__device__ bool merge_decide(const merge_args& a, const merge_actor& A, uint32_t r) {
-  const uint32_t tg = 255u - r, i = (uint32_t)A.i;
-  auto blocked = [&](uint32_t x) {
-    const uint32_t v = ld32(&a.minunres[x]);
-    return (v >> 24) == tg && (v & 0xFFFFFFu) < i;
-  };
-  if (blocked(i)) return false;
-  if (ld32(&a.zeroer[i]) < i) return true;  // removed before its turn
-  int32_t victim = -1;
-#pragma unroll
-  for (int j = 0; j < 8; j++)
-    if (victim < 0 && (A.mask & (1u << j))) {
-      const uint32_t b = (uint32_t)A.b[j];
-      if (blocked(b)) return false;
-      if (ld32(&a.zeroer[b]) < i) continue;
-      victim = (int32_t)b;
-    }
-  if (victim >= 0) {
-    st32(&a.zeroer[victim], i);
-    a.parts[victim].age = 0;
-  }
-  return true;
-}
-
-// Bitonic sort of keys[0, np) ascending, np a power of two, by one workgroup.
-template <bool GLOBAL>
-__device__ void bitonic_sort(unsigned long long* keys, uint32_t np) {
-  for (uint32_t size = 2; size <= np; size <<= 1)
-    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-      for (uint32_t t = threadIdx.x; t < np / 2; t += blockDim.x) {
-        const uint32_t lo = 2 * t - (t & (stride - 1)), hi = lo + stride;
-        const bool asc = (lo & size) == 0;
-        const unsigned long long x = GLOBAL ? ld64(&keys[lo]) : keys[lo];
-        const unsigned long long y = GLOBAL ? ld64(&keys[hi]) : keys[hi];
-        if ((x > y) == asc) {
-          if (GLOBAL) {
-            st64(&keys[lo], y);
-            st64(&keys[hi], x);
-          } else {
-            keys[lo] = y;
-            keys[hi] = x;
-          }
-        }
-      }
-      if (GLOBAL) __threadfence();
-      __syncthreads();
-    }
-}
-
-// Ascending sort of one key per lane across the wave.
-__device__ unsigned long long wave_sort(unsigned long long key) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int k = 2; k <= 64; k <<= 1)
-#pragma unroll
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      const unsigned long long o = __shfl_xor(key, j, 64);
-      const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
-      key = keep_min ? (o < key ? o : key) : (o > key ? o : key);
-    }
-  return key;
-}
-
-__device__ __forceinline__ uint32_t rdl(uint32_t v, uint32_t l) {
-  return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
-}
-
-// Serial replay of up to 64 open actors in index order (one wave; `key` of
-// lane l = the l-th actor, kNoIdx keys past the end).
-__device__ void replay_chunk(const merge_args& a, unsigned long long key) {
-  const int lane = threadIdx.x & 63;
-  const bool valid = key != ~0ull;
-  uint32_t ai = kNoIdx, zs = kNoIdx, amask = 0;
-  uint32_t ab[8], zb[8];
-#pragma unroll
-  for (int j = 0; j < 8; j++) ab[j] = zb[j] = kNoIdx;
-  if (valid) {
-    const merge_actor A = a.actors[(uint32_t)key];
-    ai = (uint32_t)A.i;
-    amask = A.mask;
-    zs = ld32(&a.zeroer[ai]);
-#pragma unroll
-    for (int j = 0; j < 8; j++)
-      if (amask & (1u << j)) {
-        ab[j] = (uint32_t)A.b[j];
-        zb[j] = ld32(&a.zeroer[ab[j]]);
-      }
-  }
-  const uint32_t cnt = (uint32_t)__popcll(__ballot(valid));  // valid keys come first
-  uint32_t victim = kNoIdx;
-  for (uint32_t l = 0; l < cnt; l++) {
-    const uint32_t i = rdl(ai, l);
-    const uint64_t below = l == 0 ? 0ull : (~0ull >> (64 - l));
-    if (rdl(zs, l) < i || (__ballot(victim == i) & below)) continue;  // removed before its turn
-    const uint32_t msk = rdl(amask, l);
-    bool done = false;
-#pragma unroll
-    for (int j = 0; j < 8; j++)
-      if (!done && (msk & (1u << j))) {
-        const uint32_t b = rdl(ab[j], l);
-        if (!(rdl(zb[j], l) < i || (__ballot(victim == b) & below))) {
-          if ((uint32_t)lane == l) victim = b;
-          done = true;
-        }
-      }
-  }
-  if (valid && victim != kNoIdx) {
-    st32(&a.zeroer[victim], ai);
-    a.parts[victim].age = 0;
-  }
-  __threadfence();
-}
-
-// One workgroup: round 1 over all actors, later rounds over the open ones,
-// serial replay of the rest, restoration of the scratch.
-__global__ __launch_bounds__(1024) void merge_resolve_kernel(merge_args a) {
-  __shared__ unsigned long long keys_lds[kPendLds];
-  __shared__ unsigned long long small[64];
-  __shared__ uint32_t npend, nopen[2], nsmall;
-  const uint32_t m = a.cnt->actors;
-  if (threadIdx.x == 0) npend = 0;
-  __syncthreads();
-  for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
-    const merge_actor A = a.actors[k];
-    if (!merge_decide(a, A, 1)) {
-      const uint32_t slot = atomicAdd(&npend, 1u);
-      const unsigned long long key = ((unsigned long long)(uint32_t)A.i << 32) | k;
-      if (slot < kPendLds) keys_lds[slot] = key;
-      else st64(&a.pend[slot], key);
-    }
-  }
-  __threadfence();
-  __syncthreads();
-  const uint32_t np = npend;
-  const bool in_lds = np <= kPendLds;
-  if (!in_lds) {
-    for (uint32_t k = threadIdx.x; k < kPendLds; k += blockDim.x) st64(&a.pend[k], keys_lds[k]);
-    __threadfence();
-    __syncthreads();
-  }
-  unsigned long long* keys = in_lds ? keys_lds : a.pend;
-  auto kget = [&](uint32_t s) { return in_lds ? keys_lds[s] : ld64(&a.pend[s]); };
-  auto kset = [&](uint32_t s, unsigned long long v) {
-    if (in_lds) keys_lds[s] = v;
-    else st64(&a.pend[s], v);
-  };
-
-  uint32_t open = np, r = 1;
-  while (open > kSerialSwitch && r < kMaxRounds) {
-    r++;
-    if (threadIdx.x == 0) nopen[r & 1] = 0;
-    for (uint32_t s = threadIdx.x; s < np; s += blockDim.x) {
-      const unsigned long long key = kget(s);
-      if (key == ~0ull) continue;
-      const merge_actor A = a.actors[(uint32_t)key];
-#pragma unroll
-      for (int j = 0; j < 8; j++)
-        if (A.mask & (1u << j)) atomicMin(&a.minunres[A.b[j]], round_tag(r) | (uint32_t)A.i);
-    }
-    __threadfence();
-    __syncthreads();
-    for (uint32_t s = threadIdx.x; s < np; s += blockDim.x) {
-      const unsigned long long key = kget(s);
-      if (key == ~0ull) continue;
-      if (merge_decide(a, a.actors[(uint32_t)key], r)) kset(s, ~0ull);
-      else atomicAdd(&nopen[r & 1], 1u);
-    }
-    __threadfence();
-    __syncthreads();
-    open = nopen[r & 1];
-  }
-
-  if (open > 0 && open <= 64) {
-    // gather the open keys into one wave, sort them there, replay
-    if (threadIdx.x == 0) nsmall = 0;
-    __syncthreads();
-    for (uint32_t s = threadIdx.x; s < np; s += blockDim.x) {
-      const unsigned long long key = kget(s);
-      if (key != ~0ull) small[atomicAdd(&nsmall, 1u)] = key;
-    }
-    __syncthreads();
-    if (threadIdx.x < 64) replay_chunk(a, wave_sort(threadIdx.x < open ? small[threadIdx.x] : ~0ull));
-  } else if (open > 0) {
-    // after kMaxRounds: full sort (open keys first), replay in chunks of 64
-    const uint32_t np2 = pow2_at_least(np);
-    for (uint32_t k = np + threadIdx.x; k < np2; k += blockDim.x) kset(k, ~0ull);
-    __threadfence();
-    __syncthreads();
-    if (np2 <= kPendLds) bitonic_sort<false>(keys, np2);
-    else bitonic_sort<true>(keys, np2);
-    if (threadIdx.x < 64)
-      for (uint32_t c0 = 0; c0 < open; c0 += 64) {
-        const uint32_t idx = c0 + threadIdx.x;
-        replay_chunk(a, idx < open ? kget(idx) : ~0ull);
-      }
-  }
-  __syncthreads();
-  // restore the scratch for the next call
-  for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
-    const merge_actor A = a.actors[k];
-#pragma unroll
-    for (int j = 0; j < 8; j++)
-      if (A.mask & (1u << j)) {
-        st32(&a.minunres[A.b[j]], kNoIdx);
-        st32(&a.zeroer[A.b[j]], kNoIdx);
-      }
-  }
-  if (threadIdx.x == 0) a.cnt->actors = 0;
-}
+__global__ __launch_bounds__(256) void merge_scan_kernel(merge_args a) { merge_scan_body(a, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(1024) void merge_resolve_kernel(merge_args a) { merge_resolve_body(a); }
 
 int launch_merge(cuimg_hip_ctx* ctx, const merge_args& a) {
   const int blocks = (int)std::min<uint32_t>((a.cap + 255) / 256, 2048);

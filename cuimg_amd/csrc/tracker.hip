@@ -1,5 +1,6 @@
-// tracker<pyrlk> frame step on the GPU: per-scale state, orchestration of the
-// kernels and the C ABI of the tracker.
+// tracker<pyrlk> frame step on the GPU, for one video stream or a batch of
+// independent streams: per-scale state, the batched step kernels, graph
+// capture and the C ABI of trackers and batches.
 //
 // Reference call stack (tracking2/): tracker::run (tracker.hpp:73-91) builds the
 // input pyramid (update_input / subsample_input, :53-69) and calls the
@@ -7,16 +8,32 @@
 // (pyrlk_tracker.hpp:831-875) matches the particles (frame > 0), runs the
 // detector and inserts new particles (every detector_frequency frames, with a
 // compaction), then merges trajectories (every filtering_frequency frames).
+// Scales only interact through the images (the recursive prediction re-solves
+// coarser scales itself), so all scales of all streams run side by side.
+//
+// Design: every (stream, scale) pair is a "job" with a device-resident
+// descriptor (level_job) that holds the argument blocks of each kernel for
+// every ping-pong combination (frame slot, current particle vector).  Which
+// combination applies, and whether this frame matches / detects / merges, is
+// derived on the device from the job's level_counters and frequencies, so a
+// frame step is the same fixed sequence of 11 launches for any number of
+// streams and any frame: it is captured once into a hipGraph and replayed.
+// Each batched launch maps its blocks to jobs through a per-kind prefix array
+// of block counts.  The per-stream input frame pointers are the only per-step
+// upload (a small table, copied from a pinned ring before the launch).
 //
 // GPU layout per scale: two u8 pyramid slots (current / previous frame, border
 // 3, clamp-filled), two FP32 blurred slots for the blurred-frame profile, the
 // detector's mask and saliency, the u32 sparse index image (border 7), two
 // dense particle vectors (compaction ping-pongs between them), the matches
-// vector, and per-particle scratch for compaction and the merge.  Particle
-// counts stay in device memory (level_counters), so a frame step is a fixed
-// sequence of launches with no host synchronisation.
-#include "tracker_state.hpp"
+// vector, and per-particle scratch for compaction and the merge.
+#include "fast_dev.hpp"
+#include "gaussian_dev.hpp"
+#include "lk_dev.hpp"
+#include "particles_dev.hpp"
+#include "pyramid_dev.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -30,7 +47,6 @@ struct level_res {
   uint8_t* boxcode = nullptr;
   uint32_t* subcnt = nullptr;
   cuimg_particle_f* parts[2] = {nullptr, nullptr};
-  int cur = 0;
   uint32_t* matches = nullptr;
   uint8_t* flags = nullptr;
   uint32_t* blockcnt = nullptr;
@@ -40,261 +56,700 @@ struct level_res {
   merge_args merge{};  // merge scratch (one allocation)
   uint32_t cap = 0;
   int nblocks = 0, nbr = 0, nbc = 0;
-  bool compact_has_run = false;
   cuimg_scale_params sp{};
 };
 
+struct job_params {
+  int det_freq, filt_freq, with_merge, fast_n;
+  float fast_th;
+};
+
+struct level_job {
+  int stream, s;
+  int nr, nc;
+  job_params prm;
+  level_counters* cnt;
+  uint8_t* u8[2];
+  int u8_pitch;
+  uint8_t* mask;
+  int mask_pitch;
+  uint8_t* sal;
+  int sal_pitch;
+  uint32_t* sparse;
+  int sparse_pitch, sparse_border;
+  int fill_sparse_blocks;  // first blocks of the fills launch: sparse reset; the rest: mask
+  int fast_tiles_x;
+  lk_args lk[4];        // [slot * 2 + cur]
+  detect_args det[2];   // [cur]
+  merge_args mg[2];     // [current vector after the detector]
+  gauss_job blur[2];    // [slot]
+};
+
+struct stream_job {
+  int first, nlev;
+  pyr_args pyr[2];  // [slot]; the frame is taken from the input table
+};
+
+struct frame_in {
+  const void* begin;
+  int pitch;
+  int vec_ok;
+};
+
+enum { K_PYR, K_BLUR, K_FILL, K_LK, K_PMASK, K_FAST, K_DETC, K_SCATI, K_MSCAN, K_NKINDS };
+constexpr int kRing = 8;
+
 }  // namespace cuimg
 
-struct cuimg_hip_tracker {
+struct cuimg_hip_batch {
   cuimg_hip_ctx* ctx = nullptr;
-  cuimg_tracker_params P{};
-  int L = 0;
-  cuimg::level_res lv[cuimg::kMaxLevels];
-  cuimg::level_counters* cnt = nullptr;  // device, one per scale
-  cuimg_img frame_buf{};
-  int frame_cpt = 0;
-  float coef[33]{};
+  int nstreams = 0, njobs = 0;
+  cuimg_tracker_params P0{};                 // matcher / blur settings shared by all streams
+  std::vector<cuimg_tracker_params> P;       // per stream
+  std::vector<int> first;                    // job index of each stream's scale 0
+  std::vector<cuimg::level_res> lv;          // per job
+  std::vector<cuimg::level_job> hj;          // host copy of the job table
+  cuimg::level_job* dj = nullptr;
+  std::vector<cuimg::stream_job> hs;
+  cuimg::stream_job* ds = nullptr;
+  cuimg::level_counters* cnt = nullptr;      // device, one per job
+  cuimg::frame_in* din = nullptr;            // device input table
+  cuimg::frame_in* hin = nullptr;            // pinned ring of input tables
+  hipEvent_t ring_ev[cuimg::kRing]{};
+  uint64_t steps = 0;
+  std::vector<uint32_t> prefix_h[cuimg::K_NKINDS];
+  uint32_t* prefix_d[cuimg::K_NKINDS]{};
+  uint32_t total[cuimg::K_NKINDS]{};
+  cuimg::coef_pack cp{};
+  int lk_div = 48;    // particles of capacity per LK workgroup (tuning)
+  int blur_th = 0;    // blur tile height, 0 = heuristic (tuning)
+  bool use_graph = true;
+  hipGraphExec_t exec = nullptr;
+  std::vector<cuimg_img> frame_buf;          // per stream, for run_host
   std::vector<void*> allocs;
+  std::vector<cuimg_hip_tracker*> views;
+};
+
+struct cuimg_hip_tracker {
+  cuimg_hip_batch* b = nullptr;
+  int stream = 0;
+  bool owner = false;
 };
 
 namespace cuimg {
 
-static int dev_alloc(cuimg_hip_tracker* t, size_t bytes, void** out) {
+// ------------------------------------------------------------ step kernels
+__device__ __forceinline__ int find_job(const uint32_t* prefix, int njobs, uint32_t b) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+struct step_flags {
+  uint32_t frame, cur, slot;
+  bool match, detect, merge;
+};
+
+__device__ __forceinline__ step_flags flags_of(const level_job& J) {
+  step_flags f;
+  const job_params prm = uni(J.prm);
+  f.frame = uni(J.cnt->frame);
+  f.cur = uni(J.cnt->cur) & 1u;
+  f.slot = f.frame & 1u;
+  f.match = f.frame > 0;
+  f.detect = f.frame % (uint32_t)prm.det_freq == 0;
+  f.merge = prm.with_merge && f.frame % (uint32_t)prm.filt_freq == 0;
+  return f;
+}
+
+#define CUIMG_JOB(prefix, njobs)                               \
+  const int j = find_job(prefix, njobs, blockIdx.x);           \
+  const level_job& J = jobs[j];                                \
+  const int lb = (int)(blockIdx.x - prefix[j]);                \
+  const int nb = (int)(prefix[j + 1] - prefix[j]);             \
+  (void)lb;                                                    \
+  (void)nb
+
+// tracker::update_input + subsample_input for every stream: the u8 pyramid of
+// the new frame into the frame's slot.
+__global__ __launch_bounds__(256) void step_pyramid_kernel(const stream_job* S, const level_job* jobs,
+                                                           const frame_in* in, const uint32_t* prefix, int nstreams) {
+  __shared__ pyr_args a;
+  const int st = find_job(prefix, nstreams, blockIdx.x);
+  const stream_job& SJ = S[st];
+  const uint32_t slot = jobs[SJ.first].cnt->frame & 1u;
+  const int* src = reinterpret_cast<const int*>(&SJ.pyr[slot]);
+  int* dst = reinterpret_cast<int*>(&a);
+  for (int i = threadIdx.x; i < (int)(sizeof(pyr_args) / 4); i += blockDim.x) dst[i] = src[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a.frame = in[st].begin;
+    a.frame_pitch = in[st].pitch;
+    a.frame_vec_ok = a.frame_vec_ok && in[st].vec_ok;
+  }
+  __syncthreads();
+  pyramid_tile<uint8_t>(a, (int)(blockIdx.x - prefix[st]));
+}
+
+// Blurred-frame profile: sigma / K Gaussian of every level of the new frame.
+template <int K>
+__global__ __launch_bounds__(64) void step_blur_kernel(const level_job* jobs, const uint32_t* prefix, int njobs,
+                                                       coef_pack cp) {
+  CUIMG_JOB(prefix, njobs);
+  const gauss_job G = uni(J.blur[J.cnt->frame & 1u]);
+  gauss_tile<uint8_t, float, 4, K>(G, lb % G.tiles_x, lb / G.tiles_x, cp);
+}
+
+// before_matching (sparse <- -1, particle_container.hpp:187-193) and the
+// detector mask reset (mask <- 255).
+__global__ __launch_bounds__(256) void step_fill_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
+  CUIMG_JOB(prefix, njobs);
+  const step_flags f = flags_of(J);
+  const int F = J.fill_sparse_blocks;
+  if (lb < F) {
+    if (f.match) {
+      const int b = J.sparse_border;
+      fill_u32_body(J.sparse, J.sparse_pitch, -b, J.nr + b, -b, J.nc + b, kEmpty, lb, F);
+    }
+  } else if (f.detect) {
+    fill_u8_body(J.mask, J.mask_pitch, J.nr, J.nc, 0xFFFFFFFFu, lb - F, nb - F);
+  }
+}
+
+template <typename TI, int NPL, bool SEL>
+__global__ __launch_bounds__(256) void step_lk_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
+  CUIMG_JOB(prefix, njobs);
+  const step_flags f = flags_of(J);
+  if (!f.match) return;
+  lk_body<TI, NPL, SEL>(J.lk[f.slot * 2 + f.cur], lb, nb);
+}
+
+__global__ __launch_bounds__(256) void step_pmask_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
+  CUIMG_JOB(prefix, njobs);
+  const step_flags f = flags_of(J);
+  if (!f.detect) return;
+  const detect_args d = uni(J.det[f.cur]);
+  particle_mask_body(d, lb, nb);
+}
+
+__global__ __launch_bounds__(256) void step_fast_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
+  CUIMG_JOB(prefix, njobs);
+  const step_flags f = flags_of(J);
+  if (!f.detect) return;
+  fast_tile_body<false>(J.u8[f.slot], J.u8_pitch, J.mask, J.mask_pitch, J.sal, J.sal_pitch, J.nr, J.nc, lb,
+                        J.fast_tiles_x, J.prm.fast_n, (int)J.prm.fast_th);
+}
+
+// Box selection with sub-row counts, and the compaction counts.
+__global__ __launch_bounds__(256) void step_detc_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
+  CUIMG_JOB(prefix, njobs);
+  const step_flags f = flags_of(J);
+  if (!f.detect) return;
+  const detect_args d = uni(J.det[f.cur]);
+  if (lb < d.nbr) nms_row_body(d, J.prm.fast_th, lb);
+  else compact_count_body(d, lb - d.nbr);
+}
+
+__global__ __launch_bounds__(1024) void step_cscan_kernel(const level_job* jobs) {
+  const level_job& J = jobs[blockIdx.x];
+  const step_flags f = flags_of(J);
+  if (!f.detect) return;
+  const detect_args d = uni(J.det[f.cur]);
+  compact_scan_body(d);
+}
+
+// Compaction scatter and insertion of the new points (disjoint outputs).
+__global__ __launch_bounds__(256) void step_scati_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
+  CUIMG_JOB(prefix, njobs);
+  const step_flags f = flags_of(J);
+  if (!f.detect) return;
+  const detect_args d = uni(J.det[f.cur]);
+  if (lb < d.nblocks) compact_scatter_body(d, lb);
+  else insert_new_body(d, lb - d.nblocks);
+}
+
+__global__ __launch_bounds__(256) void step_mscan_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
+  CUIMG_JOB(prefix, njobs);
+  const step_flags f = flags_of(J);
+  if (!f.merge) return;
+  const merge_args m = uni(J.mg[f.cur ^ (f.detect ? 1u : 0u)]);
+  merge_scan_body(m, lb, nb);
+}
+
+// Merge resolution, then the end of the job's frame step.
+__global__ __launch_bounds__(1024) void step_mres_kernel(const level_job* jobs) {
+  const level_job& J = jobs[blockIdx.x];
+  const step_flags f = flags_of(J);
+  const uint32_t c = f.cur ^ (f.detect ? 1u : 0u);
+  if (f.merge) {
+    const merge_args m = uni(J.mg[c]);
+    merge_resolve_body(m);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    J.cnt->cur = c;
+    J.cnt->compact_has_run = f.detect ? 1u : 0u;
+    J.cnt->frame = f.frame + 1;
+  }
+}
+
+// ------------------------------------------------------------------ host
+static int dev_alloc(cuimg_hip_batch* b, size_t bytes, void** out) {
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
   if (e != hipSuccess)
-    return set_error(t->ctx, CUIMG_E_NOMEM, "tracker: hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
-  t->allocs.push_back(p);
+    return set_error(b->ctx, CUIMG_E_NOMEM, "tracker: hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  b->allocs.push_back(p);
   *out = p;
   return 0;
 }
 
 template <typename T>
-static int dev_alloc_n(cuimg_hip_tracker* t, size_t n, T** out) {
+static int dev_alloc_n(cuimg_hip_batch* b, size_t n, T** out) {
   void* p = nullptr;
-  CUIMG_TRY(dev_alloc(t, n * sizeof(T), &p));
+  CUIMG_TRY(dev_alloc(b, n * sizeof(T), &p));
   *out = static_cast<T*>(p);
   return 0;
 }
 
-static void destroy(cuimg_hip_tracker* t) {
-  if (!t) return;
-  (void)hipStreamSynchronize(t->ctx->stream);
-  for (int s = 0; s < t->L; s++) {
-    level_res& R = t->lv[s];
+static void destroy_batch(cuimg_hip_batch* b) {
+  if (!b) return;
+  (void)hipStreamSynchronize(b->ctx->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  for (auto& R : b->lv) {
     for (int k = 0; k < 2; k++) {
-      image_free(t->ctx, &R.u8[k]);
-      image_free(t->ctx, &R.blur[k]);
+      image_free(b->ctx, &R.u8[k]);
+      image_free(b->ctx, &R.blur[k]);
     }
-    image_free(t->ctx, &R.sal);
-    image_free(t->ctx, &R.mask);
-    image_free(t->ctx, &R.sparse);
+    image_free(b->ctx, &R.sal);
+    image_free(b->ctx, &R.mask);
+    image_free(b->ctx, &R.sparse);
   }
-  image_free(t->ctx, &t->frame_buf);
-  for (void* p : t->allocs) (void)hipFree(p);
-  delete t;
+  for (auto& f : b->frame_buf) image_free(b->ctx, &f);
+  for (void* p : b->allocs) (void)hipFree(p);
+  if (b->hin) (void)hipHostFree(b->hin);
+  for (auto& e : b->ring_ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto* v : b->views) delete v;
+  delete b;
 }
 
-static int fill_sparse(cuimg_hip_tracker* t, level_res& R) {
-  const int b = R.sparse.border;
-  return launch_fill_u32(t->ctx, static_cast<uint32_t*>(R.sparse.begin), R.sparse.pitch, -b, R.nr + b, -b,
-                         R.nc + b, 0xFFFFFFFFu);
+static int fill_sparse(cuimg_hip_ctx* ctx, level_res& R) {
+  const int bd = R.sparse.border;
+  return launch_fill_u32(ctx, static_cast<uint32_t*>(R.sparse.begin), R.sparse.pitch, -bd, R.nr + bd, -bd,
+                         R.nc + bd, 0xFFFFFFFFu);
 }
 
-static int create(cuimg_hip_ctx* ctx, int nrows, int ncols, const cuimg_tracker_params* P,
-                  cuimg_hip_tracker** out) {
-  if (!ctx || !P || !out) return CUIMG_E_INVALID;
-  *out = nullptr;
+static int check_params(cuimg_hip_ctx* ctx, int nrows, int ncols, const cuimg_tracker_params* P) {
   if (P->nscales < 1 || P->nscales > 7)
     return set_error(ctx, CUIMG_E_INVALID, "tracker: nscales %d not in 1..7", P->nscales);
   if (P->lk_ws < 1 || P->lk_ws > 15 || !(P->lk_ws & 1))
     return set_error(ctx, CUIMG_E_UNSUPPORTED, "tracker: LK window %d must be odd and <= 15", P->lk_ws);
-  if (P->box_size < 1 || P->box_size > 15)
-    return set_error(ctx, CUIMG_E_UNSUPPORTED, "tracker: detector box %d not in 1..15", P->box_size);
+  if (P->box_size < 1 || P->box_size > kMaxBox)
+    return set_error(ctx, CUIMG_E_UNSUPPORTED, "tracker: detector box %d not in 1..%d", P->box_size, kMaxBox);
   if (P->detector_frequency < 1 || P->filtering_frequency < 1 || P->fast_th < 0.f || P->fast_n < 1)
     return set_error(ctx, CUIMG_E_INVALID, "tracker: bad detector / filtering settings");
   if (P->profile != 0 && P->profile != 1)
     return set_error(ctx, CUIMG_E_INVALID, "tracker: profile %d not in {0,1}", P->profile);
-  if (P->profile == 1 && (P->blur_K < 1 || P->blur_K > 16 || P->blur_sigma < 1))
+  if (P->profile == 1 && (P->blur_K < 1 || P->blur_K > kMaxK || P->blur_sigma < 1))
     return set_error(ctx, CUIMG_E_UNSUPPORTED, "tracker: blur sigma %d / half width %d", P->blur_sigma, P->blur_K);
   int r = nrows, c = ncols;
   for (int s = 0; s < P->nscales; s++) {
     if (r < 1 || c < 1)
       return set_error(ctx, CUIMG_E_INVALID, "tracker: scale %d of %dx%d is empty", s, nrows, ncols);
+    if ((size_t)r * c / 4 > kMergeMaxCap)
+      return set_error(ctx, CUIMG_E_UNSUPPORTED, "tracker: %dx%d exceeds the particle capacity limit", r, c);
     r /= 2;
     c /= 2;
   }
-  auto* t = new cuimg_hip_tracker();
-  t->ctx = ctx;
-  t->P = *P;
-  t->L = P->nscales;
-  if (P->profile == 1) gaussian_coefs(P->blur_sigma, P->blur_K, t->coef);
-  int rc = 0;
-  auto fail = [&](int code) {
-    destroy(t);
-    return code;
-  };
-  if ((rc = dev_alloc_n(t, t->L, &t->cnt))) return fail(rc);
-  if (hipMemsetAsync(t->cnt, 0, sizeof(level_counters) * t->L, ctx->stream) != hipSuccess) return fail(CUIMG_E_NOMEM);
-  if ((rc = image_alloc(ctx, nrows, ncols, 0, CUIMG_U8, &t->frame_buf))) return fail(rc);
-  r = nrows;
-  c = ncols;
-  for (int s = 0; s < t->L; s++) {
-    level_res& R = t->lv[s];
-    R.nr = r;
-    R.nc = c;
-    R.sp = cuimg_scale_params{P->fast_n, P->fast_th, P->detector_frequency, P->filtering_frequency, P->k,
-                              P->with_merge};
-    for (int k = 0; k < 2; k++) {
-      if ((rc = image_alloc(ctx, r, c, 3, CUIMG_U8, &R.u8[k]))) return fail(rc);
-      if (P->profile == 1 && (rc = image_alloc(ctx, r, c, 0, CUIMG_F32, &R.blur[k]))) return fail(rc);
-    }
-    if ((rc = image_alloc(ctx, r, c, 1, CUIMG_U8, &R.sal))) return fail(rc);
-    if ((rc = image_alloc(ctx, r, c, 1, CUIMG_U8, &R.mask))) return fail(rc);
-    if ((rc = image_alloc(ctx, r, c, 7, CUIMG_U32, &R.sparse))) return fail(rc);
-    R.nbr = (int)((float)r / (float)P->box_size);
-    R.nbc = (int)((float)c / (float)P->box_size);
-    const size_t nboxes = (size_t)R.nbr * R.nbc;
-    R.cap = (uint32_t)std::max<size_t>((size_t)r * c / 4, 4096);
-    if (R.cap > kMergeMaxCap)
-      return fail(set_error(ctx, CUIMG_E_UNSUPPORTED, "tracker: %dx%d exceeds the particle capacity limit", r, c));
-    R.nblocks = ceil_div((int)R.cap, 1024);
-    if ((rc = dev_alloc_n(t, std::max<size_t>(nboxes, 1), &R.boxcode))) return fail(rc);
-    if ((rc = dev_alloc_n(t, std::max<size_t>((size_t)R.nbr * P->box_size, 1), &R.subcnt))) return fail(rc);
-    for (int k = 0; k < 2; k++)
-      if ((rc = dev_alloc_n(t, R.cap, &R.parts[k]))) return fail(rc);
-    if ((rc = dev_alloc_n(t, R.cap + nboxes, &R.matches))) return fail(rc);
-    if ((rc = dev_alloc_n(t, R.cap, &R.flags))) return fail(rc);
-    if ((rc = dev_alloc_n(t, R.nblocks, &R.blockcnt))) return fail(rc);
-    if ((rc = dev_alloc_n(t, R.cap, &R.dec_accept))) return fail(rc);
-    if ((rc = dev_alloc_n(t, R.cap, &R.dec_tr))) return fail(rc);
-    if ((rc = dev_alloc_n(t, R.cap, &R.dec_dist))) return fail(rc);
-    {
-      void* m = nullptr;
-      if ((rc = dev_alloc(t, merge_scratch_bytes(R.cap), &m))) return fail(rc);
-      R.merge = merge_scratch_carve(m, R.cap);
-      if ((rc = launch_merge_reset(ctx, R.merge))) return fail(rc);
-    }
-    // particle_container ctor: sparse <- -1 (tracking2/particle_container.hpp:23-31)
-    if ((rc = fill_sparse(t, R))) return fail(rc);
-    r /= 2;
-    c /= 2;
-  }
-  if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(CUIMG_E_NOMEM);
-  *out = t;
   return 0;
 }
 
-static lk_level_view matcher_view(const cuimg_hip_tracker* t, int s, int prev, int cur) {
-  const level_res& R = t->lv[s];
-  const cuimg_img& A = t->P.profile == 0 ? R.u8[prev] : R.blur[prev];
-  const cuimg_img& B = t->P.profile == 0 ? R.u8[cur] : R.blur[cur];
+static int alloc_level(cuimg_hip_batch* b, const cuimg_tracker_params* P, int r, int c, level_res& R) {
+  cuimg_hip_ctx* ctx = b->ctx;
+  R.nr = r;
+  R.nc = c;
+  R.sp = cuimg_scale_params{P->fast_n, P->fast_th, P->detector_frequency, P->filtering_frequency, P->k,
+                            P->with_merge};
+  for (int k = 0; k < 2; k++) {
+    CUIMG_TRY(image_alloc(ctx, r, c, 3, CUIMG_U8, &R.u8[k]));
+    if (P->profile == 1) CUIMG_TRY(image_alloc(ctx, r, c, 0, CUIMG_F32, &R.blur[k]));
+  }
+  CUIMG_TRY(image_alloc(ctx, r, c, 1, CUIMG_U8, &R.sal));
+  CUIMG_TRY(image_alloc(ctx, r, c, 1, CUIMG_U8, &R.mask));
+  CUIMG_TRY(image_alloc(ctx, r, c, 7, CUIMG_U32, &R.sparse));
+  R.nbr = (int)((float)r / (float)P->box_size);
+  R.nbc = (int)((float)c / (float)P->box_size);
+  const size_t nboxes = (size_t)R.nbr * R.nbc;
+  R.cap = (uint32_t)std::max<size_t>((size_t)r * c / 4, 4096);
+  R.nblocks = ceil_div((int)R.cap, kCompactBlock);
+  CUIMG_TRY(dev_alloc_n(b, std::max<size_t>(nboxes, 1), &R.boxcode));
+  CUIMG_TRY(dev_alloc_n(b, std::max<size_t>((size_t)R.nbr * P->box_size, 1), &R.subcnt));
+  for (int k = 0; k < 2; k++) CUIMG_TRY(dev_alloc_n(b, R.cap, &R.parts[k]));
+  CUIMG_TRY(dev_alloc_n(b, R.cap + nboxes, &R.matches));
+  CUIMG_TRY(dev_alloc_n(b, R.cap, &R.flags));
+  CUIMG_TRY(dev_alloc_n(b, R.nblocks, &R.blockcnt));
+  CUIMG_TRY(dev_alloc_n(b, R.cap, &R.dec_accept));
+  CUIMG_TRY(dev_alloc_n(b, R.cap, &R.dec_tr));
+  CUIMG_TRY(dev_alloc_n(b, R.cap, &R.dec_dist));
+  void* m = nullptr;
+  CUIMG_TRY(dev_alloc(b, merge_scratch_bytes(R.cap), &m));
+  R.merge = merge_scratch_carve(m, R.cap);
+  CUIMG_TRY(launch_merge_reset(ctx, R.merge));
+  // particle_container ctor: sparse <- -1 (tracking2/particle_container.hpp:23-31)
+  CUIMG_TRY(fill_sparse(ctx, R));
+  return 0;
+}
+
+static lk_level_view matcher_view(const cuimg_hip_batch* b, int j, int prev, int cur) {
+  const level_res& R = b->lv[j];
+  const cuimg_img& A = b->P0.profile == 0 ? R.u8[prev] : R.blur[prev];
+  const cuimg_img& B = b->P0.profile == 0 ? R.u8[cur] : R.blur[cur];
   return lk_level_view{A.begin, B.begin, A.pitch, B.pitch, R.nr, R.nc};
 }
 
-static int update_level(cuimg_hip_tracker* t, int s, int slot) {
-  cuimg_hip_ctx* ctx = t->ctx;
-  level_res& R = t->lv[s];
-  const cuimg_tracker_params& P = t->P;
-  uint32_t* sparse = static_cast<uint32_t*>(R.sparse.begin);
+static int blur_th(const cuimg_hip_batch* b) {
+  if (b->blur_th > 0) return b->blur_th;
+  // aim for >= ~2048 waves over the whole launch, 16..128 rows per tile
+  long strips = 0;
+  for (const auto& R : b->lv) strips += (long)R.nr * ceil_div(R.nc, 256);
+  return (int)std::max<long>(16, std::min<long>(128, (strips + 2047) / 2048));
+}
 
-  if (t->frame_cpt > 0) {
-    // particle_container::before_matching: sparse <- -1
-    CUIMG_TRY(fill_sparse(t, R));
-    R.compact_has_run = false;
-    lk_args a{};
-    for (int j = 0; j < t->L; j++) a.lv[j] = matcher_view(t, j, slot ^ 1, slot);
-    a.nlevels = t->L;
-    a.s = s;
-    a.parts = R.parts[R.cur];
-    a.cnt = t->cnt + s;
-    a.sparse = sparse;
-    a.sparse_pitch = R.sparse.pitch;
-    a.dec_accept = R.dec_accept;
-    a.dec_tr = R.dec_tr;
-    a.dec_dist = R.dec_dist;
-    // particle_domain_ = domain - border(3) (pyrlk_tracker.hpp:792)
-    a.pd_r1 = 3;
-    a.pd_c1 = 3;
-    a.pd_r2 = R.nr - 4;
-    a.pd_c2 = R.nc - 4;
-    a.P = lk_params_dev{P.lk_ws, P.lk_max_iters, P.lk_top_n, P.lk_eps, 1e-4f, R.sp.k, P.profile == 0 ? 1 : 0};
-    CUIMG_TRY(launch_lk(ctx, a, P.profile == 0));
-  }
-
-  if (!(t->frame_cpt % R.sp.detector_frequency)) {
-    detect_args d{};
+// Job descriptors of job j from its resources and settings.
+static void build_job(cuimg_hip_batch* b, int i, int s) {
+  const int j = b->first[i] + s;
+  const level_res& R = b->lv[j];
+  const cuimg_tracker_params& P = b->P[i];
+  level_job& J = b->hj[j];
+  J = level_job{};
+  J.stream = i;
+  J.s = s;
+  J.nr = R.nr;
+  J.nc = R.nc;
+  J.prm = job_params{R.sp.detector_frequency, R.sp.filtering_frequency, R.sp.with_merge, R.sp.fast_n, R.sp.fast_th};
+  J.cnt = b->cnt + j;
+  for (int k = 0; k < 2; k++) J.u8[k] = static_cast<uint8_t*>(R.u8[k].begin);
+  J.u8_pitch = R.u8[0].pitch;
+  J.mask = static_cast<uint8_t*>(R.mask.begin);
+  J.mask_pitch = R.mask.pitch;
+  J.sal = static_cast<uint8_t*>(R.sal.begin);
+  J.sal_pitch = R.sal.pitch;
+  J.sparse = static_cast<uint32_t*>(R.sparse.begin);
+  J.sparse_pitch = R.sparse.pitch;
+  J.sparse_border = R.sparse.border;
+  J.fast_tiles_x = ceil_div(R.nc, kFTW);
+  const long sp_px = (long)(R.nr + 2 * R.sparse.border) * (R.nc + 2 * R.sparse.border);
+  J.fill_sparse_blocks = (int)std::max<long>(1, std::min<long>(1024, (sp_px + 1023) / 1024));
+  for (int slot = 0; slot < 2; slot++)
+    for (int cur = 0; cur < 2; cur++) {
+      lk_args& a = J.lk[slot * 2 + cur];
+      for (int k = 0; k < P.nscales; k++) a.lv[k] = matcher_view(b, b->first[i] + k, slot ^ 1, slot);
+      a.nlevels = P.nscales;
+      a.s = s;
+      a.parts = R.parts[cur];
+      a.cnt = J.cnt;
+      a.sparse = J.sparse;
+      a.sparse_pitch = J.sparse_pitch;
+      a.dec_accept = R.dec_accept;
+      a.dec_tr = R.dec_tr;
+      a.dec_dist = R.dec_dist;
+      // particle_domain_ = domain - border(3) (pyrlk_tracker.hpp:792)
+      a.pd_r1 = 3;
+      a.pd_c1 = 3;
+      a.pd_r2 = R.nr - 4;
+      a.pd_c2 = R.nc - 4;
+      a.P = lk_params_dev{P.lk_ws, P.lk_max_iters, P.lk_top_n, P.lk_eps, 1e-4f, R.sp.k, P.profile == 0 ? 1 : 0};
+    }
+  for (int cur = 0; cur < 2; cur++) {
+    detect_args& d = J.det[cur];
     d.nr = R.nr;
     d.nc = R.nc;
     d.box = P.box_size;
     d.nbr = R.nbr;
     d.nbc = R.nbc;
     d.fast_th = R.sp.fast_th;
-    d.sal = static_cast<const uint8_t*>(R.sal.begin);
-    d.sal_pitch = R.sal.pitch;
-    d.mask = static_cast<uint8_t*>(R.mask.begin);
-    d.mask_pitch = R.mask.pitch;
-    d.sparse = sparse;
-    d.sparse_pitch = R.sparse.pitch;
-    d.parts = R.parts[R.cur];
-    d.parts_out = R.parts[R.cur ^ 1];
+    d.sal = J.sal;
+    d.sal_pitch = J.sal_pitch;
+    d.mask = J.mask;
+    d.mask_pitch = J.mask_pitch;
+    d.sparse = J.sparse;
+    d.sparse_pitch = J.sparse_pitch;
+    d.parts = R.parts[cur];
+    d.parts_out = R.parts[cur ^ 1];
     d.matches = R.matches;
     d.flags = R.flags;
     d.blockcnt = R.blockcnt;
     d.boxcode = R.boxcode;
     d.subcnt = R.subcnt;
-    d.cnt = t->cnt + s;
+    d.cnt = J.cnt;
     d.cap = R.cap;
     d.nblocks = R.nblocks;
-    CUIMG_TRY(launch_particle_mask(ctx, d));
-    CUIMG_TRY(fast_saliency(ctx, &R.u8[slot], &R.mask, &R.sal, R.sp.fast_n, (int)R.sp.fast_th));
-    CUIMG_TRY(launch_detect_and_compact(ctx, d));
-    R.cur ^= 1;
-    R.compact_has_run = true;
-  }
-
-  if (!(t->frame_cpt % R.sp.filtering_frequency) && R.sp.with_merge) {
-    merge_args m = R.merge;
+    merge_args& m = J.mg[cur];
+    m = R.merge;
     m.nr = R.nr;
     m.nc = R.nc;
-    m.parts = R.parts[R.cur];
-    m.sparse = sparse;
-    m.sparse_pitch = R.sparse.pitch;
-    m.cnt = t->cnt + s;
-    CUIMG_TRY(launch_merge(ctx, m));
+    m.parts = R.parts[cur];
+    m.sparse = J.sparse;
+    m.sparse_pitch = J.sparse_pitch;
+    m.cnt = J.cnt;
+  }
+  const int th = blur_th(b);
+  for (int slot = 0; slot < 2; slot++) {
+    gauss_job& g = J.blur[slot];
+    g = gauss_job{};
+    if (P.profile != 1) continue;
+    g.in = R.u8[slot].begin;
+    g.out = R.blur[slot].begin;
+    g.in_pitch = R.u8[slot].pitch;
+    g.out_pitch = R.blur[slot].pitch;
+    g.nr = R.nr;
+    g.nc = R.nc;
+    g.th = th;
+    g.tiles_x = ceil_div(R.nc, 256);
+    g.vec_ok = (R.u8[slot].pitch % 4 == 0) && ((uintptr_t)R.u8[slot].begin % 4 == 0) &&
+               (R.blur[slot].pitch % 16 == 0) && ((uintptr_t)R.blur[slot].begin % 16 == 0);
+  }
+}
+
+static void build_stream(cuimg_hip_batch* b, int i) {
+  stream_job& S = b->hs[i];
+  S = stream_job{};
+  S.first = b->first[i];
+  S.nlev = b->P[i].nscales;
+  for (int slot = 0; slot < 2; slot++) {
+    pyr_args& a = S.pyr[slot];
+    a.copy_level0 = 1;
+    a.n = S.nlev;
+    for (int s = 0; s < S.nlev; s++) {
+      const cuimg_img& L = b->lv[S.first + s].u8[slot];
+      a.lv[s] = pyr_level{L.begin, L.pitch, L.nrows, L.ncols, L.border};
+    }
+    const cuimg_img& L0 = b->lv[S.first].u8[slot];
+    a.frame_vec_ok = ((uintptr_t)L0.begin % 16 == 0) && (L0.pitch % 16 == 0);
+    a.tiles_x = ceil_div(L0.ncols, kTile);
+  }
+}
+
+// Per-kind block counts of every job, their prefix sums, and the upload.
+static int build_maps(cuimg_hip_batch* b) {
+  std::vector<uint32_t> cnt[K_NKINDS];
+  for (int i = 0; i < b->nstreams; i++) {
+    const level_res& R0 = b->lv[b->first[i]];
+    cnt[K_PYR].push_back((uint32_t)(ceil_div(R0.nc, kTile) * ceil_div(R0.nr, kTile)));
+  }
+  const int th = blur_th(b);
+  for (int j = 0; j < b->njobs; j++) {
+    const level_res& R = b->lv[j];
+    const level_job& J = b->hj[j];
+    const long cap = R.cap;
+    cnt[K_BLUR].push_back(b->P0.profile == 1 ? (uint32_t)(ceil_div(R.nc, 256) * ceil_div(R.nr, th)) : 0u);
+    const long mask_words = (long)R.nr * ((R.nc + 3) / 4);
+    const int G = (int)std::max<long>(1, std::min<long>(1024, (mask_words + 1023) / 1024));
+    cnt[K_FILL].push_back((uint32_t)(J.fill_sparse_blocks + G));
+    cnt[K_LK].push_back((uint32_t)std::max<long>(4, std::min<long>(2048, (cap + b->lk_div - 1) / b->lk_div)));
+    cnt[K_PMASK].push_back((uint32_t)std::max<long>(1, std::min<long>(512, (cap + 255) / 256)));
+    cnt[K_FAST].push_back((uint32_t)(J.fast_tiles_x * ceil_div(R.nr, kFTH)));
+    cnt[K_DETC].push_back((uint32_t)(R.nbr + R.nblocks));
+    cnt[K_SCATI].push_back((uint32_t)(R.nblocks + ceil_div(R.nbr * b->P[J.stream].box_size, 4)));
+    cnt[K_MSCAN].push_back((uint32_t)std::max<long>(1, std::min<long>(512, (cap + 255) / 256)));
+  }
+  for (int k = 0; k < K_NKINDS; k++) {
+    std::vector<uint32_t>& p = b->prefix_h[k];
+    p.assign(cnt[k].size() + 1, 0);
+    for (size_t x = 0; x < cnt[k].size(); x++) p[x + 1] = p[x] + cnt[k][x];
+    b->total[k] = p.back();
+    if (!b->prefix_d[k]) CUIMG_TRY(dev_alloc_n(b, p.size(), &b->prefix_d[k]));
+    CUIMG_HIP(b->ctx, hipMemcpyAsync(b->prefix_d[k], p.data(), p.size() * 4, hipMemcpyHostToDevice, b->ctx->stream));
+  }
+  CUIMG_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+  return 0;
+}
+
+static int upload_tables(cuimg_hip_batch* b) {
+  CUIMG_HIP(b->ctx, hipMemcpyAsync(b->dj, b->hj.data(), sizeof(level_job) * b->njobs, hipMemcpyHostToDevice,
+                                   b->ctx->stream));
+  CUIMG_HIP(b->ctx, hipMemcpyAsync(b->ds, b->hs.data(), sizeof(stream_job) * b->nstreams, hipMemcpyHostToDevice,
+                                   b->ctx->stream));
+  CUIMG_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+  return 0;
+}
+
+static int rebuild(cuimg_hip_batch* b) {
+  for (int i = 0; i < b->nstreams; i++) {
+    for (int s = 0; s < b->P[i].nscales; s++) build_job(b, i, s);
+    build_stream(b, i);
+  }
+  CUIMG_TRY(build_maps(b));
+  CUIMG_TRY(upload_tables(b));
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
   }
   return 0;
 }
 
-static int run(cuimg_hip_tracker* t, const cuimg_img* frame) {
-  if (!t || !frame) return CUIMG_E_INVALID;
-  const level_res& R0 = t->lv[0];
-  if (frame->dtype != CUIMG_U8 || frame->nrows != R0.nr || frame->ncols != R0.nc || !frame->begin)
-    return set_error(t->ctx, CUIMG_E_INVALID, "tracker_run: frame must be U8 %dx%d", R0.nr, R0.nc);
-  const int slot = t->frame_cpt & 1;
-  cuimg_img levels[kMaxLevels];
-  for (int s = 0; s < t->L; s++) levels[s] = t->lv[s].u8[slot];
-  CUIMG_TRY(pyramid_build(t->ctx, frame, levels, t->L));
-  if (t->P.profile == 1) {
-    cuimg_img outs[kMaxLevels];
-    for (int s = 0; s < t->L; s++) outs[s] = t->lv[s].blur[slot];
-    CUIMG_TRY(convolve_sep_multi(t->ctx, levels, outs, t->L, t->coef, t->P.blur_K, 0));
+static int create_batch(cuimg_hip_ctx* ctx, int nstreams, const int* nrows, const int* ncols,
+                        const cuimg_tracker_params* params, cuimg_hip_batch** out) {
+  if (!ctx || !out || nstreams < 1 || !nrows || !ncols || !params) return CUIMG_E_INVALID;
+  *out = nullptr;
+  const cuimg_tracker_params& P0 = params[0];
+  for (int i = 0; i < nstreams; i++) {
+    CUIMG_TRY(check_params(ctx, nrows[i], ncols[i], &params[i]));
+    const cuimg_tracker_params& P = params[i];
+    if (P.profile != P0.profile || P.lk_ws != P0.lk_ws || P.lk_top_n != P0.lk_top_n ||
+        (P0.profile == 1 && (P.blur_sigma != P0.blur_sigma || P.blur_K != P0.blur_K)))
+      return set_error(ctx, CUIMG_E_UNSUPPORTED,
+                       "batch: stream %d differs in profile / LK window / selection / blur from stream 0", i);
   }
-  for (int s = t->L - 1; s >= 0; s--) CUIMG_TRY(update_level(t, s, slot));
-  t->frame_cpt++;
+  auto* b = new cuimg_hip_batch();
+  b->ctx = ctx;
+  b->nstreams = nstreams;
+  b->P0 = P0;
+  b->P.assign(params, params + nstreams);
+  auto fail = [&](int code) {
+    destroy_batch(b);
+    return code;
+  };
+  for (int i = 0; i < nstreams; i++) {
+    b->first.push_back(b->njobs);
+    b->njobs += params[i].nscales;
+  }
+  b->lv.resize(b->njobs);
+  b->hj.resize(b->njobs);
+  b->hs.resize(nstreams);
+  b->frame_buf.resize(nstreams);
+  int rc = 0;
+  if ((rc = dev_alloc_n(b, b->njobs, &b->cnt))) return fail(rc);
+  if (hipMemsetAsync(b->cnt, 0, sizeof(level_counters) * b->njobs, ctx->stream) != hipSuccess)
+    return fail(set_error(ctx, CUIMG_E_NOMEM, "batch: counter reset failed"));
+  for (int i = 0; i < nstreams; i++) {
+    int r = nrows[i], c = ncols[i];
+    for (int s = 0; s < params[i].nscales; s++) {
+      if ((rc = alloc_level(b, &params[i], r, c, b->lv[b->first[i] + s]))) return fail(rc);
+      r /= 2;
+      c /= 2;
+    }
+  }
+  if (P0.profile == 1) gaussian_coefs(P0.blur_sigma, P0.blur_K, b->cp.c);
+  if ((rc = dev_alloc_n(b, b->njobs, &b->dj))) return fail(rc);
+  if ((rc = dev_alloc_n(b, nstreams, &b->ds))) return fail(rc);
+  if ((rc = dev_alloc_n(b, nstreams, &b->din))) return fail(rc);
+  if (hipHostMalloc(reinterpret_cast<void**>(&b->hin), sizeof(frame_in) * nstreams * kRing) != hipSuccess)
+    return fail(set_error(ctx, CUIMG_E_NOMEM, "batch: pinned input table allocation failed"));
+  for (auto& e : b->ring_ev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+      return fail(set_error(ctx, CUIMG_E_NOMEM, "batch: event creation failed"));
+  if ((rc = rebuild(b))) return fail(rc);
+  *out = b;
   return 0;
 }
 
-static int read_counters(cuimg_hip_tracker* t, int s, level_counters* c) {
-  CUIMG_HIP(t->ctx, hipMemcpyAsync(c, t->cnt + s, sizeof(level_counters), hipMemcpyDeviceToHost, t->ctx->stream));
-  CUIMG_HIP(t->ctx, hipStreamSynchronize(t->ctx->stream));
+static void launch_blur(cuimg_hip_batch* b) {
+  const dim3 grid(b->total[K_BLUR]);
+  hipStream_t st = b->ctx->stream;
+  switch (b->P0.blur_K) {
+#define CUIMG_K(k) \
+  case k: step_blur_kernel<k><<<grid, 64, 0, st>>>(b->dj, b->prefix_d[K_BLUR], b->njobs, b->cp); break;
+    CUIMG_K(1) CUIMG_K(2) CUIMG_K(3) CUIMG_K(4) CUIMG_K(5) CUIMG_K(6) CUIMG_K(7) CUIMG_K(8)
+    CUIMG_K(9) CUIMG_K(10) CUIMG_K(11) CUIMG_K(12) CUIMG_K(13) CUIMG_K(14) CUIMG_K(15) CUIMG_K(16)
+#undef CUIMG_K
+  }
+}
+
+// The fixed launch sequence of one frame step of every stream.
+static int enqueue_step(cuimg_hip_batch* b) {
+  cuimg_hip_ctx* ctx = b->ctx;
+  hipStream_t st = ctx->stream;
+  const int nj = b->njobs;
+  const level_job* J = b->dj;
+  uint32_t* const* pf = b->prefix_d;
+  CUIMG_TRY(launch(ctx, "pyramid_build", [&] {
+    step_pyramid_kernel<<<b->total[K_PYR], 256, 0, st>>>(b->ds, J, b->din, pf[K_PYR], b->nstreams);
+  }));
+  if (b->total[K_BLUR]) CUIMG_TRY(launch(ctx, "gaussian_sep", [&] { launch_blur(b); }));
+  CUIMG_TRY(launch(ctx, "fills", [&] { step_fill_kernel<<<b->total[K_FILL], 256, 0, st>>>(J, pf[K_FILL], nj); }));
+  CUIMG_TRY(launch(ctx, "lk_track", [&] {
+    lk_variant(b->P0.lk_ws, b->P0.lk_top_n, b->P0.profile == 0, [&](auto ti, auto npl, auto sel) {
+      using TI = typename decltype(ti)::type;
+      step_lk_kernel<TI, decltype(npl)::value, decltype(sel)::value><<<b->total[K_LK], 256, 0, st>>>(J, pf[K_LK], nj);
+    });
+  }));
+  CUIMG_TRY(launch(ctx, "particle_mask",
+                   [&] { step_pmask_kernel<<<b->total[K_PMASK], 256, 0, st>>>(J, pf[K_PMASK], nj); }));
+  CUIMG_TRY(launch(ctx, "fast_saliency", [&] { step_fast_kernel<<<b->total[K_FAST], 256, 0, st>>>(J, pf[K_FAST], nj); }));
+  CUIMG_TRY(launch(ctx, "detect_count", [&] { step_detc_kernel<<<b->total[K_DETC], 256, 0, st>>>(J, pf[K_DETC], nj); }));
+  CUIMG_TRY(launch(ctx, "compact_scan", [&] { step_cscan_kernel<<<nj, 1024, 0, st>>>(J); }));
+  CUIMG_TRY(launch(ctx, "scatter_insert",
+                   [&] { step_scati_kernel<<<b->total[K_SCATI], 256, 0, st>>>(J, pf[K_SCATI], nj); }));
+  CUIMG_TRY(launch(ctx, "merge_scan", [&] { step_mscan_kernel<<<b->total[K_MSCAN], 256, 0, st>>>(J, pf[K_MSCAN], nj); }));
+  return launch(ctx, "merge_resolve", [&] { step_mres_kernel<<<nj, 1024, 0, st>>>(J); });
+}
+
+static int batch_step(cuimg_hip_batch* b, const cuimg_img* frames) {
+  if (!b || !frames) return CUIMG_E_INVALID;
+  cuimg_hip_ctx* ctx = b->ctx;
+  const int k = (int)(b->steps % kRing);
+  frame_in* row = b->hin + (size_t)k * b->nstreams;
+  CUIMG_HIP(ctx, hipEventSynchronize(b->ring_ev[k]));  // the table copied kRing steps ago has been read
+  for (int i = 0; i < b->nstreams; i++) {
+    const cuimg_img& f = frames[i];
+    const level_res& R0 = b->lv[b->first[i]];
+    if (f.dtype != CUIMG_U8 || f.nrows != R0.nr || f.ncols != R0.nc || !f.begin)
+      return set_error(ctx, CUIMG_E_INVALID, "step: frame %d must be U8 %dx%d", i, R0.nr, R0.nc);
+    row[i] = frame_in{f.begin, f.pitch, ((uintptr_t)f.begin % 16 == 0) && (f.pitch % 16 == 0)};
+  }
+  CUIMG_HIP(ctx, hipMemcpyAsync(b->din, row, sizeof(frame_in) * b->nstreams, hipMemcpyHostToDevice, ctx->stream));
+  CUIMG_HIP(ctx, hipEventRecord(b->ring_ev[k], ctx->stream));
+  if (b->use_graph && !ctx->profiling) {
+    if (!b->exec) {
+      hipGraph_t g = nullptr;
+      CUIMG_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue_step(b);
+      const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      CUIMG_HIP(ctx, e);
+      const hipError_t ei = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      CUIMG_HIP(ctx, ei);
+    }
+    CUIMG_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
+  } else {
+    CUIMG_TRY(enqueue_step(b));
+  }
+  b->steps++;
   return 0;
 }
 
-static int check_scale(cuimg_hip_tracker* t, int s) {
-  if (!t) return CUIMG_E_INVALID;
-  if (s < 0 || s >= t->L) return set_error(t->ctx, CUIMG_E_INVALID, "tracker: scale %d not in 0..%d", s, t->L - 1);
+static int read_counters(cuimg_hip_batch* b, int j, level_counters* c) {
+  CUIMG_HIP(b->ctx, hipMemcpyAsync(c, b->cnt + j, sizeof(level_counters), hipMemcpyDeviceToHost, b->ctx->stream));
+  CUIMG_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+  return 0;
+}
+
+static int job_of(cuimg_hip_tracker* t, int s, int* j) {
+  if (!t || !t->b) return CUIMG_E_INVALID;
+  const int L = t->b->P[t->stream].nscales;
+  if (s < 0 || s >= L) return set_error(t->b->ctx, CUIMG_E_INVALID, "tracker: scale %d not in 0..%d", s, L - 1);
+  *j = t->b->first[t->stream] + s;
+  return 0;
+}
+
+static int clear_stream(cuimg_hip_batch* b, int i) {
+  for (int s = 0; s < b->P[i].nscales; s++) {
+    level_res& R = b->lv[b->first[i] + s];
+    CUIMG_TRY(fill_sparse(b->ctx, R));
+    CUIMG_TRY(launch_merge_reset(b->ctx, R.merge));
+  }
+  CUIMG_HIP(b->ctx, hipMemsetAsync(b->cnt + b->first[i], 0, sizeof(level_counters) * b->P[i].nscales, b->ctx->stream));
+  CUIMG_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
   return 0;
 }
 
@@ -304,153 +759,275 @@ using namespace cuimg;
 
 extern "C" {
 
-int cuimg_hip_tracker_create(cuimg_hip_ctx* ctx, int nrows, int ncols, const cuimg_tracker_params* params,
-                             cuimg_hip_tracker** out) {
-  return create(ctx, nrows, ncols, params, out);
+// ------------------------------------------------------------------ batch
+int cuimg_hip_batch_create(cuimg_hip_ctx* ctx, int nstreams, const int* nrows, const int* ncols,
+                           const cuimg_tracker_params* params, cuimg_hip_batch** out) {
+  return create_batch(ctx, nstreams, nrows, ncols, params, out);
 }
 
-int cuimg_hip_tracker_destroy(cuimg_hip_tracker* t) {
-  destroy(t);
+int cuimg_hip_batch_destroy(cuimg_hip_batch* b) {
+  destroy_batch(b);
   return 0;
 }
 
-int cuimg_hip_tracker_run(cuimg_hip_tracker* t, const cuimg_img* frame) { return run(t, frame); }
+int cuimg_hip_batch_step(cuimg_hip_batch* b, const cuimg_img* frames) { return batch_step(b, frames); }
+
+int cuimg_hip_batch_set_graph(cuimg_hip_batch* b, int enable) {
+  if (!b) return CUIMG_E_INVALID;
+  b->use_graph = enable != 0;
+  return 0;
+}
+
+int cuimg_hip_batch_tracker(cuimg_hip_batch* b, int stream, cuimg_hip_tracker** out) {
+  if (!b || !out) return CUIMG_E_INVALID;
+  if (stream < 0 || stream >= b->nstreams)
+    return set_error(b->ctx, CUIMG_E_INVALID, "batch: stream %d not in 0..%d", stream, b->nstreams - 1);
+  auto* t = new cuimg_hip_tracker();
+  t->b = b;
+  t->stream = stream;
+  b->views.push_back(t);
+  *out = t;
+  return 0;
+}
+
+int cuimg_hip_batch_set_tuning(cuimg_hip_batch* b, int lk_div, int blur_th) {
+  if (!b || lk_div < 1 || blur_th < 0 || blur_th > 1024) return CUIMG_E_INVALID;
+  b->lk_div = lk_div;
+  b->blur_th = blur_th;
+  return rebuild(b);
+}
+
+int cuimg_hip_batch_get_tuning(cuimg_hip_batch* b, int* lk_div, int* blur_th) {
+  if (!b) return CUIMG_E_INVALID;
+  if (lk_div) *lk_div = b->lk_div;
+  if (blur_th) *blur_th = ::cuimg::blur_th(b);
+  return 0;
+}
+
+int cuimg_hip_batch_clear(cuimg_hip_batch* b) {
+  if (!b) return CUIMG_E_INVALID;
+  for (int i = 0; i < b->nstreams; i++) CUIMG_TRY(clear_stream(b, i));
+  return 0;
+}
+
+// ---------------------------------------------------------------- tracker
+int cuimg_hip_tracker_create(cuimg_hip_ctx* ctx, int nrows, int ncols, const cuimg_tracker_params* params,
+                             cuimg_hip_tracker** out) {
+  if (!out) return CUIMG_E_INVALID;
+  *out = nullptr;
+  cuimg_hip_batch* b = nullptr;
+  CUIMG_TRY(create_batch(ctx, 1, &nrows, &ncols, params, &b));
+  auto* t = new cuimg_hip_tracker();
+  t->b = b;
+  t->stream = 0;
+  t->owner = true;
+  *out = t;
+  return 0;
+}
+
+int cuimg_hip_tracker_destroy(cuimg_hip_tracker* t) {
+  if (!t) return 0;
+  if (t->owner) {
+    destroy_batch(t->b);
+    delete t;
+  }
+  return 0;  // views belong to their batch
+}
+
+int cuimg_hip_tracker_run(cuimg_hip_tracker* t, const cuimg_img* frame) {
+  if (!t || !frame) return CUIMG_E_INVALID;
+  if (t->b->nstreams != 1)
+    return set_error(t->b->ctx, CUIMG_E_INVALID, "tracker_run: a stream of a batch steps with batch_step");
+  return batch_step(t->b, frame);
+}
 
 int cuimg_hip_tracker_run_host(cuimg_hip_tracker* t, const uint8_t* frame, size_t pitch) {
   if (!t || !frame) return CUIMG_E_INVALID;
-  CUIMG_HIP(t->ctx, hipMemcpy2DAsync(t->frame_buf.begin, t->frame_buf.pitch, frame, pitch, t->frame_buf.ncols,
-                                     t->frame_buf.nrows, hipMemcpyHostToDevice, t->ctx->stream));
-  return run(t, &t->frame_buf);
+  cuimg_hip_batch* b = t->b;
+  cuimg_img& fb = b->frame_buf[t->stream];
+  const level_res& R0 = b->lv[b->first[t->stream]];
+  if (!fb.begin) CUIMG_TRY(image_alloc(b->ctx, R0.nr, R0.nc, 0, CUIMG_U8, &fb));
+  CUIMG_HIP(b->ctx, hipMemcpy2DAsync(fb.begin, fb.pitch, frame, pitch, fb.ncols, fb.nrows, hipMemcpyHostToDevice,
+                                     b->ctx->stream));
+  return cuimg_hip_tracker_run(t, &fb);
+}
+
+int cuimg_hip_tracker_set_graph(cuimg_hip_tracker* t, int enable) {
+  if (!t) return CUIMG_E_INVALID;
+  return cuimg_hip_batch_set_graph(t->b, enable);
 }
 
 int cuimg_hip_tracker_set_scale_params(cuimg_hip_tracker* t, int scale, const cuimg_scale_params* in) {
-  CUIMG_TRY(check_scale(t, scale));
+  int j;
+  CUIMG_TRY(job_of(t, scale, &j));
+  cuimg_hip_batch* b = t->b;
   if (!in || in->fast_n < 1 || in->fast_th < 0.f || in->detector_frequency < 1 || in->filtering_frequency < 1)
-    return set_error(t->ctx, CUIMG_E_INVALID, "set_scale_params: bad settings");
-  t->lv[scale].sp = *in;
+    return set_error(b->ctx, CUIMG_E_INVALID, "set_scale_params: bad settings");
+  b->lv[j].sp = *in;
+  build_job(b, t->stream, scale);
+  CUIMG_HIP(b->ctx, hipMemcpyAsync(b->dj + j, &b->hj[j], sizeof(level_job), hipMemcpyHostToDevice, b->ctx->stream));
+  CUIMG_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
   return 0;
 }
 
 int cuimg_hip_tracker_get_scale_params(cuimg_hip_tracker* t, int scale, cuimg_scale_params* out) {
-  CUIMG_TRY(check_scale(t, scale));
+  int j;
+  CUIMG_TRY(job_of(t, scale, &j));
   if (!out) return CUIMG_E_INVALID;
-  *out = t->lv[scale].sp;
+  *out = t->b->lv[j].sp;
   return 0;
 }
 
 int cuimg_hip_tracker_clear(cuimg_hip_tracker* t) {
-  if (!t) return CUIMG_E_INVALID;
-  t->frame_cpt = 0;
-  for (int s = 0; s < t->L; s++) {
-    CUIMG_TRY(fill_sparse(t, t->lv[s]));
-    CUIMG_TRY(launch_merge_reset(t->ctx, t->lv[s].merge));
-    t->lv[s].compact_has_run = false;
-  }
-  CUIMG_HIP(t->ctx, hipMemsetAsync(t->cnt, 0, sizeof(level_counters) * t->L, t->ctx->stream));
-  CUIMG_HIP(t->ctx, hipStreamSynchronize(t->ctx->stream));
-  return 0;
+  if (!t || !t->b) return CUIMG_E_INVALID;
+  return clear_stream(t->b, t->stream);
 }
 
 int cuimg_hip_tracker_nparticles(cuimg_hip_tracker* t, int scale, uint32_t* n) {
-  CUIMG_TRY(check_scale(t, scale));
+  int j;
+  CUIMG_TRY(job_of(t, scale, &j));
   level_counters c;
-  CUIMG_TRY(read_counters(t, scale, &c));
+  CUIMG_TRY(read_counters(t->b, j, &c));
   if (n) *n = c.n;
   if (c.overflow)
-    return set_error(t->ctx, CUIMG_E_CAPACITY, "tracker: scale %d exceeded its particle capacity %u", scale,
-                     t->lv[scale].cap);
+    return set_error(t->b->ctx, CUIMG_E_CAPACITY, "tracker: scale %d exceeded its particle capacity %u", scale,
+                     t->b->lv[j].cap);
   return 0;
 }
 
 int cuimg_hip_tracker_get_particles(cuimg_hip_tracker* t, int scale, cuimg_particle_f* out, uint32_t cap,
                                     uint32_t* n) {
-  CUIMG_TRY(check_scale(t, scale));
+  int j;
+  CUIMG_TRY(job_of(t, scale, &j));
   level_counters c;
-  CUIMG_TRY(read_counters(t, scale, &c));
+  CUIMG_TRY(read_counters(t->b, j, &c));
   if (n) *n = c.n;
   if (!out) return 0;
-  if (c.n > cap) return set_error(t->ctx, CUIMG_E_CAPACITY, "get_particles: %u particles, buffer holds %u", c.n, cap);
-  const level_res& R = t->lv[scale];
-  CUIMG_HIP(t->ctx, hipMemcpyAsync(out, R.parts[R.cur], sizeof(cuimg_particle_f) * c.n, hipMemcpyDeviceToHost,
-                                   t->ctx->stream));
-  CUIMG_HIP(t->ctx, hipStreamSynchronize(t->ctx->stream));
+  if (c.n > cap)
+    return set_error(t->b->ctx, CUIMG_E_CAPACITY, "get_particles: %u particles, buffer holds %u", c.n, cap);
+  CUIMG_HIP(t->b->ctx, hipMemcpyAsync(out, t->b->lv[j].parts[c.cur & 1], sizeof(cuimg_particle_f) * c.n,
+                                      hipMemcpyDeviceToHost, t->b->ctx->stream));
+  CUIMG_HIP(t->b->ctx, hipStreamSynchronize(t->b->ctx->stream));
   return 0;
 }
 
 int cuimg_hip_tracker_set_particles(cuimg_hip_tracker* t, int scale, const cuimg_particle_f* in, uint32_t n) {
-  CUIMG_TRY(check_scale(t, scale));
-  level_res& R = t->lv[scale];
-  if (n > R.cap) return set_error(t->ctx, CUIMG_E_CAPACITY, "set_particles: %u exceeds capacity %u", n, R.cap);
+  int j;
+  CUIMG_TRY(job_of(t, scale, &j));
+  cuimg_hip_batch* b = t->b;
+  level_res& R = b->lv[j];
+  if (n > R.cap) return set_error(b->ctx, CUIMG_E_CAPACITY, "set_particles: %u exceeds capacity %u", n, R.cap);
+  level_counters c;
+  CUIMG_TRY(read_counters(b, j, &c));
   if (n)
-    CUIMG_HIP(t->ctx, hipMemcpyAsync(R.parts[R.cur], in, sizeof(cuimg_particle_f) * n, hipMemcpyHostToDevice,
-                                     t->ctx->stream));
-  CUIMG_HIP(t->ctx, hipMemcpyAsync(&t->cnt[scale].n, &n, sizeof(uint32_t), hipMemcpyHostToDevice, t->ctx->stream));
-  CUIMG_HIP(t->ctx, hipStreamSynchronize(t->ctx->stream));
+    CUIMG_HIP(b->ctx, hipMemcpyAsync(R.parts[c.cur & 1], in, sizeof(cuimg_particle_f) * n, hipMemcpyHostToDevice,
+                                     b->ctx->stream));
+  CUIMG_HIP(b->ctx, hipMemcpyAsync(&b->cnt[j].n, &n, sizeof(uint32_t), hipMemcpyHostToDevice, b->ctx->stream));
+  CUIMG_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
   return 0;
 }
 
 int cuimg_hip_tracker_get_matches(cuimg_hip_tracker* t, int scale, uint32_t* out, uint32_t cap, uint32_t* n) {
-  CUIMG_TRY(check_scale(t, scale));
+  int j;
+  CUIMG_TRY(job_of(t, scale, &j));
   level_counters c;
-  CUIMG_TRY(read_counters(t, scale, &c));
+  CUIMG_TRY(read_counters(t->b, j, &c));
   const uint32_t m = c.n_old + c.n_new;
   if (n) *n = m;
   if (!out) return 0;
-  if (m > cap) return set_error(t->ctx, CUIMG_E_CAPACITY, "get_matches: %u entries, buffer holds %u", m, cap);
-  CUIMG_HIP(t->ctx, hipMemcpyAsync(out, t->lv[scale].matches, sizeof(uint32_t) * m, hipMemcpyDeviceToHost,
-                                   t->ctx->stream));
-  CUIMG_HIP(t->ctx, hipStreamSynchronize(t->ctx->stream));
+  if (m > cap) return set_error(t->b->ctx, CUIMG_E_CAPACITY, "get_matches: %u entries, buffer holds %u", m, cap);
+  CUIMG_HIP(t->b->ctx, hipMemcpyAsync(out, t->b->lv[j].matches, sizeof(uint32_t) * m, hipMemcpyDeviceToHost,
+                                      t->b->ctx->stream));
+  CUIMG_HIP(t->b->ctx, hipStreamSynchronize(t->b->ctx->stream));
   return 0;
 }
 
 int cuimg_hip_tracker_compact_has_run(cuimg_hip_tracker* t, int scale, int* out) {
-  CUIMG_TRY(check_scale(t, scale));
-  *out = t->lv[scale].compact_has_run ? 1 : 0;
+  int j;
+  CUIMG_TRY(job_of(t, scale, &j));
+  if (!out) return CUIMG_E_INVALID;
+  level_counters c;
+  CUIMG_TRY(read_counters(t->b, j, &c));
+  *out = c.compact_has_run ? 1 : 0;
   return 0;
 }
 
 int cuimg_hip_tracker_sparse(cuimg_hip_tracker* t, int scale, cuimg_img* view) {
-  CUIMG_TRY(check_scale(t, scale));
-  *view = t->lv[scale].sparse;
+  int j;
+  CUIMG_TRY(job_of(t, scale, &j));
+  if (!view) return CUIMG_E_INVALID;
+  *view = t->b->lv[j].sparse;
   return 0;
 }
 
 int cuimg_hip_tracker_level(cuimg_hip_tracker* t, int scale, cuimg_img* view) {
-  CUIMG_TRY(check_scale(t, scale));
-  *view = t->lv[scale].u8[(t->frame_cpt + 1) & 1];  // slot of the last frame
+  int j;
+  CUIMG_TRY(job_of(t, scale, &j));
+  if (!view) return CUIMG_E_INVALID;
+  level_counters c;
+  CUIMG_TRY(read_counters(t->b, j, &c));
+  *view = t->b->lv[j].u8[(c.frame + 1) & 1];  // slot of the last frame
   return 0;
 }
 
 int cuimg_hip_tracker_saliency(cuimg_hip_tracker* t, int scale, cuimg_img* view) {
-  CUIMG_TRY(check_scale(t, scale));
-  *view = t->lv[scale].sal;
+  int j;
+  CUIMG_TRY(job_of(t, scale, &j));
+  if (!view) return CUIMG_E_INVALID;
+  *view = t->b->lv[j].sal;
   return 0;
 }
 
 int cuimg_hip_tracker_get_decisions(cuimg_hip_tracker* t, int scale, int32_t* accept, float* tr2, float* dist,
                                     uint32_t cap, uint32_t* n) {
-  CUIMG_TRY(check_scale(t, scale));
+  int j;
+  CUIMG_TRY(job_of(t, scale, &j));
   level_counters c;
-  CUIMG_TRY(read_counters(t, scale, &c));
+  CUIMG_TRY(read_counters(t->b, j, &c));
   if (n) *n = c.n_matched;
   if (!accept) return 0;
-  if (c.n_matched > cap) return set_error(t->ctx, CUIMG_E_CAPACITY, "get_decisions: buffer too small");
-  const level_res& R = t->lv[scale];
+  if (c.n_matched > cap) return set_error(t->b->ctx, CUIMG_E_CAPACITY, "get_decisions: buffer too small");
+  const level_res& R = t->b->lv[j];
   const size_t m = c.n_matched;
-  CUIMG_HIP(t->ctx, hipMemcpyAsync(accept, R.dec_accept, 4 * m, hipMemcpyDeviceToHost, t->ctx->stream));
-  CUIMG_HIP(t->ctx, hipMemcpyAsync(tr2, R.dec_tr, 8 * m, hipMemcpyDeviceToHost, t->ctx->stream));
-  CUIMG_HIP(t->ctx, hipMemcpyAsync(dist, R.dec_dist, 4 * m, hipMemcpyDeviceToHost, t->ctx->stream));
-  CUIMG_HIP(t->ctx, hipStreamSynchronize(t->ctx->stream));
+  hipStream_t st = t->b->ctx->stream;
+  CUIMG_HIP(t->b->ctx, hipMemcpyAsync(accept, R.dec_accept, 4 * m, hipMemcpyDeviceToHost, st));
+  CUIMG_HIP(t->b->ctx, hipMemcpyAsync(tr2, R.dec_tr, 8 * m, hipMemcpyDeviceToHost, st));
+  CUIMG_HIP(t->b->ctx, hipMemcpyAsync(dist, R.dec_dist, 4 * m, hipMemcpyDeviceToHost, st));
+  CUIMG_HIP(t->b->ctx, hipStreamSynchronize(st));
   return 0;
 }
 
 int cuimg_hip_tracker_matcher_images(cuimg_hip_tracker* t, int scale, cuimg_img* A, cuimg_img* B, cuimg_img* G) {
-  CUIMG_TRY(check_scale(t, scale));
-  const level_res& R = t->lv[scale];
-  const int cur = (t->frame_cpt + 1) & 1, prev = cur ^ 1;
-  if (A) *A = t->P.profile == 0 ? R.u8[prev] : R.blur[prev];
-  if (B) *B = t->P.profile == 0 ? R.u8[cur] : R.blur[cur];
+  int j;
+  CUIMG_TRY(job_of(t, scale, &j));
+  level_counters c;
+  CUIMG_TRY(read_counters(t->b, j, &c));
+  const level_res& R = t->b->lv[j];
+  const int cur = (c.frame + 1) & 1, prev = cur ^ 1;
+  const bool u8 = t->b->P0.profile == 0;
+  if (A) *A = u8 ? R.u8[prev] : R.blur[prev];
+  if (B) *B = u8 ? R.u8[cur] : R.blur[cur];
   if (G) std::memset(G, 0, sizeof(*G));  // computed on the fly, not stored
+  return 0;
+}
+
+int cuimg_hip_tracker_work(cuimg_hip_tracker* t, int scale, uint64_t* live, uint64_t* accepted) {
+  int j;
+  CUIMG_TRY(job_of(t, scale, &j));
+  level_counters c;
+  CUIMG_TRY(read_counters(t->b, j, &c));
+  if (live) *live = c.live_total;
+  if (accepted) *accepted = c.accepted_total;
+  return 0;
+}
+
+int cuimg_hip_tracker_frame_count(cuimg_hip_tracker* t, int* out) {
+  int j;
+  CUIMG_TRY(job_of(t, 0, &j));
+  if (!out) return CUIMG_E_INVALID;
+  level_counters c;
+  CUIMG_TRY(read_counters(t->b, j, &c));
+  *out = (int)c.frame;
   return 0;
 }
 
@@ -485,21 +1062,6 @@ int cuimg_hip_particles_merge(cuimg_hip_ctx* ctx, cuimg_particle_f* parts, uint3
   (void)hipFree(buf);
   (void)hipFree(cnt);
   return rc;
-}
-
-int cuimg_hip_tracker_work(cuimg_hip_tracker* t, int scale, uint64_t* live, uint64_t* accepted) {
-  CUIMG_TRY(check_scale(t, scale));
-  level_counters c;
-  CUIMG_TRY(read_counters(t, scale, &c));
-  if (live) *live = c.live_total;
-  if (accepted) *accepted = c.accepted_total;
-  return 0;
-}
-
-int cuimg_hip_tracker_frame_count(cuimg_hip_tracker* t, int* out) {
-  if (!t || !out) return CUIMG_E_INVALID;
-  *out = t->frame_cpt;
-  return 0;
 }
 
 }  // extern "C"

@@ -1,5 +1,5 @@
 // Device-side state of the pyramid LK tracker, shared by particles.hip,
-// lk.hip and tracker.hip.
+// lk.hip, fast.hip, pyramid.hip, gaussian.hip and step.hip.
 #pragma once
 
 #include "common.hpp"
@@ -8,8 +8,9 @@ namespace cuimg {
 
 constexpr int kMaxLevels = 8;
 
-// Per-scale counters in device memory: the particle count lives on the device so
-// a frame step needs no host round trip.
+// Per-scale counters in device memory: particle counts and the frame-step
+// state live on the device, so a frame step is a fixed launch sequence with no
+// host round trip (and can be replayed as a graph).
 struct level_counters {
   uint32_t n;        // dense_particles().size()
   uint32_t n_old;    // size before the last compaction (= matches().size() - n_new)
@@ -18,7 +19,10 @@ struct level_counters {
   uint32_t overflow; // capacity exceeded (particles dropped)
   uint32_t actors;   // merge: particles with an eligible neighbour
   uint32_t n_matched; // size of the dense vector seen by the last matcher launch
-  uint32_t pad;
+  uint32_t frame;    // frame steps this scale has completed
+  uint32_t cur;      // index (0/1) of the current dense vector
+  uint32_t compact_has_run;  // the last frame step ran the detector
+  uint32_t pad[2];
   unsigned long long live_total;      // particles (age > 0) run through the matcher
   unsigned long long accepted_total;  // of which accepted (moved)
 };
@@ -53,9 +57,6 @@ struct lk_args {
   lk_params_dev P;
 };
 
-// Host launchers implemented in lk.hip / particles.hip.
-int launch_lk(cuimg_hip_ctx* ctx, const lk_args& a, int image_u8);
-
 struct detect_args {
   // level geometry
   int nr, nc;
@@ -79,12 +80,6 @@ struct detect_args {
   int nblocks;                  // compaction blocks covering cap
 };
 
-int launch_fill_u32(cuimg_hip_ctx* ctx, uint32_t* begin, int pitch, int r0, int r1, int c0, int c1,
-                    uint32_t v);
-int launch_fill_u8(cuimg_hip_ctx* ctx, uint8_t* begin, int pitch, int nr, int nc, uint8_t v);
-int launch_particle_mask(cuimg_hip_ctx* ctx, const detect_args& a);
-int launch_detect_and_compact(cuimg_hip_ctx* ctx, const detect_args& a);
-
 // A particle with at least one eligible merge target: b[k] is the c8
 // neighbour k (valid where bit k of mask is set).
 struct merge_actor {
@@ -105,6 +100,28 @@ struct merge_args {
   uint32_t cap, pend_cap;    // pend_cap: power of two >= cap; cap < 2^24
 };
 constexpr uint32_t kMergeMaxCap = (1u << 24) - 1;
+
+constexpr int kCompactBlock = 1024;  // particles per compaction block
+constexpr int kMaxBox = 15;          // largest detector box
+
+// ----------------------------------------------------------- device bodies
+// Each takes its block index and block count explicitly, so one launch can
+// run many (stream, scale) jobs (step.hip) or a single one.
+__device__ void fill_u32_body(uint32_t* begin, int pitch, int r0, int r1, int c0, int c1, uint32_t v, int bid,
+                              int nb);
+__device__ void fill_u8_body(uint8_t* begin, int pitch, int nr, int nc, uint32_t v4, int bid, int nb);
+__device__ void particle_mask_body(const detect_args& a, int bid, int nb);
+__device__ void nms_row_body(const detect_args& a, float fast_th, int bi);
+__device__ void compact_count_body(const detect_args& a, int bid);
+__device__ void compact_scan_body(const detect_args& a);
+__device__ void compact_scatter_body(const detect_args& a, int bid);
+__device__ void insert_new_body(const detect_args& a, int bid);
+__device__ void merge_scan_body(const merge_args& a, int bid, int nb);
+__device__ void merge_resolve_body(const merge_args& a);
+
+// Host helpers of particles.hip.
+int launch_fill_u32(cuimg_hip_ctx* ctx, uint32_t* begin, int pitch, int r0, int r1, int c0, int c1,
+                    uint32_t v);
 // Bytes of merge scratch for a capacity (pend, actors, zeroer, minunres).
 size_t merge_scratch_bytes(uint32_t cap);
 merge_args merge_scratch_carve(void* base, uint32_t cap);

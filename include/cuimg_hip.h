@@ -224,6 +224,57 @@ int cuimg_hip_tracker_work(cuimg_hip_tracker* t, int scale, uint64_t* live, uint
 int cuimg_hip_particles_merge(cuimg_hip_ctx* ctx, cuimg_particle_f* parts, uint32_t n,
                               const cuimg_img* sparse);
 
+/* Frame steps are captured once into a hipGraph and replayed (default on);
+ * disabled, every step launches its kernels (per-kernel timing needs this). */
+int cuimg_hip_tracker_set_graph(cuimg_hip_tracker* t, int enable);
+
+/* --------------------------------------------------------------- matcher */
+/* iterative_lk_match_5x5 settings (tracking2/pyrlk_tracker.hpp:114-128). */
+typedef struct {
+  int32_t ws;           /* window side (odd, <= 15) */
+  int32_t max_iters;
+  float eps;            /* stop when |delta| < eps */
+  int32_t top_n;        /* strongest-gradient pixels; >= ws*ws = all */
+  int32_t k;            /* accept when err < k (pyrlk_cpu::set_k) */
+  int32_t template_u8;  /* template samples stored as uint8 (the reference's u8 frames) */
+} cuimg_lk_params;
+
+/* pyrlk_match_particles_kernel (tracking2/pyrlk_tracker.hpp:676-769) for the n
+ * particles of scale `scale`, with the recursive coarse-to-fine prediction over
+ * levels scale+1 .. nlevels-1 (predictions.h:60-74): prev/cur are the level
+ * images (U8 or F32, nlevels each) of the previous and current frame.  An
+ * accepted particle is moved (age+1, speed, position) and, when `sparse` is
+ * given (U32 image of the scale's domain), written there with the last index
+ * winning; a rejected one gets age 0.  accept/tr2/dist (device, n entries, may
+ * all be NULL) receive the decisions.  parts are device memory.  Synchronises. */
+int cuimg_hip_lk_track(cuimg_hip_ctx* ctx, const cuimg_img* prev, const cuimg_img* cur, int nlevels,
+                       int scale, cuimg_particle_f* parts, uint32_t n, const cuimg_lk_params* params,
+                       const cuimg_img* sparse, int32_t* accept, float* tr2, float* dist);
+
+/* ----------------------------------------------------------------- batch */
+/* Many independent trackers (video streams, each its own domain and nscales;
+ * one matcher profile / window / blur for all) stepped together: one frame
+ * step of every stream is one fixed sequence of batched launches, captured
+ * once into a hipGraph.  No counterpart in the reference, which runs one
+ * tracker<F> per stream (tracking2/tracker.hpp:73-91); each stream's results
+ * equal those of its own tracker. */
+typedef struct cuimg_hip_batch cuimg_hip_batch;
+int cuimg_hip_batch_create(cuimg_hip_ctx* ctx, int nstreams, const int* nrows, const int* ncols,
+                           const cuimg_tracker_params* params, cuimg_hip_batch** out);
+int cuimg_hip_batch_destroy(cuimg_hip_batch* b);
+/* One frame step of every stream: frames[i] is a U8 device image of stream i's
+ * domain.  Asynchronous on the context stream. */
+int cuimg_hip_batch_step(cuimg_hip_batch* b, const cuimg_img* frames);
+int cuimg_hip_batch_set_graph(cuimg_hip_batch* b, int enable);
+/* Tracker view of stream i for the tracker accessors (owned by the batch;
+ * tracker_run on it is refused, tracker_destroy does nothing). */
+int cuimg_hip_batch_tracker(cuimg_hip_batch* b, int stream, cuimg_hip_tracker** out);
+/* Launch shape: lk_div = particle capacity per matcher workgroup (default 48),
+ * blur_th = blur tile height in rows (0 = chosen from the total work). */
+int cuimg_hip_batch_set_tuning(cuimg_hip_batch* b, int lk_div, int blur_th);
+int cuimg_hip_batch_get_tuning(cuimg_hip_batch* b, int* lk_div, int* blur_th);
+int cuimg_hip_batch_clear(cuimg_hip_batch* b);
+
 /* ------------------------------------------------------------- adjoint */
 /* Photometric LK loss of the blurred frames and its gradient (no counterpart in
  * the reference; it is the term iterative_lk_match_5x5 minimises,
