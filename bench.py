@@ -114,6 +114,122 @@ def summary(tr):
             "particles_per_scale": tr["nparts"], "kernels": tr["kernels"]}
 
 
+BATCH_STREAMS, BATCH_FRAMES, BATCH_SEED = 64, 8, 0xC0FFEE04
+
+
+def batch_workload(cu):
+    """64 streams, width U[320,1920], height round(9w/16), 3 scales when
+    min(h, w) < 360 else 4; each stream gets U[1k,150k] particles by seeded
+    placement after its first frame (detector threshold 256, above any u8
+    saliency, so no point is added: the count is the placement's), clamped
+    to the container capacity of one particle per 4 pixels; scale s gets
+    n / 4^s."""
+    rng = np.random.default_rng(BATCH_SEED)
+    shapes, params, parts = [], [], []
+    for i in range(BATCH_STREAMS):
+        w = int(rng.integers(320, 1921))
+        h = int(round(9 * w / 16))
+        L = 3 if min(h, w) < 360 else 4
+        n = min(int(rng.integers(1000, 150001)), h * w // 4)
+        shapes.append((h, w))
+        params.append(cu.profile_params("B", L, 256.0))
+        lv = []
+        for s in range(L):
+            hs, ws = h >> s, w >> s
+            m = min(max(n >> (2 * s), 1), hs * ws // 4)
+            p = np.zeros(m, cu.PARTICLE_DTYPE)
+            p["pos_r"] = rng.integers(3, hs - 3, m) + rng.random(m).astype(np.float32) * np.float32(0.999)
+            p["pos_c"] = rng.integers(3, ws - 3, m) + rng.random(m).astype(np.float32) * np.float32(0.999)
+            p["age"] = 1
+            lv.append(p)
+        parts.append(lv)
+    return shapes, params, parts
+
+
+def batch_bench(cu, ctx):
+    """One frame step of 64 streams as one graph replay of the batched kernels,
+    against the same 64 streams stepped one tracker after the other with plain
+    launches; 7 timed steps (frames 1..7) after the seeded placement."""
+    shapes, params, parts = batch_workload(cu)
+    t0 = time.time()
+    frames = [[None] * BATCH_STREAMS for _ in range(BATCH_FRAMES)]
+    for i, (h, w) in enumerate(shapes):
+        v = cu.synth_video(BATCH_SEED + 1 + i, h, w, BATCH_FRAMES, 1)
+        for f in range(BATCH_FRAMES):
+            frames[f][i] = ctx.upload(v[f])
+    log(f"batch: {BATCH_STREAMS} streams, {sum(h * w for h, w in shapes) / 1e6:.1f} Mpx per frame, "
+        f"{sum(len(p[0]) for p in parts)} particles at scale 0, inputs ready in {time.time() - t0:.1f}s")
+
+    def place(trackers):
+        for i, t in enumerate(trackers):
+            for s, p in enumerate(parts[i]):
+                t.set_particles(s, p)
+        ctx.sync()
+
+    def run_batch(b):
+        b.step(frames[0])
+        place([b.tracker(i) for i in range(BATCH_STREAMS)])
+        t1 = time.perf_counter()
+        for f in range(1, BATCH_FRAMES):
+            b.step(frames[f])
+        ctx.sync()
+        return time.perf_counter() - t1
+
+    batch = cu.Batch(ctx, shapes, params)
+    best, tune = batch.autotune(run_batch)
+    tb = []
+    for _ in range(3):
+        batch.clear()
+        tb.append(run_batch(batch))
+    live = sum(batch.tracker(i).work(s)[0] for i in range(BATCH_STREAMS) for s in range(params[i].nscales))
+
+    solo = [cu.Tracker(ctx, h, w, params[i]) for i, (h, w) in enumerate(shapes)]
+
+    def run_solo():
+        for i, t in enumerate(solo):
+            t.set_graph(False)
+            t.clear()
+            t.run(frames[0][i])
+        place(solo)
+        t1 = time.perf_counter()
+        for f in range(1, BATCH_FRAMES):
+            for i, t in enumerate(solo):
+                t.run(frames[f][i])
+        ctx.sync()
+        return time.perf_counter() - t1
+
+    run_solo()  # warm-up
+    ts = min(run_solo() for _ in range(2))
+    same = 0
+    for i in range(BATCH_STREAMS):
+        ok = True
+        for s in range(params[i].nscales):
+            a, b = batch.tracker(i).particles(s), solo[i].particles(s)
+            ok = ok and len(a) == len(b) and all(np.array_equal(a[k], b[k]) for k in a.dtype.names)
+        same += ok
+    steps = BATCH_FRAMES - 1
+    t_b = min(tb)
+    out = {"workload": f"{BATCH_STREAMS} streams, w~U[320,1920], h=round(9w/16), 3-4 scales, profile B, "
+                       f"{steps} timed frame steps after seeded placement of U[1k,150k] particles",
+           "pixels_per_frame": sum(h * w for h, w in shapes),
+           "particles_scale0": sum(len(p[0]) for p in parts),
+           "batch_ms_per_step": round(t_b / steps * 1000, 3),
+           "sequential_ms_per_step": round(ts / steps * 1000, 3),
+           "speedup": round(ts / t_b, 2),
+           "stream_frames_per_s": round(BATCH_STREAMS * steps / t_b, 1),
+           "particle_updates_per_s": round(live / tb[-1], 1),
+           "streams_identical_to_own_tracker": f"{same}/{BATCH_STREAMS}",
+           "tuning": {"lk_div": best[0], "blur_th": batch.tuning()[1]},
+           "tuning_ms_per_step": {f"{c[0]}/{c[1]}": round(v / steps * 1000, 3) for c, v in tune.items()}}
+    for t in solo:
+        t.close()
+    batch.close()
+    for row in frames:
+        for im in row:
+            im.free()
+    return out
+
+
 def gaussian_microbench(cu, ctx, reps=20):
     """Separable Gaussian on f32 4096x4096 and f32x4 3840x2160 (sigma 1, 2, 4;
     2*ceil(3 sigma)+1 taps), 4 distinct buffer pairs rotated so the working set
@@ -226,6 +342,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-micro", action="store_true")
     ap.add_argument("--no-sparse", action="store_true", help="skip the FAST th 20 run")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-stream batched step")
     ap.add_argument("--fast-th", type=float, default=FAST_TH)
     ap.add_argument("--prof-steps", type=int, default=30,
                     help="frame steps after the timed region run with per-kernel HIP events")
@@ -250,6 +367,12 @@ def main():
         log(f"  {sparse['particle_updates_per_s']:.4g} particle-updates/s, {sparse['frames_per_s']} frames/s")
     for im in frames:
         im.free()
+    batch = None
+    if not args.no_batch:
+        log("batched 64-stream step")
+        batch = batch_bench(cu, ctx)
+        log(f"  batch {batch['batch_ms_per_step']} ms/step vs sequential {batch['sequential_ms_per_step']} ms/step "
+            f"({batch['speedup']}x), identical {batch['streams_identical_to_own_tracker']}")
     micro = [] if args.no_micro else gaussian_microbench(cu, ctx)
     ctx.close()
 
@@ -282,6 +405,7 @@ def main():
         "roofline": lk_roofline(tr),
         "kernels": s["kernels"],
         "fast_th_20": sparse,
+        "batch_64_streams": batch,
         "gaussian_pyramid_roofline": micro,
         "cpu_baseline": cpu,
     }

@@ -502,6 +502,21 @@ class Batch:
     def clear(self):
         self.ctx.check(self.L.cuimg_hip_batch_clear(self.h), "batch_clear")
 
+    def autotune(self, run, candidates=((48, 0), (24, 0), (96, 0), (48, 16), (48, 32), (48, 64))):
+        """Launch-shape autotuner: for each (lk_div, blur_th) candidate the
+        batch is cleared and `run(batch)` is called; it steps the batch over a
+        representative sequence and returns the seconds it measured.  The
+        fastest candidate is kept; returns (best, {candidate: seconds})."""
+        results = {}
+        for c in candidates:
+            self.set_tuning(*c)
+            self.clear()
+            results[c] = run(self)
+        best = min(results, key=results.get)
+        self.set_tuning(*best)
+        self.clear()
+        return best, results
+
     def tracker(self, i):
         if i not in self._views:
             h = ctypes.c_void_p()
