@@ -47,17 +47,23 @@ __device__ __forceinline__ float mov_dpp(float x) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, false));
 }
 
-// Sum over the 64 lanes, the same bits in every lane: every step pairs lanes
-// symmetrically (xor-1 and xor-2 quad permutes, half-row and row mirrors, then
-// xor 16 and xor 32), so both partners add the same two values.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float upd_dpp(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROW_MASK, 0xF, false));
+}
+
+// Sum over the 64 lanes as one wave-uniform (scalar) value: xor-1 and xor-2
+// quad permutes, half-row and row mirrors give every lane its row's sum, then
+// row_bcast15 / row_bcast31 fold rows (1 += 0, 3 += 2, then 3 += 1) and lane
+// 63 holds ((r3 + r2) + (r1 + r0)).  No LDS traffic.
 __device__ __forceinline__ float wave_sum(float x) {
-  x += mov_dpp<0xB1>(x);   // quad_perm [1,0,3,2]
-  x += mov_dpp<0x4E>(x);   // quad_perm [2,3,0,1]
-  x += mov_dpp<0x141>(x);  // row_half_mirror
-  x += mov_dpp<0x140>(x);  // row_mirror
-  x += __shfl_xor(x, 16, 64);
-  x += __shfl_xor(x, 32, 64);
-  return x;
+  x += mov_dpp<0xB1>(x);          // quad_perm [1,0,3,2]
+  x += mov_dpp<0x4E>(x);          // quad_perm [2,3,0,1]
+  x += mov_dpp<0x141>(x);         // row_half_mirror
+  x += mov_dpp<0x140>(x);         // row_mirror
+  x += upd_dpp<0x142, 0xA>(x);    // row_bcast:15 into rows 1, 3
+  x += upd_dpp<0x143, 0xC>(x);    // row_bcast:31 into rows 2, 3
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
 }
 
 __device__ __forceinline__ int tz(float v) { return (int)v; }
@@ -111,28 +117,40 @@ struct match_out {
 
 constexpr int kMB = 4;  // motion margin of the current-frame patch
 
-// LDS of one wave, sized for the largest window of its NPL class.
+// LDS of one wave, sized for the largest window of its NPL class.  The
+// previous- and current-frame patches have a row stride of PW (16 or 32
+// floats) so a wave loads 64/PW whole rows per pass with constant offsets.
 template <int NPL>
 struct lk_lds {
   static constexpr int WSMAX = NPL == 1 ? 7 : (NPL == 2 ? 11 : 15);
   static constexpr int QA = WSMAX + 5;            // previous-frame patch side
   static constexpr int QG = WSMAX + 3;            // gradient patch side
   static constexpr int QB = WSMAX + 1 + 2 * kMB;  // current-frame patch side
+  static constexpr int PW = QB <= 16 ? 16 : 32;   // patch row stride (>= QA, QB)
+  static constexpr int RPP = 64 / PW;             // rows per load pass
+  static constexpr int AROWS = (QA + RPP - 1) / RPP * RPP;
+  static constexpr int BROWS = (QB + RPP - 1) / RPP * RPP;
+  static constexpr int GPASS = (QG * QG + 63) / 64;
   float2 G[QG * QG];
-  float A[QA * QA];
-  float B[QB * QB];
+  float A[AROWS * PW];
+  float B[BROWS * PW];
   float tpl[64 * NPL];
   float mag[64 * NPL];
 };
 
-// dst[i*q + j] = img[clamp(r0 + i), clamp(c0 + j)]: indexing the patch with an
-// unclamped coordinate reproduces the reference's clamped read.
-template <typename TI>
-__device__ __forceinline__ void load_patch(float* dst, int q, float invq, const void* img, int pitch, int nr, int nc,
-                                           int r0, int c0, int lane) {
-  for (int k = lane; k < q * q; k += 64) {
-    const int i = divq(k, invq), j = k - i * q;
-    dst[k] = ld<TI>(img, pitch, clampi(r0 + i, 0, nr - 1), clampi(c0 + j, 0, nc - 1));
+// dst[i*PW + j] = img[clamp(r0 + i), clamp(c0 + j)] for i, j < q: indexing the
+// patch with an unclamped coordinate reproduces the reference's clamped read.
+// Lane l loads column l % PW of rows l / PW + m * (64 / PW).
+template <typename TI, int PW>
+__device__ __forceinline__ void load_patch(float* dst, int q, const void* img, int pitch, int nr, int nc, int r0,
+                                           int c0, int lane) {
+  constexpr int RPP = 64 / PW;
+  const int col = lane % PW, row = lane / PW;
+  const int c = clampi(c0 + col, 0, nc - 1);
+  const int passes = (q + RPP - 1) / RPP;
+  for (int m = 0; m < passes; m++) {
+    const int i = row + m * RPP;
+    if (col < q && i < q) dst[lane + 64 * m] = ld<TI>(img, pitch, clampi(r0 + i, 0, nr - 1), c);
   }
 }
 
@@ -144,22 +162,47 @@ __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Per-lane window geometry, fixed for a launch.
+// Per-lane geometry, fixed for a launch: the window pixels of the lane and its
+// gradient-patch elements (row << 8 | col).
 template <int NPL>
 struct lane_win {
   int r[NPL], c[NPL];
   bool in[NPL];
+  int g[lk_lds<NPL>::GPASS];
 };
+
+template <int NPL>
+__device__ __forceinline__ lane_win<NPL> make_lane_win(int ws, int lane) {
+  lane_win<NPL> W;
+  const int hws = ws / 2, QG = ws + 3;
+  const float iws = 1.f / (float)ws, iqg = 1.f / (float)QG;
+#pragma unroll
+  for (int k = 0; k < NPL; k++) {
+    const int i = lane + 64 * k;
+    const int q = divq(i, iws);
+    W.in[k] = i < ws * ws;
+    W.r[k] = q - hws;
+    W.c[k] = i - q * ws - hws;
+  }
+#pragma unroll
+  for (int m = 0; m < lk_lds<NPL>::GPASS; m++) {
+    const int k = lane + 64 * m;
+    const int i = divq(k, iqg);
+    W.g[m] = k < QG * QG ? (i << 8) | (k - i * QG) : -1;
+  }
+  return W;
+}
 
 // iterative_lk_match_5x5::match for one wave.
 template <typename TI, int NPL, bool SEL>
 __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk_params_dev& P,
                                                 const lane_win<NPL>& W, float2 p, float2 pred_tr, lk_lds<NPL>& S) {
+  using LD = lk_lds<NPL>;
+  constexpr int PW = LD::PW;
   const match_out fail{make_float2(-1.f, -1.f), 1000.f};
   const int lane = threadIdx.x & 63;
   const int ws = P.ws, hws = ws / 2, n = ws * ws;
   const int QA = ws + 5, QG = ws + 3, QB = ws + 1 + 2 * kMB;
-  const float iQA = 1.f / (float)QA, iQG = 1.f / (float)QG, iQB = 1.f / (float)QB;
   const float2 prediction = make_float2(add_rn(p.x, pred_tr.x), add_rn(p.y, pred_tr.y));
   float2 v = prediction;
 
@@ -167,21 +210,23 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
   const int oAr = tz(p.x) - hws - 2, oAc = tz(p.y) - hws - 2;
   int oBr = tz(v.x) - hws - kMB, oBc = tz(v.y) - hws - kMB;
   wave_sync_lds();
-  load_patch<TI>(S.A, QA, iQA, L.A, L.pitchA, L.nr, L.nc, oAr, oAc, lane);
-  load_patch<TI>(S.B, QB, iQB, L.B, L.pitchB, L.nr, L.nc, oBr, oBc, lane);
+  load_patch<TI, PW>(S.A, QA, L.A, L.pitchA, L.nr, L.nc, oAr, oAc, lane);
+  load_patch<TI, PW>(S.B, QB, L.B, L.pitchB, L.nr, L.nc, oBr, oBc, lane);
   wave_sync_lds();
 
   // ---- Sobel/8 of the previous frame at global pixel (oA + 1 + (i, j));
   // zero off the image's interior (the outer ring, and clamped reads of it)
-  for (int k = lane; k < QG * QG; k += 64) {
-    const int i = divq(k, iQG), j = k - i * QG;
+#pragma unroll
+  for (int m = 0; m < LD::GPASS; m++) {
+    if (W.g[m] < 0) continue;
+    const int i = W.g[m] >> 8, j = W.g[m] & 0xFF;
     const int yr = oAr + 1 + i, yc = oAc + 1 + j;
     float2 g = make_float2(0.f, 0.f);
     if (yr >= 1 && yr <= L.nr - 2 && yc >= 1 && yc <= L.nc - 2) {
-      const float* a = &S.A[(i + 1) * QA + (j + 1)];
-      const float pmm = a[-QA - 1], pm0 = a[-QA], pmp = a[-QA + 1];
+      const float* a = &S.A[(i + 1) * PW + (j + 1)];
+      const float pmm = a[-PW - 1], pm0 = a[-PW], pmp = a[-PW + 1];
       const float p0m = a[-1], p0p = a[1];
-      const float ppm = a[QA - 1], pp0 = a[QA], ppp = a[QA + 1];
+      const float ppm = a[PW - 1], pp0 = a[PW], ppp = a[PW + 1];
       float gr = add_rn(ppm, mul_rn(2.f, pp0));
       gr = add_rn(gr, ppp);
       gr = sub_rn(gr, pmm);
@@ -194,7 +239,7 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
       gc = sub_rn(gc, ppm);
       g = make_float2(mul_rn(gr, 0.125f), mul_rn(gc, 0.125f));
     }
-    S.G[k] = g;
+    S.G[i * QG + j] = g;
   }
   wave_sync_lds();
 
@@ -214,8 +259,8 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
       if (w.xr >= 0 && w.xr < L.nr && w.xc >= 0 && w.xc < L.nc) {
         sel[k] = true;
         const int ar = w.xr - oAr, ac = w.xc - oAc;  // in [1, ws+2]
-        const float* a = &S.A[ar * QA + ac];
-        const float av = blend(w, a[0], a[QA], a[1], a[QA + 1]);
+        const float* a = &S.A[ar * PW + ac];
+        const float av = blend(w, a[0], a[PW], a[1], a[PW + 1]);
         as[k] = P.template_u8 ? (float)(uint8_t)(int)av : av;  // float -> uint8 through int
         const float2* g = &S.G[(ar - 1) * QG + (ac - 1)];
         const float2 g00 = g[0], g10 = g[QG], g01 = g[1], g11 = g[QG + 1];
@@ -243,17 +288,19 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
     }
   }
 
-  float g00 = 0.f, g01 = 0.f, g11 = 0.f, cnt = 0.f;
+  float g00 = 0.f, g01 = 0.f, g11 = 0.f;
+  int cnt = 0;
 #pragma unroll
-  for (int k = 0; k < NPL; k++)
+  for (int k = 0; k < NPL; k++) {
     if (sel[k]) {
       g00 += gs[k].x * gs[k].x;
       g01 += gs[k].x * gs[k].y;
       g11 += gs[k].y * gs[k].y;
-      cnt += 1.f;
     }
+    cnt += __popcll(__ballot(sel[k]));
+  }
   const float G00 = wave_sum(g00), G01 = wave_sum(g01), G11 = wave_sum(g11);
-  const float nsel = wave_sum(cnt);
+  const float nsel = (float)cnt;
   const float G10 = G01;
   {
     const float h = mul_rn(0.5f, add_rn(G00, G11));
@@ -270,8 +317,8 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
   // current-frame sample at v + (r, c) from the patch
   auto sampleB = [&](int k) -> float {
     const bw w = weights(add_rn(v.x, (float)W.r[k]), add_rn(v.y, (float)W.c[k]));
-    const float* b = &S.B[(w.xr - oBr) * QB + (w.xc - oBc)];
-    return blend(w, b[0], b[QB], b[1], b[QB + 1]);
+    const float* b = &S.B[(w.xr - oBr) * PW + (w.xc - oBc)];
+    return blend(w, b[0], b[PW], b[1], b[PW + 1]);
   };
   // the window at v has all its corners inside the patch
   auto ensure_covered = [&]() {
@@ -282,7 +329,7 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
     oBr = vr - hws - kMB;
     oBc = vc - hws - kMB;
     wave_sync_lds();
-    load_patch<TI>(S.B, QB, iQB, L.B, L.pitchB, L.nr, L.nc, oBr, oBc, lane);
+    load_patch<TI, PW>(S.B, QB, L.B, L.pitchB, L.nr, L.nc, oBr, oBc, lane);
     wave_sync_lds();
   };
 
@@ -359,19 +406,7 @@ __device__ inline void lk_body(const lk_args& a, int bid, int nb) {
     blk_live = 0;
     blk_acc = 0;
   }
-  lane_win<NPL> W;
-  {
-    const int ws = P.ws, hws = ws / 2;
-    const float iws = 1.f / (float)ws;
-#pragma unroll
-    for (int k = 0; k < NPL; k++) {
-      const int i = lane + 64 * k;
-      const int q = divq(i, iws);
-      W.in[k] = i < ws * ws;
-      W.r[k] = q - hws;
-      W.c[k] = i - q * ws - hws;
-    }
-  }
+  const lane_win<NPL> W = make_lane_win<NPL>(P.ws, lane);
   __syncthreads();
   if (bid == 0 && threadIdx.x == 0) cnt->n_matched = n;
   for (uint32_t i = bid * 4 + wave; i < n; i += nb * 4) {
