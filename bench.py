@@ -219,7 +219,7 @@ def batch_bench(cu, ctx):
            "stream_frames_per_s": round(BATCH_STREAMS * steps / t_b, 1),
            "particle_updates_per_s": round(live / tb[-1], 1),
            "streams_identical_to_own_tracker": f"{same}/{BATCH_STREAMS}",
-           "tuning": {"lk_div": best[0], "blur_th": batch.tuning()[1]},
+           "tuning": dict(zip(("lk_particles_per_wave", "blur_th"), batch.tuning())),
            "tuning_ms_per_step": {f"{c[0]}/{c[1]}": round(v / steps * 1000, 3) for c, v in tune.items()}}
     for t in solo:
         t.close()

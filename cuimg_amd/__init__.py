@@ -128,6 +128,7 @@ def lib():
         L.cuimg_hip_particles_merge.argtypes = [vp, vp, ctypes.c_uint32, P(Img)]
         L.cuimg_hip_tracker_work.argtypes = [vp, c_int, P(ctypes.c_uint64), P(ctypes.c_uint64)]
         L.cuimg_hip_tracker_set_graph.argtypes = [vp, c_int]
+        L.cuimg_hip_tracker_set_tuning.argtypes = [vp, c_int, c_int]
         L.cuimg_hip_lk_track.argtypes = [vp, P(Img), P(Img), c_int, c_int, vp, ctypes.c_uint32, P(LkParams),
                                          P(Img), vp, vp, vp]
         L.cuimg_hip_batch_create.argtypes = [vp, c_int, P(c_int), P(c_int), P(TrackerParams), P(vp)]
@@ -357,6 +358,10 @@ class Tracker:
         """Replay frame steps from a captured hipGraph (default) or launch them."""
         self.ctx.check(self.L.cuimg_hip_tracker_set_graph(self.h, 1 if on else 0), "set_graph")
 
+    def set_tuning(self, lk_ppw, blur_th=0):
+        """Launch shape: particles per matcher wave, blur tile rows (0 = automatic)."""
+        self.ctx.check(self.L.cuimg_hip_tracker_set_tuning(self.h, lk_ppw, blur_th), "set_tuning")
+
     def __del__(self):
         try:
             self.close()
@@ -491,8 +496,8 @@ class Batch:
     def set_graph(self, on):
         self.ctx.check(self.L.cuimg_hip_batch_set_graph(self.h, 1 if on else 0), "batch_set_graph")
 
-    def set_tuning(self, lk_div, blur_th=0):
-        self.ctx.check(self.L.cuimg_hip_batch_set_tuning(self.h, lk_div, blur_th), "batch_set_tuning")
+    def set_tuning(self, lk_ppw, blur_th=0):
+        self.ctx.check(self.L.cuimg_hip_batch_set_tuning(self.h, lk_ppw, blur_th), "batch_set_tuning")
 
     def tuning(self):
         a, b = ctypes.c_int(), ctypes.c_int()
@@ -502,8 +507,8 @@ class Batch:
     def clear(self):
         self.ctx.check(self.L.cuimg_hip_batch_clear(self.h), "batch_clear")
 
-    def autotune(self, run, candidates=((48, 0), (24, 0), (96, 0), (48, 16), (48, 32), (48, 64))):
-        """Launch-shape autotuner: for each (lk_div, blur_th) candidate the
+    def autotune(self, run, candidates=((8, 0), (4, 0), (16, 0), (32, 0), (8, 16), (8, 32), (8, 64))):
+        """Launch-shape autotuner: for each (lk_ppw, blur_th) candidate the
         batch is cleared and `run(batch)` is called; it steps the batch over a
         representative sequence and returns the seconds it measured.  The
         fastest candidate is kept; returns (best, {candidate: seconds})."""

@@ -9,8 +9,8 @@
 namespace cuimg {
 
 template <typename TI, int NPL, bool SEL>
-__global__ __launch_bounds__(256) void lk_kernel(lk_args a) {
-  lk_body<TI, NPL, SEL>(a, blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(256) void lk_kernel(lk_args a, int ppw) {
+  lk_body<TI, NPL, SEL>(a, blockIdx.x, ppw);
 }
 
 }  // namespace cuimg
@@ -66,11 +66,12 @@ extern "C" int cuimg_hip_lk_track(cuimg_hip_ctx* ctx, const cuimg_img* prev, con
   if (!rc && hipMemcpyAsync(a.cnt, &c, sizeof c, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
     rc = set_error(ctx, CUIMG_E_NOMEM, "lk_track: counter upload failed");
   if (!rc) {
-    const int blocks = (int)std::max<uint32_t>(1, std::min<uint32_t>(2048, (n + 15) / 16));
+    const int ppw = 8;
+    const int blocks = (int)((n + 4 * ppw - 1) / (4 * ppw));
     rc = launch(ctx, "lk_track", [&] {
       lk_variant(a.P.ws, a.P.top_n, dt == CUIMG_U8, [&](auto ti, auto npl, auto sel) {
         using TI = typename decltype(ti)::type;
-        lk_kernel<TI, decltype(npl)::value, decltype(sel)::value><<<blocks, 256, 0, ctx->stream>>>(a);
+        lk_kernel<TI, decltype(npl)::value, decltype(sel)::value><<<blocks, 256, 0, ctx->stream>>>(a, ppw);
       });
     });
   }

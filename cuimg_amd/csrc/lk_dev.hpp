@@ -131,6 +131,7 @@ struct lk_lds {
   static constexpr int AROWS = (QA + RPP - 1) / RPP * RPP;
   static constexpr int BROWS = (QB + RPP - 1) / RPP * RPP;
   static constexpr int GPASS = (QG * QG + 63) / 64;
+  static constexpr int FPASS = ((WSMAX + 1) * (WSMAX + 1) + 63) / 64;
   float2 G[QG * QG];
   float A[AROWS * PW];
   float B[BROWS * PW];
@@ -167,8 +168,10 @@ __device__ __forceinline__ void wave_sync_lds() {
 template <int NPL>
 struct lane_win {
   int r[NPL], c[NPL];
+  float fr[NPL], fc[NPL];
   bool in[NPL];
-  int g[lk_lds<NPL>::GPASS];
+  int g[lk_lds<NPL>::GPASS];   // whole gradient patch
+  int gf[lk_lds<NPL>::FPASS];  // its (ws+1)^2 core
 };
 
 template <int NPL>
@@ -183,12 +186,22 @@ __device__ __forceinline__ lane_win<NPL> make_lane_win(int ws, int lane) {
     W.in[k] = i < ws * ws;
     W.r[k] = q - hws;
     W.c[k] = i - q * ws - hws;
+    W.fr[k] = (float)W.r[k];
+    W.fc[k] = (float)W.c[k];
   }
 #pragma unroll
   for (int m = 0; m < lk_lds<NPL>::GPASS; m++) {
     const int k = lane + 64 * m;
     const int i = divq(k, iqg);
     W.g[m] = k < QG * QG ? (i << 8) | (k - i * QG) : -1;
+  }
+  const int QF = ws + 1;
+  const float iqf = 1.f / (float)QF;
+#pragma unroll
+  for (int m = 0; m < lk_lds<NPL>::FPASS; m++) {
+    const int k = lane + 64 * m;
+    const int i = divq(k, iqf);
+    W.gf[m] = k < QF * QF ? ((i + 1) << 8) | (k - i * QF + 1) : -1;
   }
   return W;
 }
@@ -215,11 +228,20 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
   wave_sync_lds();
 
   // ---- Sobel/8 of the previous frame at global pixel (oA + 1 + (i, j));
-  // zero off the image's interior (the outer ring, and clamped reads of it)
+  // zero off the image's interior (the outer ring, and clamped reads of it).
+  // A template sample (r, c) reads the gradient at its truncated coordinate
+  // and one row / column further; when every sample's truncated coordinate is
+  // tz(p) + (r, c) only the (ws+1)^2 core of the patch is read.
+  bool dev = false;
 #pragma unroll
-  for (int m = 0; m < LD::GPASS; m++) {
-    if (W.g[m] < 0) continue;
-    const int i = W.g[m] >> 8, j = W.g[m] & 0xFF;
+  for (int k = 0; k < NPL; k++)
+    if (W.in[k]) {
+      const int xr = tz(add_rn(p.x, W.fr[k])), xc = tz(add_rn(p.y, W.fc[k]));
+      dev |= xr != oAr + hws + 2 + W.r[k] || xc != oAc + hws + 2 + W.c[k];
+    }
+  auto sobel = [&](int code) {
+    if (code < 0) return;
+    const int i = code >> 8, j = code & 0xFF;
     const int yr = oAr + 1 + i, yc = oAc + 1 + j;
     float2 g = make_float2(0.f, 0.f);
     if (yr >= 1 && yr <= L.nr - 2 && yc >= 1 && yc <= L.nc - 2) {
@@ -240,6 +262,13 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
       g = make_float2(mul_rn(gr, 0.125f), mul_rn(gc, 0.125f));
     }
     S.G[i * QG + j] = g;
+  };
+  if (__ballot(dev)) {
+#pragma unroll
+    for (int m = 0; m < LD::GPASS; m++) sobel(W.g[m]);
+  } else {
+#pragma unroll
+    for (int m = 0; m < LD::FPASS; m++) sobel(W.gf[m]);
   }
   wave_sync_lds();
 
@@ -255,7 +284,7 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
     as[k] = 0.f;
     sel[k] = false;
     if (W.in[k]) {
-      const bw w = weights(add_rn(p.x, (float)W.r[k]), add_rn(p.y, (float)W.c[k]));
+      const bw w = weights(add_rn(p.x, W.fr[k]), add_rn(p.y, W.fc[k]));
       if (w.xr >= 0 && w.xr < L.nr && w.xc >= 0 && w.xc < L.nc) {
         sel[k] = true;
         const int ar = w.xr - oAr, ac = w.xc - oAc;  // in [1, ws+2]
@@ -316,7 +345,7 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
 
   // current-frame sample at v + (r, c) from the patch
   auto sampleB = [&](int k) -> float {
-    const bw w = weights(add_rn(v.x, (float)W.r[k]), add_rn(v.y, (float)W.c[k]));
+    const bw w = weights(add_rn(v.x, W.fr[k]), add_rn(v.y, W.fc[k]));
     const float* b = &S.B[(w.xr - oBr) * PW + (w.xc - oBc)];
     return blend(w, b[0], b[PW], b[1], b[PW + 1]);
   };
@@ -379,7 +408,10 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
 // outside its level's domain, and a failed match (-1,-1) is kept; without a
 // coarser scale the prediction is the speed (zero at age 1).
 template <typename TI, int NPL, bool SEL>
-__device__ inline void lk_body(const lk_args& a, int bid, int nb) {
+// Workgroup `bid` matches particles [bid * 4 * ppw, (bid + 1) * 4 * ppw); the
+// grid covers the capacity, so the hardware's workgroup dispatch balances the
+// particles' very different costs and workgroups past the count exit at once.
+__device__ inline void lk_body(const lk_args& a, int bid, int ppw) {
   __shared__ lk_lds<NPL> lds_all[4];
   __shared__ lk_level_view lvs[kMaxLevels];
   __shared__ uint32_t blk_live, blk_acc;
@@ -399,17 +431,20 @@ __device__ inline void lk_body(const lk_args& a, int bid, int nb) {
   int32_t* const dec_accept = uni(a.dec_accept);
   float2* const dec_tr = uni(a.dec_tr);
   float* const dec_dist = uni(a.dec_dist);
+  const uint32_t n = uni(cnt->n);
+  const uint32_t first = (uint32_t)bid * 4u * (uint32_t)ppw;
+  if (bid == 0 && threadIdx.x == 0) cnt->n_matched = n;
+  if (first >= n) return;
+  const uint32_t last = min(n, first + 4u * (uint32_t)ppw);
   if (threadIdx.x < kMaxLevels * (int)(sizeof(lk_level_view) / 4))
     reinterpret_cast<int*>(lvs)[threadIdx.x] = reinterpret_cast<const int*>(a.lv)[threadIdx.x];
-  const uint32_t n = uni(cnt->n);
   if (threadIdx.x == 0) {
     blk_live = 0;
     blk_acc = 0;
   }
   const lane_win<NPL> W = make_lane_win<NPL>(P.ws, lane);
   __syncthreads();
-  if (bid == 0 && threadIdx.x == 0) cnt->n_matched = n;
-  for (uint32_t i = bid * 4 + wave; i < n; i += nb * 4) {
+  for (uint32_t i = first + wave; i < last; i += 4) {
     cuimg_particle_f part = parts[i];
     int accept = 0;
     float2 tr = make_float2(0.f, 0.f);

@@ -121,7 +121,7 @@ struct cuimg_hip_batch {
   uint32_t* prefix_d[cuimg::K_NKINDS]{};
   uint32_t total[cuimg::K_NKINDS]{};
   cuimg::coef_pack cp{};
-  int lk_div = 48;    // particles of capacity per LK workgroup (tuning)
+  int lk_ppw = 8;     // particles per matcher wave (tuning)
   int blur_th = 0;    // blur tile height, 0 = heuristic (tuning)
   bool use_graph = true;
   hipGraphExec_t exec = nullptr;
@@ -221,11 +221,12 @@ __global__ __launch_bounds__(256) void step_fill_kernel(const level_job* jobs, c
 }
 
 template <typename TI, int NPL, bool SEL>
-__global__ __launch_bounds__(256) void step_lk_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
+__global__ __launch_bounds__(256) void step_lk_kernel(const level_job* jobs, const uint32_t* prefix, int njobs,
+                                                      int ppw) {
   CUIMG_JOB(prefix, njobs);
   const step_flags f = flags_of(J);
   if (!f.match) return;
-  lk_body<TI, NPL, SEL>(J.lk[f.slot * 2 + f.cur], lb, nb);
+  lk_body<TI, NPL, SEL>(J.lk[f.slot * 2 + f.cur], lb, ppw);
 }
 
 __global__ __launch_bounds__(256) void step_pmask_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
@@ -551,13 +552,15 @@ static int build_maps(cuimg_hip_batch* b) {
     const long mask_words = (long)R.nr * ((R.nc + 3) / 4);
     const int G = (int)std::max<long>(1, std::min<long>(1024, (mask_words + 1023) / 1024));
     cnt[K_FILL].push_back((uint32_t)(J.fill_sparse_blocks + G));
-    cnt[K_LK].push_back((uint32_t)std::max<long>(4, std::min<long>(2048, (cap + b->lk_div - 1) / b->lk_div)));
     cnt[K_PMASK].push_back((uint32_t)std::max<long>(1, std::min<long>(512, (cap + 255) / 256)));
     cnt[K_FAST].push_back((uint32_t)(J.fast_tiles_x * ceil_div(R.nr, kFTH)));
     cnt[K_DETC].push_back((uint32_t)(R.nbr + R.nblocks));
     cnt[K_SCATI].push_back((uint32_t)(R.nblocks + ceil_div(R.nbr * b->P[J.stream].box_size, 4)));
     cnt[K_MSCAN].push_back((uint32_t)std::max<long>(1, std::min<long>(512, (cap + 255) / 256)));
   }
+  // Matcher: 4 * ppw particles of capacity per workgroup (see lk_body).
+  for (int j = 0; j < b->njobs; j++)
+    cnt[K_LK].push_back((uint32_t)ceil_div((int)b->lv[j].cap, 4 * b->lk_ppw));
   for (int k = 0; k < K_NKINDS; k++) {
     std::vector<uint32_t>& p = b->prefix_h[k];
     p.assign(cnt[k].size() + 1, 0);
@@ -676,7 +679,8 @@ static int enqueue_step(cuimg_hip_batch* b) {
   CUIMG_TRY(launch(ctx, "lk_track", [&] {
     lk_variant(b->P0.lk_ws, b->P0.lk_top_n, b->P0.profile == 0, [&](auto ti, auto npl, auto sel) {
       using TI = typename decltype(ti)::type;
-      step_lk_kernel<TI, decltype(npl)::value, decltype(sel)::value><<<b->total[K_LK], 256, 0, st>>>(J, pf[K_LK], nj);
+      step_lk_kernel<TI, decltype(npl)::value, decltype(sel)::value><<<b->total[K_LK], 256, 0, st>>>(J, pf[K_LK], nj,
+                                                                                                b->lk_ppw);
     });
   }));
   CUIMG_TRY(launch(ctx, "particle_mask",
@@ -790,16 +794,16 @@ int cuimg_hip_batch_tracker(cuimg_hip_batch* b, int stream, cuimg_hip_tracker** 
   return 0;
 }
 
-int cuimg_hip_batch_set_tuning(cuimg_hip_batch* b, int lk_div, int blur_th) {
-  if (!b || lk_div < 1 || blur_th < 0 || blur_th > 1024) return CUIMG_E_INVALID;
-  b->lk_div = lk_div;
+int cuimg_hip_batch_set_tuning(cuimg_hip_batch* b, int lk_ppw, int blur_th) {
+  if (!b || lk_ppw < 1 || lk_ppw > 256 || blur_th < 0 || blur_th > 1024) return CUIMG_E_INVALID;
+  b->lk_ppw = lk_ppw;
   b->blur_th = blur_th;
   return rebuild(b);
 }
 
-int cuimg_hip_batch_get_tuning(cuimg_hip_batch* b, int* lk_div, int* blur_th) {
+int cuimg_hip_batch_get_tuning(cuimg_hip_batch* b, int* lk_ppw, int* blur_th) {
   if (!b) return CUIMG_E_INVALID;
-  if (lk_div) *lk_div = b->lk_div;
+  if (lk_ppw) *lk_ppw = b->lk_ppw;
   if (blur_th) *blur_th = ::cuimg::blur_th(b);
   return 0;
 }
@@ -850,6 +854,13 @@ int cuimg_hip_tracker_run_host(cuimg_hip_tracker* t, const uint8_t* frame, size_
   CUIMG_HIP(b->ctx, hipMemcpy2DAsync(fb.begin, fb.pitch, frame, pitch, fb.ncols, fb.nrows, hipMemcpyHostToDevice,
                                      b->ctx->stream));
   return cuimg_hip_tracker_run(t, &fb);
+}
+
+int cuimg_hip_tracker_set_tuning(cuimg_hip_tracker* t, int lk_ppw, int blur_th) {
+  if (!t || !t->b) return CUIMG_E_INVALID;
+  if (t->b->nstreams != 1)
+    return set_error(t->b->ctx, CUIMG_E_INVALID, "tracker_set_tuning: a stream of a batch is tuned with its batch");
+  return cuimg_hip_batch_set_tuning(t->b, lk_ppw, blur_th);
 }
 
 int cuimg_hip_tracker_set_graph(cuimg_hip_tracker* t, int enable) {

@@ -1,5 +1,5 @@
 // Device-side state of the pyramid LK tracker, shared by particles.hip,
-// lk.hip, fast.hip, pyramid.hip, gaussian.hip and step.hip.
+// lk.hip and tracker.hip (and the device headers *_dev.hpp).
 #pragma once
 
 #include "common.hpp"
@@ -103,21 +103,6 @@ constexpr uint32_t kMergeMaxCap = (1u << 24) - 1;
 
 constexpr int kCompactBlock = 1024;  // particles per compaction block
 constexpr int kMaxBox = 15;          // largest detector box
-
-// ----------------------------------------------------------- device bodies
-// Each takes its block index and block count explicitly, so one launch can
-// run many (stream, scale) jobs (step.hip) or a single one.
-__device__ void fill_u32_body(uint32_t* begin, int pitch, int r0, int r1, int c0, int c1, uint32_t v, int bid,
-                              int nb);
-__device__ void fill_u8_body(uint8_t* begin, int pitch, int nr, int nc, uint32_t v4, int bid, int nb);
-__device__ void particle_mask_body(const detect_args& a, int bid, int nb);
-__device__ void nms_row_body(const detect_args& a, float fast_th, int bi);
-__device__ void compact_count_body(const detect_args& a, int bid);
-__device__ void compact_scan_body(const detect_args& a);
-__device__ void compact_scatter_body(const detect_args& a, int bid);
-__device__ void insert_new_body(const detect_args& a, int bid);
-__device__ void merge_scan_body(const merge_args& a, int bid, int nb);
-__device__ void merge_resolve_body(const merge_args& a);
 
 // Host helpers of particles.hip.
 int launch_fill_u32(cuimg_hip_ctx* ctx, uint32_t* begin, int pitch, int r0, int r1, int c0, int c1,

@@ -227,6 +227,8 @@ int cuimg_hip_particles_merge(cuimg_hip_ctx* ctx, cuimg_particle_f* parts, uint3
 /* Frame steps are captured once into a hipGraph and replayed (default on);
  * disabled, every step launches its kernels (per-kernel timing needs this). */
 int cuimg_hip_tracker_set_graph(cuimg_hip_tracker* t, int enable);
+/* Launch shape of a tracker's frame step (see cuimg_hip_batch_set_tuning). */
+int cuimg_hip_tracker_set_tuning(cuimg_hip_tracker* t, int lk_ppw, int blur_th);
 
 /* --------------------------------------------------------------- matcher */
 /* iterative_lk_match_5x5 settings (tracking2/pyrlk_tracker.hpp:114-128). */
@@ -269,10 +271,12 @@ int cuimg_hip_batch_set_graph(cuimg_hip_batch* b, int enable);
 /* Tracker view of stream i for the tracker accessors (owned by the batch;
  * tracker_run on it is refused, tracker_destroy does nothing). */
 int cuimg_hip_batch_tracker(cuimg_hip_batch* b, int stream, cuimg_hip_tracker** out);
-/* Launch shape: lk_div = particle capacity per matcher workgroup (default 48),
- * blur_th = blur tile height in rows (0 = chosen from the total work). */
-int cuimg_hip_batch_set_tuning(cuimg_hip_batch* b, int lk_div, int blur_th);
-int cuimg_hip_batch_get_tuning(cuimg_hip_batch* b, int* lk_div, int* blur_th);
+/* Launch shape: lk_ppw = particles per matcher wave (a workgroup of 4 waves
+ * takes 4 * lk_ppw consecutive particles; default 8), blur_th = blur tile
+ * height in rows (0 = chosen from the total work).  Reading reports the
+ * values in use. */
+int cuimg_hip_batch_set_tuning(cuimg_hip_batch* b, int lk_ppw, int blur_th);
+int cuimg_hip_batch_get_tuning(cuimg_hip_batch* b, int* lk_ppw, int* blur_th);
 int cuimg_hip_batch_clear(cuimg_hip_batch* b);
 
 /* ------------------------------------------------------------- adjoint */
