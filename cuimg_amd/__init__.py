@@ -94,6 +94,7 @@ def lib():
         L.cuimg_hip_free.argtypes = [vp, vp]
         L.cuimg_hip_memcpy_h2d.argtypes = [vp, vp, vp, ctypes.c_size_t]
         L.cuimg_hip_memcpy_d2h.argtypes = [vp, vp, vp, ctypes.c_size_t]
+        L.cuimg_hip_memcpy_d2d.argtypes = [vp, vp, vp, ctypes.c_size_t]
         L.cuimg_hip_memset.argtypes = [vp, vp, c_int, ctypes.c_size_t]
         L.cuimg_hip_image_alloc.argtypes = [vp, c_int, c_int, c_int, c_int, P(Img)]
         L.cuimg_hip_image_free.argtypes = [vp, P(Img)]
@@ -597,6 +598,11 @@ class DeviceBuffer:
         if arr.nbytes:
             ctx.check(ctx.L.cuimg_hip_memcpy_h2d(ctx.h, b.p, arr.ctypes.data, arr.nbytes), "h2d")
         return b
+
+    def copy_from(self, src, nbytes=None):
+        """Device-to-device copy of nbytes (default: all) from another buffer."""
+        n = self.nbytes if nbytes is None else nbytes
+        self.ctx.check(self.ctx.L.cuimg_hip_memcpy_d2d(self.ctx.h, self.p, src.p, n), "memcpy_d2d")
 
     def download(self, dtype, shape):
         out = np.zeros(shape, dtype)

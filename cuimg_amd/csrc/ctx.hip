@@ -257,6 +257,11 @@ int cuimg_hip_memcpy_d2h(cuimg_hip_ctx* ctx, void* dst, const void* src, size_t 
   return 0;
 }
 
+int cuimg_hip_memcpy_d2d(cuimg_hip_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return CUIMG_E_INVALID;
+  return cuimg::launch(ctx, "memcpy_d2d", [&] { (void)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream); });
+}
+
 int cuimg_hip_memset(cuimg_hip_ctx* ctx, void* dst, int value, size_t bytes) {
   CUIMG_HIP(ctx, hipMemsetAsync(dst, value, bytes, ctx->stream));
   return 0;

@@ -4,8 +4,15 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace cuimg {
+
+// Rows in flight per wave on the direct-to-LDS path.
+template <int K>
+constexpr int dma_depth() {
+  return K <= 6 ? 8 : 6;
+}
 
 template <typename TI, typename E, int VEC, int K>
 __global__ __launch_bounds__(64) void gauss_sep_kernel(gauss_jobs jobs, coef_pack cp) {
@@ -14,7 +21,14 @@ __global__ __launch_bounds__(64) void gauss_sep_kernel(gauss_jobs jobs, coef_pac
   while (k + 1 < jobs.n && b >= jobs.j[k + 1].tile_begin) k++;
   const gauss_job& J = jobs.j[k];
   b -= J.tile_begin;
-  gauss_tile<TI, E, VEC, K>(J, b % J.tiles_x, b / J.tiles_x, cp);
+  const int tx = b % J.tiles_x, ty = b / J.tiles_x;
+  if constexpr (std::is_same<TI, E>::value && sizeof(E) * VEC == 16) {
+    if (J.vec_ok && (tx + 1) * 64 * VEC <= J.nc) {
+      gauss_tile_dma<E, VEC, K, dma_depth<K>()>(J, tx, ty, cp);
+      return;
+    }
+  }
+  gauss_tile<TI, E, VEC, K>(J, tx, ty, cp);
 }
 
 // ------------------------------------------------------------------ host

@@ -225,4 +225,184 @@ __device__ __forceinline__ void gauss_tile(const gauss_job& J, int tx, int ty, c
   }
 }
 
+
+// ------------------------------------------------------ direct-to-LDS path
+// For full-width strips of F32 / F32X4 images the input rows stream straight
+// from HBM into an LDS ring with global_load_lds (16 bytes per lane, no
+// registers held while in flight), D rows ahead of the row being filtered, so
+// every wave keeps D rows of memory traffic in flight instead of one.  Each
+// ring row is [left halo K][strip W][right halo K] with the strip 16-byte
+// aligned; a row costs three DMA instructions (strip, left halo, right halo).
+//
+// Waits are placed by hand.  Loads complete in issue order, so while row r's
+// DMAs are pending all later rows' DMAs are too: at least 3 D operations are
+// outstanding.  Waiting until at most 3 (D - 1) + 2 are outstanding therefore
+// guarantees row r has landed, whatever the order in which the (at least two)
+// stores issued since then complete; only older stores can hold the wait up.
+// Each step stores the previous step's output right after its wait.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(3))) void* lds_addr(T* p) {
+  return (__attribute__((address_space(3))) void*)p;
+}
+
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) void* glb_addr(const T* p) {
+  return (__attribute__((address_space(1))) void*)(p);
+}
+
+// LDS reads of DMA-filled rows.  The compiler cannot tell which DMA wrote
+// which ring slot and would wait for all of them before any LDS read, so these
+// reads are written as ds_read_b128 in asm, each block waiting for its own
+// results (lgkmcnt(0)) before it ends; the DMA waits are placed by hand.
+__device__ __forceinline__ uint32_t lds_u32(const void* p) {
+  return (uint32_t)(uintptr_t)(__attribute__((address_space(3))) const void*)p;
+}
+
+template <int N>
+__device__ __forceinline__ void lds_read16(const uint32_t* a, float4* r);
+
+template <>
+__device__ __forceinline__ void lds_read16<1>(const uint32_t* a, float4* r) {
+  asm volatile("ds_read_b128 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=&v"(r[0]) : "v"(a[0]) : "memory");
+}
+template <>
+__device__ __forceinline__ void lds_read16<2>(const uint32_t* a, float4* r) {
+  asm volatile("ds_read_b128 %0, %2\n\tds_read_b128 %1, %3\n\ts_waitcnt lgkmcnt(0)"
+               : "=&v"(r[0]), "=&v"(r[1])
+               : "v"(a[0]), "v"(a[1])
+               : "memory");
+}
+template <>
+__device__ __forceinline__ void lds_read16<3>(const uint32_t* a, float4* r) {
+  asm volatile("ds_read_b128 %0, %3\n\tds_read_b128 %1, %4\n\tds_read_b128 %2, %5\n\ts_waitcnt lgkmcnt(0)"
+               : "=&v"(r[0]), "=&v"(r[1]), "=&v"(r[2])
+               : "v"(a[0]), "v"(a[1]), "v"(a[2])
+               : "memory");
+}
+template <>
+__device__ __forceinline__ void lds_read16<4>(const uint32_t* a, float4* r) {
+  asm volatile(
+      "ds_read_b128 %0, %4\n\tds_read_b128 %1, %5\n\tds_read_b128 %2, %6\n\tds_read_b128 %3, %7\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(r[0]), "=&v"(r[1]), "=&v"(r[2]), "=&v"(r[3])
+      : "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3])
+      : "memory");
+}
+
+// 16-byte store out of the compiler's view, so that no wait for it is placed
+// (nothing in the kernel reads what it writes); the s_nop covers the gfx9
+// hazard of overwriting a wide store's data registers.
+__device__ __forceinline__ void global_store16(void* p, float4 v) {
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  const v4f w = {v.x, v.y, v.z, v.w};
+  asm volatile("global_store_dwordx4 %0, %1, off\n\ts_nop 1" : : "v"(p), "v"(w) : "memory");
+}
+
+// r[k] = the 16 bytes at LDS byte address base + 16 k, k < N, in blocks of 4.
+template <int N>
+__device__ __forceinline__ void lds_read16_run(uint32_t base, float4 (&r)[N]) {
+  static_for<(N + 3) / 4>([&](auto gc) {
+    constexpr int g = decltype(gc)::value;
+    constexpr int cnt = N - 4 * g < 4 ? N - 4 * g : 4;
+    uint32_t a[4];
+#pragma unroll
+    for (int k = 0; k < cnt; k++) a[k] = base + 16u * (uint32_t)(4 * g + k);
+    lds_read16<cnt>(a, &r[4 * g]);
+  });
+}
+
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  // s_waitcnt vmcnt(N), expcnt and lgkmcnt left at their maxima (gfx9 encoding)
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+template <typename E, int VEC, int K, int D>
+__device__ __forceinline__ void gauss_tile_dma(const gauss_job& J, int tx, int ty, const coef_pack& cp) {
+  constexpr int W = 64 * VEC;
+  constexpr int P = 2 * K + 1;
+  constexpr int R = D + 1;
+  constexpr int EPB = 16 / (int)sizeof(E);             // elements per 16 bytes
+  constexpr int HL = (K + EPB - 1) / EPB * EPB;        // left pad: strip 16-byte aligned
+  constexpr int RW = (HL + W + K + EPB - 1) / EPB * EPB;
+  static_assert(sizeof(E) * VEC == 16, "16 bytes per lane");
+  static_assert(D >= 3, "the store allowance needs two stores between a row's DMA and its wait");
+  constexpr unsigned ESZ = (unsigned)sizeof(E);  // DMA size operands must be 32-bit constants
+  __shared__ __align__(16) E ring[R][RW];
+
+  const int lane = threadIdx.x;
+  const int c0 = tx * W;
+  const int r0 = ty * J.th;
+  const int r_end = min(r0 + J.th, J.nr);
+  const int first = r0 - K, last = r_end - 1 + K;
+  const int col = c0 + VEC * lane;
+  const int hcl = max(c0 - K + lane, 0), hcr = min(c0 + W + lane, J.nc - 1);
+
+  float c[P];
+#pragma unroll
+  for (int m = 0; m < P; m++) c[m] = cp.c[m];
+
+  auto issue = [&](int ri) {
+    E* row_lds = ring[__builtin_amdgcn_readfirstlane((ri - first) % R)];
+    const E* row = px<E>(J.in, J.in_pitch, clampi(ri, 0, J.nr - 1), 0);
+    __builtin_amdgcn_global_load_lds(glb_addr(row + col), lds_addr(row_lds + HL), 16u, 0, 0);
+    if (lane < K) {
+      __builtin_amdgcn_global_load_lds(glb_addr(row + hcl), lds_addr(row_lds + HL - K), ESZ, 0, 0);
+      __builtin_amdgcn_global_load_lds(glb_addr(row + hcr), lds_addr(row_lds + HL + W), ESZ, 0, 0);
+    }
+  };
+#pragma unroll
+  for (int d = 0; d < D; d++)
+    if (first + d <= last) issue(first + d);
+
+  E hist[P][VEC];
+  E pend[VEC];        // output row of the previous step, stored after this step's wait
+  int pend_row = -1;
+  for (int base = first; base <= last; base += P) {
+    static_for<P>([&](auto tc) {
+      constexpr int t = decltype(tc)::value;
+      const int ri = base + t;
+      if (ri > last) return;
+      // a store is issued at step s iff s >= first + 2K + 1: steps ri-1 and
+      // ri-2 both stored (after DMA(ri), issued at step ri - D <= ri - 3)
+      if (ri + D - 1 > last) wait_vm<0>();
+      else if (ri >= first + 2 * K + 3) wait_vm<3 * (D - 1) + 2>();
+      else wait_vm<3 * (D - 1)>();
+      if (pend_row >= 0) global_store16(px<E>(J.out, J.out_pitch, pend_row, 0) + col, *reinterpret_cast<float4*>(pend));
+      pend_row = -1;
+      const E* row_lds = ring[(ri - first) % R];
+      // 16-byte chunks covering elements [HL - K + VEC*lane, HL + K + VEC*(lane+1))
+      constexpr int CL = (HL - K) / EPB;
+      constexpr int NCH = (HL + K + VEC + EPB - 1) / EPB - CL;
+      float4 ch[NCH];
+      lds_read16_run<NCH>(lds_u32(row_lds) + 16u * (uint32_t)(CL + lane * VEC / EPB), ch);
+      E x[VEC + 2 * K];
+      const E* che = reinterpret_cast<const E*>(ch);
+#pragma unroll
+      for (int m = 0; m < VEC + 2 * K; m++) x[m] = che[HL - K - CL * EPB + m];
+      // the slot of row ri + D held row ri - 1, read by the previous step
+      if (ri + D <= last) issue(ri + D);
+#pragma unroll
+      for (int i = 0; i < VEC; i++) {
+        E s = mul_e(c[2 * K], x[i + 2 * K]);
+#pragma unroll
+        for (int m = 2 * K - 1; m >= 0; m--) s = fma_e(c[m], x[i + m], s);
+        hist[t][i] = s;
+      }
+      if (ri - K >= r0) {
+#pragma unroll
+        for (int i = 0; i < VEC; i++) {
+          E s = mul_e(c[2 * K], hist[t][i]);
+#pragma unroll
+          for (int m = 2 * K - 1; m >= 0; m--) s = fma_e(c[m], hist[(t + m + 1) % P][i], s);
+          pend[i] = s;
+        }
+        pend_row = ri - K;
+      }
+    });
+  }
+  if (pend_row >= 0) global_store16(px<E>(J.out, J.out_pitch, pend_row, 0) + col, *reinterpret_cast<float4*>(pend));
+}
+
 }  // namespace cuimg

@@ -82,6 +82,9 @@ int cuimg_hip_malloc(cuimg_hip_ctx* ctx, size_t bytes, void** out);
 int cuimg_hip_free(cuimg_hip_ctx* ctx, void* p);
 int cuimg_hip_memcpy_h2d(cuimg_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
 int cuimg_hip_memcpy_d2h(cuimg_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Device-to-device copy, asynchronous on the context stream (the copy-kernel
+ * reference for bandwidth and traffic measurements). */
+int cuimg_hip_memcpy_d2d(cuimg_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
 int cuimg_hip_memset(cuimg_hip_ctx* ctx, void* dst, int value, size_t bytes);
 
 /* Replaces device_image2d<V>(domain, border) (cuimg/gpu/device_image2d.hpp:23-38):

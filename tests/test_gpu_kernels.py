@@ -37,6 +37,13 @@ GAUSS_CASES = [
     ("f32", (777, 1031), 4),
     ("f32x4", (270, 480), 1),
     ("f32x4", (135, 250), 4),
+    # full-width strips (the direct-to-LDS row pipeline), short images
+    ("f32", (2048, 512), 4),
+    ("f32", (5, 512), 1),
+    ("f32", (3, 256), 3),
+    ("f32", (40, 1024), 5),
+    ("f32x4", (200, 640), 2),
+    ("f32x4", (64, 128), 4),
 ]
 
 
@@ -76,13 +83,14 @@ def test_gaussian_multi_levels(ctx, orc):
         assert rel_err(d.download(), orc.gaussian(i, coef, 3)) < 1e-5
 
 
-@pytest.mark.parametrize("th", [16, 64, 128])
-def test_gaussian_tile_heights_agree(ctx, orc, th):
+@pytest.mark.parametrize("th", [1, 2, 16, 64, 128])
+@pytest.mark.parametrize("width", [700, 768])
+def test_gaussian_tile_heights_agree(ctx, orc, th, width):
     rng = np.random.default_rng(th)
-    img = rng.uniform(0, 255, (300, 700)).astype(np.float32)
+    img = rng.uniform(0, 255, (300, width)).astype(np.float32)
     coef = orc.coefs(2, 6)
     src = ctx.upload(img)
-    dst = ctx.image(300, 700, cu.F32)
+    dst = ctx.image(300, width, cu.F32)
     ctx.convolve(src, dst, coef, 6, th=th)
     assert rel_err(dst.download(), orc.gaussian(img, coef, 6)) < 1e-5
 
