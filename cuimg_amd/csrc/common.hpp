@@ -98,6 +98,15 @@ __device__ __forceinline__ const T* px(const void* begin, int pitch, int r, int 
 }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on XCD
+// b mod 8).  xcd_order(b, nb) renumbers a grid of nb workgroups so that each
+// XCD takes a contiguous run of tile indices: neighbouring tiles, which share
+// halo lines, then share an L2.
+__device__ __forceinline__ int xcd_order(int b, int nb) {
+  const int x = b & 7, q = nb >> 3, r = nb & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 // A wave-uniform value moved to scalar registers (all lanes hold the same bits).
 template <typename T>
 __device__ __forceinline__ T uni(const T& v) {

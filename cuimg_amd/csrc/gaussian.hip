@@ -1,5 +1,6 @@
 // Separable Gaussian: host side and standalone launches (the device code and
-// its reference notes are in gaussian_dev.hpp).
+// its reference notes are in gaussian_dev.hpp, the kernel in
+// gaussian_kernel_dev.hpp).
 #include "gaussian_dev.hpp"
 
 #include <algorithm>
@@ -8,34 +9,8 @@
 
 namespace cuimg {
 
-// Rows in flight per wave on the direct-to-LDS path.
-template <int K>
-constexpr int dma_depth() {
-  return K <= 6 ? 8 : 6;
-}
-
 template <typename TI, typename E, int VEC, int K>
-__global__ __launch_bounds__(64) void gauss_sep_kernel(gauss_jobs jobs, coef_pack cp) {
-  int b = blockIdx.x;
-  int k = 0;
-  while (k + 1 < jobs.n && b >= jobs.j[k + 1].tile_begin) k++;
-  const gauss_job& J = jobs.j[k];
-  b -= J.tile_begin;
-  const int tx = b % J.tiles_x, ty = b / J.tiles_x;
-  if constexpr (std::is_same<TI, E>::value && sizeof(E) * VEC == 16) {
-    if (J.vec_ok && (tx + 1) * 64 * VEC <= J.nc) {
-      gauss_tile_dma<E, VEC, K, dma_depth<K>()>(J, tx, ty, cp);
-      return;
-    }
-  }
-  gauss_tile<TI, E, VEC, K>(J, tx, ty, cp);
-}
-
-// ------------------------------------------------------------------ host
-template <typename TI, typename E, int VEC, int K>
-static void launch_k(cuimg_hip_ctx* ctx, const gauss_jobs& jobs, int blocks, const coef_pack& cp) {
-  gauss_sep_kernel<TI, E, VEC, K><<<blocks, 64, 0, ctx->stream>>>(jobs, cp);
-}
+void launch_k(cuimg_hip_ctx* ctx, const gauss_jobs& jobs, int blocks, const coef_pack& cp);
 
 template <typename TI, typename E, int VEC>
 static int dispatch_k(cuimg_hip_ctx* ctx, const gauss_jobs& jobs, int blocks, const coef_pack& cp, int K) {
@@ -47,6 +22,10 @@ static int dispatch_k(cuimg_hip_ctx* ctx, const gauss_jobs& jobs, int blocks, co
   }
   return set_error(ctx, CUIMG_E_UNSUPPORTED, "gaussian: half width %d not supported (1..%d)", K, kMaxK);
 }
+
+// Output size above which stores are non-temporal: the 8 XCDs' L2s (4 MiB
+// each) could not hand the image to the next kernel anyway.
+constexpr size_t kStreamOutBytes = 32u << 20;
 
 static int pick_th(int nr, int tiles_x, int total_tiles_other) {
   // aim for >= ~2048 waves in flight over the whole launch, 16..128 rows per tile
@@ -103,6 +82,10 @@ int convolve_sep_multi(cuimg_hip_ctx* ctx, const cuimg_img* ins, const cuimg_img
     J.tile_begin = total;
     total += J.tiles_x * ceil_div(J.nr, th);
   }
+  // outputs larger than the XCDs' L2s together are streamed past the cache
+  size_t out_bytes = 0;
+  for (int i = 0; i < n; i++) out_bytes += (size_t)jobs.j[i].nr * jobs.j[i].nc * es_out;
+  for (int i = 0; i < n; i++) jobs.j[i].stream_out = out_bytes > kStreamOutBytes;
   coef_pack cp{};
   for (int m = 0; m < 2 * K + 1; m++) cp.c[m] = coefs[m];
   return launch(ctx, "gaussian_sep", [&] {

@@ -41,6 +41,7 @@ struct gauss_job {
   int tiles_x;
   int tile_begin;  // first flattened tile index of this job
   int vec_ok;      // in/out aligned for vector access
+  int stream_out;  // write the output with non-temporal stores
 };
 
 struct gauss_jobs {
@@ -115,21 +116,36 @@ struct io<float4, float4> {
   __device__ static float4 load1(const float4* p) { return *p; }
 };
 
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+typedef float v2f_t __attribute__((ext_vector_type(2)));
+
+// Vector store; nt: non-temporal (not kept in L2).
 template <int VEC>
-__device__ __forceinline__ void store_vec(float* p, const float (&v)[VEC]) {
+__device__ __forceinline__ void store_vec(float* p, const float (&v)[VEC], bool nt) {
   if constexpr (VEC == 4) {
-    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    const v4f_t w = {v[0], v[1], v[2], v[3]};
+    if (nt) __builtin_nontemporal_store(w, reinterpret_cast<v4f_t*>(p));
+    else *reinterpret_cast<v4f_t*>(p) = w;
   } else if constexpr (VEC == 2) {
-    *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
+    const v2f_t w = {v[0], v[1]};
+    if (nt) __builtin_nontemporal_store(w, reinterpret_cast<v2f_t*>(p));
+    else *reinterpret_cast<v2f_t*>(p) = w;
   } else {
 #pragma unroll
-    for (int i = 0; i < VEC; i++) p[i] = v[i];
+    for (int i = 0; i < VEC; i++) {
+      if (nt) __builtin_nontemporal_store(v[i], p + i);
+      else p[i] = v[i];
+    }
   }
 }
 template <int VEC>
-__device__ __forceinline__ void store_vec(float4* p, const float4 (&v)[VEC]) {
+__device__ __forceinline__ void store_vec(float4* p, const float4 (&v)[VEC], bool nt) {
 #pragma unroll
-  for (int i = 0; i < VEC; i++) p[i] = v[i];
+  for (int i = 0; i < VEC; i++) {
+    const v4f_t w = {v[i].x, v[i].y, v[i].z, v[i].w};
+    if (nt) __builtin_nontemporal_store(w, reinterpret_cast<v4f_t*>(p + i));
+    else *reinterpret_cast<v4f_t*>(p + i) = w;
+  }
 }
 
 // Loads the VEC elements of this lane plus the halo elements held by lanes < K.
@@ -152,12 +168,20 @@ __device__ __forceinline__ void load_row(const gauss_job& J, int ri, int c0, int
   }
 }
 
+// LDS of gauss_tile: two rows of W + 2K elements.
+template <typename E, int VEC, int K>
+constexpr int gauss_tile_lds_bytes() {
+  return 2 * (64 * VEC + 2 * K) * (int)sizeof(E);
+}
+
+// One tile, staging rows in lds_mem (gauss_tile_lds_bytes, 16-byte aligned).
 template <typename TI, typename E, int VEC, int K>
-__device__ __forceinline__ void gauss_tile(const gauss_job& J, int tx, int ty, const coef_pack& cp) {
+__device__ __forceinline__ void gauss_tile(const gauss_job& J, int tx, int ty, const coef_pack& cp,
+                                           void* lds_mem) {
   constexpr int W = 64 * VEC;
   constexpr int P = 2 * K + 1;
   constexpr int LW = W + 2 * K;
-  __shared__ E lds[2][LW];
+  E (*lds)[LW] = reinterpret_cast<E (*)[LW]>(lds_mem);
 
   const int lane = threadIdx.x;
   const int c0 = tx * W;
@@ -214,7 +238,7 @@ __device__ __forceinline__ void gauss_tile(const gauss_job& J, int tx, int ty, c
         }
         E* orow = px<E>(J.out, J.out_pitch, ri - K, 0);
         if (J.vec_ok && col + VEC <= J.nc) {
-          store_vec<VEC>(orow + col, o);
+          store_vec<VEC>(orow + col, o, J.stream_out);
         } else {
 #pragma unroll
           for (int i = 0; i < VEC; i++)
@@ -226,6 +250,12 @@ __device__ __forceinline__ void gauss_tile(const gauss_job& J, int tx, int ty, c
 }
 
 
+template <typename TI, typename E, int VEC, int K>
+__device__ __forceinline__ void gauss_tile(const gauss_job& J, int tx, int ty, const coef_pack& cp) {
+  __shared__ __align__(16) char lds_mem[gauss_tile_lds_bytes<E, VEC, K>()];
+  gauss_tile<TI, E, VEC, K>(J, tx, ty, cp, lds_mem);
+}
+
 // ------------------------------------------------------ direct-to-LDS path
 // For full-width strips of F32 / F32X4 images the input rows stream straight
 // from HBM into an LDS ring with global_load_lds (16 bytes per lane, no
@@ -234,12 +264,18 @@ __device__ __forceinline__ void gauss_tile(const gauss_job& J, int tx, int ty, c
 // ring row is [left halo K][strip W][right halo K] with the strip 16-byte
 // aligned; a row costs three DMA instructions (strip, left halo, right halo).
 //
-// Waits are placed by hand.  Loads complete in issue order, so while row r's
-// DMAs are pending all later rows' DMAs are too: at least 3 D operations are
-// outstanding.  Waiting until at most 3 (D - 1) + 2 are outstanding therefore
-// guarantees row r has landed, whatever the order in which the (at least two)
-// stores issued since then complete; only older stores can hold the wait up.
-// Each step stores the previous step's output right after its wait.
+// Waits are placed by hand.  Loads complete in issue order, so while any DMA
+// of row r is pending, the 3 (D - 1) DMAs of the rows issued after it are too:
+// at least 3 (D - 1) + 1 operations are outstanding.  Waiting until at most
+// 3 (D - 1) are outstanding therefore guarantees row r has landed, however the
+// stores issued in between complete.  Each step stores the previous step's
+// output right after its wait.
+//
+// Odd tile rows stream bottom-up: two vertically adjacent tiles then read the
+// 2K rows they share at about the same time (both first or both last), so the
+// second read is an L2 hit when the tiles run on one XCD (the launch orders
+// tiles per XCD, gaussian.hip).  Outputs too large to stay in cache are
+// written with non-temporal stores (gauss_job::stream_out).
 template <typename T>
 __device__ __forceinline__ __attribute__((address_space(3))) void* lds_addr(T* p) {
   return (__attribute__((address_space(3))) void*)p;
@@ -291,11 +327,15 @@ __device__ __forceinline__ void lds_read16<4>(const uint32_t* a, float4* r) {
 
 // 16-byte store out of the compiler's view, so that no wait for it is placed
 // (nothing in the kernel reads what it writes); the s_nop covers the gfx9
-// hazard of overwriting a wide store's data registers.
-__device__ __forceinline__ void global_store16(void* p, float4 v) {
+// hazard of overwriting a wide store's data registers.  nt: non-temporal (the
+// line is not kept in L2), for outputs that no later kernel reads from cache.
+__device__ __forceinline__ void global_store16(void* p, float4 v, bool nt) {
   typedef float v4f __attribute__((ext_vector_type(4)));
   const v4f w = {v.x, v.y, v.z, v.w};
-  asm volatile("global_store_dwordx4 %0, %1, off\n\ts_nop 1" : : "v"(p), "v"(w) : "memory");
+  if (nt)
+    asm volatile("global_store_dwordx4 %0, %1, off nt\n\ts_nop 1" : : "v"(p), "v"(w) : "memory");
+  else
+    asm volatile("global_store_dwordx4 %0, %1, off\n\ts_nop 1" : : "v"(p), "v"(w) : "memory");
 }
 
 // r[k] = the 16 bytes at LDS byte address base + 16 k, k < N, in blocks of 4.
@@ -318,34 +358,52 @@ __device__ __forceinline__ void wait_vm() {
   __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
 }
 
+// Ring geometry: R = D + 1 rows of RW elements, [pad HL][strip W][halo K].
 template <typename E, int VEC, int K, int D>
-__device__ __forceinline__ void gauss_tile_dma(const gauss_job& J, int tx, int ty, const coef_pack& cp) {
-  constexpr int W = 64 * VEC;
+struct dma_ring {
+  static constexpr int W = 64 * VEC;
+  static constexpr int R = D + 1;
+  static constexpr int EPB = 16 / (int)sizeof(E);       // elements per 16 bytes
+  static constexpr int HL = (K + EPB - 1) / EPB * EPB;  // left pad: strip 16-byte aligned
+  static constexpr int RW = (HL + W + K + EPB - 1) / EPB * EPB;
+  static constexpr int bytes = R * RW * (int)sizeof(E);
+};
+
+// One tile (UP: bottom-up), its ring in ring_mem (dma_ring::bytes, 16-byte
+// aligned; both directions of a kernel share it).
+template <typename E, int VEC, int K, int D, bool UP>
+__device__ __forceinline__ void gauss_tile_dma(const gauss_job& J, int tx, int ty, const coef_pack& cp,
+                                               void* ring_mem) {
+  using G = dma_ring<E, VEC, K, D>;
+  constexpr int W = G::W;
   constexpr int P = 2 * K + 1;
-  constexpr int R = D + 1;
-  constexpr int EPB = 16 / (int)sizeof(E);             // elements per 16 bytes
-  constexpr int HL = (K + EPB - 1) / EPB * EPB;        // left pad: strip 16-byte aligned
-  constexpr int RW = (HL + W + K + EPB - 1) / EPB * EPB;
+  constexpr int R = G::R;
+  constexpr int EPB = G::EPB;
+  constexpr int HL = G::HL;
+  constexpr int RW = G::RW;
   static_assert(sizeof(E) * VEC == 16, "16 bytes per lane");
-  static_assert(D >= 3, "the store allowance needs two stores between a row's DMA and its wait");
+  static_assert(D >= 2 && 3 * (D - 1) < 64, "vmcnt range");
   constexpr unsigned ESZ = (unsigned)sizeof(E);  // DMA size operands must be 32-bit constants
-  __shared__ __align__(16) E ring[R][RW];
+  E (*ring)[RW] = reinterpret_cast<E (*)[RW]>(ring_mem);
 
   const int lane = threadIdx.x;
   const int c0 = tx * W;
   const int r0 = ty * J.th;
   const int r_end = min(r0 + J.th, J.nr);
   const int first = r0 - K, last = r_end - 1 + K;
+  const int nsteps = last - first + 1;  // step q filters input row row_of(q)
   const int col = c0 + VEC * lane;
   const int hcl = max(c0 - K + lane, 0), hcr = min(c0 + W + lane, J.nc - 1);
+  const bool nt = J.stream_out;
 
   float c[P];
 #pragma unroll
   for (int m = 0; m < P; m++) c[m] = cp.c[m];
 
-  auto issue = [&](int ri) {
-    E* row_lds = ring[__builtin_amdgcn_readfirstlane((ri - first) % R)];
-    const E* row = px<E>(J.in, J.in_pitch, clampi(ri, 0, J.nr - 1), 0);
+  auto row_of = [&](int q) { return UP ? last - q : first + q; };
+  auto issue = [&](int q) {
+    E* row_lds = ring[__builtin_amdgcn_readfirstlane(q % R)];
+    const E* row = px<E>(J.in, J.in_pitch, clampi(row_of(q), 0, J.nr - 1), 0);
     __builtin_amdgcn_global_load_lds(glb_addr(row + col), lds_addr(row_lds + HL), 16u, 0, 0);
     if (lane < K) {
       __builtin_amdgcn_global_load_lds(glb_addr(row + hcl), lds_addr(row_lds + HL - K), ESZ, 0, 0);
@@ -354,24 +412,22 @@ __device__ __forceinline__ void gauss_tile_dma(const gauss_job& J, int tx, int t
   };
 #pragma unroll
   for (int d = 0; d < D; d++)
-    if (first + d <= last) issue(first + d);
+    if (d < nsteps) issue(d);
 
   E hist[P][VEC];
   E pend[VEC];        // output row of the previous step, stored after this step's wait
   int pend_row = -1;
-  for (int base = first; base <= last; base += P) {
+  for (int qb = 0; qb < nsteps; qb += P) {
     static_for<P>([&](auto tc) {
       constexpr int t = decltype(tc)::value;
-      const int ri = base + t;
-      if (ri > last) return;
-      // a store is issued at step s iff s >= first + 2K + 1: steps ri-1 and
-      // ri-2 both stored (after DMA(ri), issued at step ri - D <= ri - 3)
-      if (ri + D - 1 > last) wait_vm<0>();
-      else if (ri >= first + 2 * K + 3) wait_vm<3 * (D - 1) + 2>();
+      const int q = qb + t;
+      if (q >= nsteps) return;
+      if (q + D - 1 >= nsteps) wait_vm<0>();
       else wait_vm<3 * (D - 1)>();
-      if (pend_row >= 0) global_store16(px<E>(J.out, J.out_pitch, pend_row, 0) + col, *reinterpret_cast<float4*>(pend));
+      if (pend_row >= 0)
+        global_store16(px<E>(J.out, J.out_pitch, pend_row, 0) + col, *reinterpret_cast<float4*>(pend), nt);
       pend_row = -1;
-      const E* row_lds = ring[(ri - first) % R];
+      const E* row_lds = ring[q % R];
       // 16-byte chunks covering elements [HL - K + VEC*lane, HL + K + VEC*(lane+1))
       constexpr int CL = (HL - K) / EPB;
       constexpr int NCH = (HL + K + VEC + EPB - 1) / EPB - CL;
@@ -381,8 +437,8 @@ __device__ __forceinline__ void gauss_tile_dma(const gauss_job& J, int tx, int t
       const E* che = reinterpret_cast<const E*>(ch);
 #pragma unroll
       for (int m = 0; m < VEC + 2 * K; m++) x[m] = che[HL - K - CL * EPB + m];
-      // the slot of row ri + D held row ri - 1, read by the previous step
-      if (ri + D <= last) issue(ri + D);
+      // the slot of step q + D held step q - 1, read by the previous step
+      if (q + D < nsteps) issue(q + D);
 #pragma unroll
       for (int i = 0; i < VEC; i++) {
         E s = mul_e(c[2 * K], x[i + 2 * K]);
@@ -390,19 +446,31 @@ __device__ __forceinline__ void gauss_tile_dma(const gauss_job& J, int tx, int t
         for (int m = 2 * K - 1; m >= 0; m--) s = fma_e(c[m], x[i + m], s);
         hist[t][i] = s;
       }
-      if (ri - K >= r0) {
+      if (q >= 2 * K) {
+        // output row o = row_of(q) -+ K; tap m (weight c[m]) is row o - K + m,
+        // accumulated from m = 2K down to 0.  Going down, row o - K + m was
+        // filtered at step q - 2K + m (slot (t + m + 1) mod P); going up, at
+        // step q - m (slot (t + P - m) mod P).
 #pragma unroll
         for (int i = 0; i < VEC; i++) {
-          E s = mul_e(c[2 * K], hist[t][i]);
+          E s;
+          if constexpr (!UP) {
+            s = mul_e(c[2 * K], hist[t][i]);
 #pragma unroll
-          for (int m = 2 * K - 1; m >= 0; m--) s = fma_e(c[m], hist[(t + m + 1) % P][i], s);
+            for (int m = 2 * K - 1; m >= 0; m--) s = fma_e(c[m], hist[(t + m + 1) % P][i], s);
+          } else {
+            s = mul_e(c[2 * K], hist[(t + 1) % P][i]);
+#pragma unroll
+            for (int m = 2 * K - 1; m >= 0; m--) s = fma_e(c[m], hist[(t + P - m) % P][i], s);
+          }
           pend[i] = s;
         }
-        pend_row = ri - K;
+        pend_row = UP ? row_of(q) + K : row_of(q) - K;
       }
     });
   }
-  if (pend_row >= 0) global_store16(px<E>(J.out, J.out_pitch, pend_row, 0) + col, *reinterpret_cast<float4*>(pend));
+  if (pend_row >= 0)
+    global_store16(px<E>(J.out, J.out_pitch, pend_row, 0) + col, *reinterpret_cast<float4*>(pend), nt);
 }
 
 }  // namespace cuimg

@@ -44,6 +44,10 @@ GAUSS_CASES = [
     ("f32", (40, 1024), 5),
     ("f32x4", (200, 640), 2),
     ("f32x4", (64, 128), 4),
+    # outputs above 32 MiB: non-temporal stores
+    ("f32", (2500, 3584), 1),
+    ("f32x4", (1100, 2048), 2),
+    ("u8", (3000, 3000), 1),
 ]
 
 
@@ -93,6 +97,20 @@ def test_gaussian_tile_heights_agree(ctx, orc, th, width):
     dst = ctx.image(300, width, cu.F32)
     ctx.convolve(src, dst, coef, 6, th=th)
     assert rel_err(dst.download(), orc.gaussian(img, coef, 6)) < 1e-5
+
+
+def test_gaussian_tile_directions_bit_identical(ctx):
+    """Odd tile rows stream bottom-up; every output must carry the same bits as
+    when a single top-down tile computes it."""
+    rng = np.random.default_rng(11)
+    img = rng.uniform(0, 255, (300, 768)).astype(np.float32)
+    src = ctx.upload(img)
+    one, alt = ctx.image(300, 768, cu.F32), ctx.image(300, 768, cu.F32)
+    for sigma in (1, 3):
+        coef = cu.gaussian_coefs(sigma, 3 * sigma)
+        ctx.convolve(src, one, coef, 3 * sigma, th=300)
+        ctx.convolve(src, alt, coef, 3 * sigma, th=16)
+        assert np.array_equal(one.download(), alt.download()), sigma
 
 
 def clamp_pad(a, b):
