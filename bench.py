@@ -230,7 +230,20 @@ def batch_bench(cu, ctx):
     return out
 
 
-def gaussian_microbench(cu, ctx, reps=20):
+def span_us(ctx, launch, reps):
+    """Average microseconds per launch of `reps` back-to-back launches, timed by
+    two events on the context stream (after one untimed pass over the 4 buffer
+    sets)."""
+    for i in range(4):
+        launch(i)
+    ctx.sync()
+    ctx.timer_start()
+    for i in range(reps):
+        launch(i)
+    return ctx.timer_stop() * 1000 / reps
+
+
+def gaussian_microbench(cu, ctx, reps=40):
     """Separable Gaussian on f32 4096x4096 and f32x4 3840x2160 (sigma 1, 2, 4;
     2*ceil(3 sigma)+1 taps), 4 distinct buffer pairs rotated so the working set
     (>= 512 MiB) exceeds the 256 MB Infinity Cache; and the 5-level 2x pyramid
@@ -249,16 +262,7 @@ def gaussian_microbench(cu, ctx, reps=20):
         nbytes = shape[0] * shape[1] * es * 2
         for sigma in (1, 2, 4):
             K = 3 * sigma
-            for i in range(4):
-                ctx.gaussian(ins[i], outs[i], sigma, K)
-            ctx.sync()
-            ctx.profile(True)
-            ctx.profile_reset()
-            for i in range(reps):
-                ctx.gaussian(ins[i % 4], outs[i % 4], sigma, K)
-            n, ms = ctx.profile_query("gaussian_sep")
-            ctx.profile(False)
-            us = ms / n * 1000
+            us = span_us(ctx, lambda i: ctx.gaussian(ins[i % 4], outs[i % 4], sigma, K), reps)
             gbs = nbytes / us / 1e3
             out.append({"kernel": "gaussian_sep", "image": f"{kind} {shape[1]}x{shape[0]}", "sigma": sigma,
                         "taps": 2 * K + 1, "avg_us": round(us, 2), "bytes": nbytes, "GBps": round(gbs, 1),
@@ -275,16 +279,7 @@ def gaussian_microbench(cu, ctx, reps=20):
             lv.append(ctx.image(r // 2, c // 2, cu.F32))
         pyrs.append(lv)
     nbytes = 4 * 4096 * 4096 * (1 + 1 / 4 + 1 / 16 + 1 / 64 + 1 / 256)
-    for i in range(4):
-        ctx.pyramid(pyrs[i][0], pyrs[i])
-    ctx.sync()
-    ctx.profile(True)
-    ctx.profile_reset()
-    for i in range(reps):
-        ctx.pyramid(pyrs[i % 4][0], pyrs[i % 4])
-    n, ms = ctx.profile_query("pyramid_build")
-    ctx.profile(False)
-    us = ms / n * 1000
+    us = span_us(ctx, lambda i: ctx.pyramid(pyrs[i % 4][0], pyrs[i % 4]), reps)
     gbs = nbytes / us / 1e3
     out.append({"kernel": "pyramid_build", "image": "f32 4096x4096, 5 levels", "avg_us": round(us, 2),
                 "bytes": int(nbytes), "GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})

@@ -95,6 +95,8 @@ def lib():
         L.cuimg_hip_memcpy_h2d.argtypes = [vp, vp, vp, ctypes.c_size_t]
         L.cuimg_hip_memcpy_d2h.argtypes = [vp, vp, vp, ctypes.c_size_t]
         L.cuimg_hip_memcpy_d2d.argtypes = [vp, vp, vp, ctypes.c_size_t]
+        L.cuimg_hip_timer_start.argtypes = [vp]
+        L.cuimg_hip_timer_stop.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
         L.cuimg_hip_memset.argtypes = [vp, vp, c_int, ctypes.c_size_t]
         L.cuimg_hip_image_alloc.argtypes = [vp, c_int, c_int, c_int, c_int, P(Img)]
         L.cuimg_hip_image_free.argtypes = [vp, P(Img)]
@@ -206,6 +208,16 @@ class Context:
         self.check(self.L.cuimg_hip_profile_query(self.h, name.encode(), ctypes.byref(n), ctypes.byref(ms)),
                    "profile_query")
         return n.value, ms.value
+
+    def timer_start(self):
+        """Records the start event of a span on the context stream."""
+        self.check(self.L.cuimg_hip_timer_start(self.h), "timer_start")
+
+    def timer_stop(self):
+        """Milliseconds of the context stream's work since timer_start."""
+        ms = ctypes.c_double()
+        self.check(self.L.cuimg_hip_timer_stop(self.h, ctypes.byref(ms)), "timer_stop")
+        return ms.value
 
     # images
     def image(self, nrows, ncols, dtype, border=0):

@@ -27,6 +27,8 @@ struct cuimg_hip_ctx {
   std::vector<timed_launch> pending;
   std::vector<hipEvent_t> free_events;
   std::map<std::string, std::pair<int64_t, double>> stats;
+  // span timer (cuimg_hip_timer_start / _stop)
+  hipEvent_t span[2] = {nullptr, nullptr};
 };
 
 namespace cuimg {
@@ -97,6 +99,30 @@ __device__ __forceinline__ const T* px(const void* begin, int pitch, int r, int 
   return reinterpret_cast<const T*>(static_cast<const char*>(begin) + (ptrdiff_t)r * pitch) + c;
 }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Output size above which image kernels write with non-temporal stores: the 8
+// XCDs' L2s (4 MiB each) could not hand such an image to the next kernel anyway.
+constexpr size_t kStreamOutBytes = 32u << 20;
+
+// Non-temporal stores (the line is not kept in L2).  Written in asm: as
+// __builtin_nontemporal_store the compiler merges `nt ? nt_store : store` into
+// one ordinary store.  Nothing in the kernels reads these locations back, so no
+// wait is placed for them; the s_nop after a 16-byte store covers the gfx9
+// hazard of overwriting a wide store's data registers.
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+typedef float v2f_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_nt(float* p, float v) {
+  asm volatile("global_store_dword %0, %1, off nt" : : "v"(p), "v"(v) : "memory");
+}
+__device__ __forceinline__ void store_nt(uint8_t* p, uint8_t v) {
+  asm volatile("global_store_byte %0, %1, off nt" : : "v"(p), "v"((uint32_t)v) : "memory");
+}
+__device__ __forceinline__ void store_nt(void* p, v2f_t v) {
+  asm volatile("global_store_dwordx2 %0, %1, off nt" : : "v"(p), "v"(v) : "memory");
+}
+__device__ __forceinline__ void store_nt(void* p, v4f_t v) {
+  asm volatile("global_store_dwordx4 %0, %1, off nt\n\ts_nop 1" : : "v"(p), "v"(v) : "memory");
+}
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on XCD
 // b mod 8).  xcd_order(b, nb) renumbers a grid of nb workgroups so that each

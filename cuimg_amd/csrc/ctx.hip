@@ -182,6 +182,8 @@ int cuimg_hip_ctx_destroy(cuimg_hip_ctx* ctx) {
     (void)hipEventDestroy(t.b);
   }
   for (auto e : ctx->free_events) (void)hipEventDestroy(e);
+  for (auto e : ctx->span)
+    if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return 0;
@@ -230,6 +232,24 @@ int cuimg_hip_profile_query(cuimg_hip_ctx* ctx, const char* kernel, int64_t* lau
   auto it = ctx->stats.find(kernel ? kernel : "");
   if (launches) *launches = it == ctx->stats.end() ? 0 : it->second.first;
   if (total_ms) *total_ms = it == ctx->stats.end() ? 0.0 : it->second.second;
+  return 0;
+}
+
+int cuimg_hip_timer_start(cuimg_hip_ctx* ctx) {
+  if (!ctx) return CUIMG_E_INVALID;
+  for (auto& e : ctx->span)
+    if (!e) CUIMG_HIP(ctx, hipEventCreate(&e));
+  CUIMG_HIP(ctx, hipEventRecord(ctx->span[0], ctx->stream));
+  return 0;
+}
+
+int cuimg_hip_timer_stop(cuimg_hip_ctx* ctx, double* ms) {
+  if (!ctx || !ms || !ctx->span[0]) return CUIMG_E_INVALID;
+  CUIMG_HIP(ctx, hipEventRecord(ctx->span[1], ctx->stream));
+  CUIMG_HIP(ctx, hipEventSynchronize(ctx->span[1]));
+  float t = 0.f;
+  CUIMG_HIP(ctx, hipEventElapsedTime(&t, ctx->span[0], ctx->span[1]));
+  *ms = t;
   return 0;
 }
 

@@ -23,10 +23,6 @@ static int dispatch_k(cuimg_hip_ctx* ctx, const gauss_jobs& jobs, int blocks, co
   return set_error(ctx, CUIMG_E_UNSUPPORTED, "gaussian: half width %d not supported (1..%d)", K, kMaxK);
 }
 
-// Output size above which stores are non-temporal: the 8 XCDs' L2s (4 MiB
-// each) could not hand the image to the next kernel anyway.
-constexpr size_t kStreamOutBytes = 32u << 20;
-
 static int pick_th(int nr, int tiles_x, int total_tiles_other) {
   // aim for >= ~2048 waves in flight over the whole launch, 16..128 rows per tile
   long want = 2048 - total_tiles_other;

@@ -116,24 +116,21 @@ struct io<float4, float4> {
   __device__ static float4 load1(const float4* p) { return *p; }
 };
 
-typedef float v4f_t __attribute__((ext_vector_type(4)));
-typedef float v2f_t __attribute__((ext_vector_type(2)));
-
 // Vector store; nt: non-temporal (not kept in L2).
 template <int VEC>
 __device__ __forceinline__ void store_vec(float* p, const float (&v)[VEC], bool nt) {
   if constexpr (VEC == 4) {
     const v4f_t w = {v[0], v[1], v[2], v[3]};
-    if (nt) __builtin_nontemporal_store(w, reinterpret_cast<v4f_t*>(p));
+    if (nt) store_nt(p, w);
     else *reinterpret_cast<v4f_t*>(p) = w;
   } else if constexpr (VEC == 2) {
     const v2f_t w = {v[0], v[1]};
-    if (nt) __builtin_nontemporal_store(w, reinterpret_cast<v2f_t*>(p));
+    if (nt) store_nt(p, w);
     else *reinterpret_cast<v2f_t*>(p) = w;
   } else {
 #pragma unroll
     for (int i = 0; i < VEC; i++) {
-      if (nt) __builtin_nontemporal_store(v[i], p + i);
+      if (nt) store_nt(p + i, v[i]);
       else p[i] = v[i];
     }
   }
@@ -143,7 +140,7 @@ __device__ __forceinline__ void store_vec(float4* p, const float4 (&v)[VEC], boo
 #pragma unroll
   for (int i = 0; i < VEC; i++) {
     const v4f_t w = {v[i].x, v[i].y, v[i].z, v[i].w};
-    if (nt) __builtin_nontemporal_store(w, reinterpret_cast<v4f_t*>(p + i));
+    if (nt) store_nt(p + i, w);
     else *reinterpret_cast<v4f_t*>(p + i) = w;
   }
 }
@@ -327,15 +324,11 @@ __device__ __forceinline__ void lds_read16<4>(const uint32_t* a, float4* r) {
 
 // 16-byte store out of the compiler's view, so that no wait for it is placed
 // (nothing in the kernel reads what it writes); the s_nop covers the gfx9
-// hazard of overwriting a wide store's data registers.  nt: non-temporal (the
-// line is not kept in L2), for outputs that no later kernel reads from cache.
+// hazard of overwriting a wide store's data registers.  nt: non-temporal.
 __device__ __forceinline__ void global_store16(void* p, float4 v, bool nt) {
-  typedef float v4f __attribute__((ext_vector_type(4)));
-  const v4f w = {v.x, v.y, v.z, v.w};
-  if (nt)
-    asm volatile("global_store_dwordx4 %0, %1, off nt\n\ts_nop 1" : : "v"(p), "v"(w) : "memory");
-  else
-    asm volatile("global_store_dwordx4 %0, %1, off\n\ts_nop 1" : : "v"(p), "v"(w) : "memory");
+  const v4f_t w = {v.x, v.y, v.z, v.w};
+  if (nt) store_nt(p, w);
+  else asm volatile("global_store_dwordx4 %0, %1, off\n\ts_nop 1" : : "v"(p), "v"(w) : "memory");
 }
 
 // r[k] = the 16 bytes at LDS byte address base + 16 k, k < N, in blocks of 4.

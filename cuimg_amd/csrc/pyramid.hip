@@ -2,11 +2,29 @@
 // in pyramid_dev.hpp).
 #include "pyramid_dev.hpp"
 
+#include <algorithm>
+
 namespace cuimg {
 
 template <typename T>
-__global__ __launch_bounds__(256) void pyramid_kernel(pyr_args a) {
+__global__ __launch_bounds__(kPyrThreads) void pyramid_kernel(pyr_args a) {
   pyramid_tile<T>(a, blockIdx.x);
+}
+
+// Workgroups wanted per launch (about 4 per CU) before tiles get narrower.
+constexpr int kPyrMinTiles = 960;
+
+int pyr_tiling(pyr_args& a, int nr, int nc, int es) {
+  int tile_log2 = 0;
+  while ((1 << (tile_log2 + 1)) <= kPyrTileBytes / es) tile_log2++;  // elements per tile
+  const int thl = std::max(4, a.n - 1);          // >= 16 rows and 2^(n-1)
+  int twl = tile_log2 - thl;                     // widest 16 KiB tile
+  auto tiles = [&](int w) { return (long)ceil_div(nc, 1 << w) * ceil_div(nr, 1 << thl); };
+  while (twl > 6 && twl > a.n - 1 && tiles(twl) < kPyrMinTiles) twl--;
+  a.tw_log2 = twl;
+  a.th_log2 = thl;
+  a.tiles_x = ceil_div(nc, 1 << twl);
+  return (int)tiles(twl);
 }
 
 static bool aligned16(const void* p, int pitch) {
@@ -36,12 +54,13 @@ int pyramid_build(cuimg_hip_ctx* ctx, const cuimg_img* frame, const cuimg_img* l
     a.lv[l] = pyr_level{L.begin, L.pitch, L.nrows, L.ncols, L.border};
   }
   a.frame_vec_ok = aligned16(frame->begin, frame->pitch) && aligned16(levels[0].begin, levels[0].pitch);
-  a.tiles_x = ceil_div(levels[0].ncols, kTile);
-  const int blocks = a.tiles_x * ceil_div(levels[0].nrows, kTile);
+  const int es = dt == CUIMG_U8 ? 1 : 4;
+  a.stream_out = (size_t)levels[0].nrows * levels[0].ncols * es > kStreamOutBytes;
+  const int blocks = pyr_tiling(a, levels[0].nrows, levels[0].ncols, es);
   if (blocks == 0) return 0;
   return launch(ctx, "pyramid_build", [&] {
-    if (dt == CUIMG_U8) pyramid_kernel<uint8_t><<<blocks, 256, 0, ctx->stream>>>(a);
-    else pyramid_kernel<float><<<blocks, 256, 0, ctx->stream>>>(a);
+    if (dt == CUIMG_U8) pyramid_kernel<uint8_t><<<blocks, kPyrThreads, 0, ctx->stream>>>(a);
+    else pyramid_kernel<float><<<blocks, kPyrThreads, 0, ctx->stream>>>(a);
   });
 }
 

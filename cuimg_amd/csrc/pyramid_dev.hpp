@@ -7,12 +7,17 @@
 // that order then / 4 for float; level l+1 has floor(n_l / 2) rows and cols.
 // Every level's border is then clamp-filled (cuimg/border.h:128-137).
 //
-// Design: a 256-thread workgroup owns a 64x64 tile of level 0 and the nested
-// 32x32, 16x16, ... tiles of the coarser levels (2x2 blocks never cross tiles).
-// It reads the tile once with 16-byte loads into LDS, writes level 0 (unless
-// level 0 is the source itself), and builds each coarser level from the LDS copy
-// of the previous one, writing it with its share of the clamped border.  HBM
-// traffic is one read of level 0 plus one write of every produced level.
+// Design: a 256-thread workgroup owns a TW x TH tile of level 0 (at most 16 KiB,
+// TH >= 2^(n-1)) and the nested TW/2 x TH/2, ... tiles of the coarser levels
+// (2x2 blocks never cross tiles).  Full 16-byte chunks of the tile stream
+// straight into LDS (global_load_lds, all of a thread's chunks in flight at
+// once), ragged edges go through registers.  Level 0 is written back (unless
+// the frame is level 0 itself) and each coarser level is built from the LDS copy
+// of the previous one and written with its share of the clamped border.  HBM
+// traffic is one read of level 0 plus one write of every produced level.  Wide,
+// short tiles make each wave's loads whole row segments (one 1 KiB row per
+// wave instruction for f32 256-wide tiles); images too large to stay in cache
+// are written with non-temporal stores.
 #pragma once
 
 #include "common.hpp"
@@ -20,7 +25,8 @@
 namespace cuimg {
 
 constexpr int kPyrMaxLevels = 8;
-constexpr int kTile = 64;
+constexpr int kPyrThreads = 256;          // workgroup size of every launch of pyramid_tile
+constexpr int kPyrTileBytes = 16384;      // LDS of the level-0 tile
 
 struct pyr_level {
   void* begin;
@@ -34,8 +40,15 @@ struct pyr_args {
   int copy_level0;    // frame is not level 0 itself
   int n;
   int tiles_x;
+  int tw_log2, th_log2; // level-0 tile: 2^tw_log2 columns x 2^th_log2 rows
+  int stream_out;       // non-temporal stores
   pyr_level lv[kPyrMaxLevels];
 };
+
+// Host: the tile shape for an nr x nc level 0 of es-byte pixels and n levels
+// (pyramid.hip).  Sets a.tw_log2, a.th_log2 and a.tiles_x; returns the number
+// of tiles.
+int pyr_tiling(pyr_args& a, int nr, int nc, int es);
 
 template <typename T>
 struct pyr_ops;
@@ -73,57 +86,96 @@ __device__ inline void tile_border(const pyr_level& L, const T* tile, int tw, in
   }
 }
 
-// One 64x64 level-0 tile (and its nested coarser tiles) per 256-thread
-// workgroup.
+template <typename T>
+__device__ __forceinline__ void pyr_store(T* p, T v, bool nt) {
+  if (nt) store_nt(p, v);
+  else *p = v;
+}
+__device__ __forceinline__ void pyr_store16(void* p, uint4 v, bool nt) {
+  typedef unsigned v4u __attribute__((ext_vector_type(4)));
+  const v4u w = {v.x, v.y, v.z, v.w};
+  if (nt) store_nt(p, __builtin_bit_cast(v4f_t, w));
+  else *reinterpret_cast<v4u*>(p) = w;
+}
+
+// One level-0 tile (and its nested coarser tiles) per kPyrThreads workgroup.
 template <typename T>
 __device__ inline void pyramid_tile(const pyr_args& a, int tile) {
-  __shared__ T t0[kTile * kTile];
-  __shared__ T t1[(kTile / 2) * (kTile / 2)];
+  __shared__ __align__(16) T t0[kPyrTileBytes / sizeof(T)];
+  __shared__ __align__(16) T t1[kPyrTileBytes / 4 / sizeof(T)];
   T* cur = t0;
   T* nxt = t1;
 
+  const int twl = a.tw_log2, thl = a.th_log2;
+  const int TW = 1 << twl, TH = 1 << thl;
   const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
   const pyr_level& L0 = a.lv[0];
-  const int r0 = ty * kTile, c0 = tx * kTile;
-  const int r1 = min(r0 + kTile, L0.nr), c1 = min(c0 + kTile, L0.nc);
+  const int r0 = ty * TH, c0 = tx * TW;
+  const int r1 = min(r0 + TH, L0.nr), c1 = min(c0 + TW, L0.nc);
+  const bool nt = a.stream_out;
 
-  // ---- level 0 tile: load (and copy when the frame is a separate image)
-  constexpr int CH = 16 / sizeof(T);           // elements per 16-byte chunk
-  constexpr int CPR = kTile / CH;              // chunks per tile row
-  if (a.frame_vec_ok && c1 - c0 == kTile) {
-    for (int i = threadIdx.x; i < kTile * CPR; i += blockDim.x) {
-      const int lr = i / CPR, lc = (i % CPR) * CH;
-      if (r0 + lr >= r1) continue;
-      uint4 v = *reinterpret_cast<const uint4*>(px<T>(a.frame, a.frame_pitch, r0 + lr, c0 + lc));
-      *reinterpret_cast<uint4*>(&t0[lr * kTile + lc]) = v;
-      if (a.copy_level0) *reinterpret_cast<uint4*>(px<T>(L0.begin, L0.pitch, r0 + lr, c0 + lc)) = v;
+  // ---- level-0 tile: chunk i covers tile row i / CPR, columns (i % CPR) * CH ..
+  constexpr int CH = 16 / sizeof(T);        // elements per 16-byte chunk
+  constexpr int CH_LOG2 = sizeof(T) == 1 ? 4 : 2;
+  static_assert(sizeof(T) == 1 || sizeof(T) == 4, "u8 or f32 pixels");
+  constexpr int NPT = kPyrTileBytes / 16 / kPyrThreads;  // chunks per thread, at most
+  const int cprl = twl - CH_LOG2;
+  const int nchunks = 1 << (twl + thl - CH_LOG2);
+  if (a.frame_vec_ok && c1 - c0 == TW) {
+    // whole chunks: the tile is dense in LDS, so the 64 chunks of one wave
+    // instruction land at consecutive 16-byte slots from the wave's first one
+#pragma unroll
+    for (int k = 0; k < NPT; k++) {
+      const int i = threadIdx.x + kPyrThreads * k;
+      if (i < nchunks && r0 + (i >> cprl) < r1) {
+        const int wi = __builtin_amdgcn_readfirstlane(i & ~63);
+        const T* g = px<T>(a.frame, a.frame_pitch, r0 + (i >> cprl), c0 + ((i & ((1 << cprl) - 1)) << CH_LOG2));
+        __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void*)g,
+                                         (__attribute__((address_space(3))) void*)(t0 + (wi << CH_LOG2)), 16u, 0, 0);
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0 | (7 << 4) | (15 << 8));  // vmcnt(0)
+    if (a.copy_level0) {
+#pragma unroll
+      for (int k = 0; k < NPT; k++) {
+        const int i = threadIdx.x + kPyrThreads * k;
+        if (i < nchunks && r0 + (i >> cprl) < r1) {
+          // own chunk: written by this lane's DMA, complete after the wait
+          const uint4 v = *reinterpret_cast<const uint4*>(t0 + (i << CH_LOG2));
+          pyr_store16(px<T>(L0.begin, L0.pitch, r0 + (i >> cprl), c0 + ((i & ((1 << cprl) - 1)) << CH_LOG2)), v, nt);
+        }
+      }
     }
   } else {
-    for (int i = threadIdx.x; i < kTile * kTile; i += blockDim.x) {
-      const int lr = i / kTile, lc = i % kTile;
-      if (r0 + lr >= r1 || c0 + lc >= c1) continue;
-      T v = *px<T>(a.frame, a.frame_pitch, r0 + lr, c0 + lc);
-      t0[lr * kTile + lc] = v;
-      if (a.copy_level0) *px<T>(L0.begin, L0.pitch, r0 + lr, c0 + lc) = v;
+    for (int i = threadIdx.x; i < nchunks; i += kPyrThreads) {
+      const int lr = i >> cprl, lc = (i & ((1 << cprl) - 1)) << CH_LOG2;
+      const int r = r0 + lr, c = c0 + lc;
+      if (r >= r1 || c >= c1) continue;
+      for (int k = 0; k < CH && c + k < c1; k++) {
+        const T v = *px<T>(a.frame, a.frame_pitch, r, c + k);
+        t0[(lr << twl) + lc + k] = v;
+        if (a.copy_level0) pyr_store(px<T>(L0.begin, L0.pitch, r, c + k), v, nt);
+      }
     }
   }
   __syncthreads();
-  tile_border<T>(L0, t0, kTile, r0, r1, c0, c1);
+  tile_border<T>(L0, t0, TW, r0, r1, c0, c1);
 
   // ---- coarser levels from the LDS copy of the previous level
-  int tw = kTile;
+  int wl = twl, hl = thl;
   for (int l = 1; l < a.n; l++) {
     const pyr_level& L = a.lv[l];
-    const int ntw = tw / 2;
-    const int lr0 = ty * ntw, lc0 = tx * ntw;
-    const int lr1 = min(lr0 + ntw, L.nr), lc1 = min(lc0 + ntw, L.nc);
-    for (int i = threadIdx.x; i < ntw * ntw; i += blockDim.x) {
-      const int r = i / ntw, c = i % ntw;
+    const int nwl = wl - 1, nhl = hl - 1;
+    const int ntw = 1 << nwl;
+    const int lr0 = ty << nhl, lc0 = tx << nwl;
+    const int lr1 = min(lr0 + (1 << nhl), L.nr), lc1 = min(lc0 + ntw, L.nc);
+    for (int i = threadIdx.x; i < (1 << (nwl + nhl)); i += kPyrThreads) {
+      const int r = i >> nwl, c = i & (ntw - 1);
       if (lr0 + r >= lr1 || lc0 + c >= lc1) continue;
-      const T* s = &cur[(2 * r) * tw + 2 * c];
-      T v = pyr_ops<T>::mean4(s[0], s[1], s[tw], s[tw + 1]);
-      nxt[r * ntw + c] = v;
-      *px<T>(L.begin, L.pitch, lr0 + r, lc0 + c) = v;
+      const T* s = &cur[((2 * r) << wl) + 2 * c];
+      T v = pyr_ops<T>::mean4(s[0], s[1], s[1 << wl], s[(1 << wl) + 1]);
+      nxt[(r << nwl) + c] = v;
+      pyr_store(px<T>(L.begin, L.pitch, lr0 + r, lc0 + c), v, nt);
     }
     __syncthreads();
     if (lr0 < lr1 && lc0 < lc1) tile_border<T>(L, nxt, ntw, lr0, lr1, lc0, lc1);
@@ -131,7 +183,8 @@ __device__ inline void pyramid_tile(const pyr_args& a, int tile) {
     T* t = cur;
     cur = nxt;
     nxt = t;
-    tw = ntw;
+    wl = nwl;
+    hl = nhl;
   }
 }
 

@@ -176,7 +176,7 @@ __device__ __forceinline__ step_flags flags_of(const level_job& J) {
 
 // tracker::update_input + subsample_input for every stream: the u8 pyramid of
 // the new frame into the frame's slot.
-__global__ __launch_bounds__(256) void step_pyramid_kernel(const stream_job* S, const level_job* jobs,
+__global__ __launch_bounds__(kPyrThreads) void step_pyramid_kernel(const stream_job* S, const level_job* jobs,
                                                            const frame_in* in, const uint32_t* prefix, int nstreams) {
   __shared__ pyr_args a;
   const int st = find_job(prefix, nstreams, blockIdx.x);
@@ -532,7 +532,7 @@ static void build_stream(cuimg_hip_batch* b, int i) {
     }
     const cuimg_img& L0 = b->lv[S.first].u8[slot];
     a.frame_vec_ok = ((uintptr_t)L0.begin % 16 == 0) && (L0.pitch % 16 == 0);
-    a.tiles_x = ceil_div(L0.ncols, kTile);
+    pyr_tiling(a, L0.nrows, L0.ncols, 1);
   }
 }
 
@@ -540,8 +540,8 @@ static void build_stream(cuimg_hip_batch* b, int i) {
 static int build_maps(cuimg_hip_batch* b) {
   std::vector<uint32_t> cnt[K_NKINDS];
   for (int i = 0; i < b->nstreams; i++) {
-    const level_res& R0 = b->lv[b->first[i]];
-    cnt[K_PYR].push_back((uint32_t)(ceil_div(R0.nc, kTile) * ceil_div(R0.nr, kTile)));
+    const pyr_args& a = b->hs[i].pyr[0];  // both slots have the same tiling
+    cnt[K_PYR].push_back((uint32_t)(a.tiles_x * ceil_div(a.lv[0].nr, 1 << a.th_log2)));
   }
   const int th = blur_th(b);
   for (int j = 0; j < b->njobs; j++) {
@@ -672,7 +672,7 @@ static int enqueue_step(cuimg_hip_batch* b) {
   const level_job* J = b->dj;
   uint32_t* const* pf = b->prefix_d;
   CUIMG_TRY(launch(ctx, "pyramid_build", [&] {
-    step_pyramid_kernel<<<b->total[K_PYR], 256, 0, st>>>(b->ds, J, b->din, pf[K_PYR], b->nstreams);
+    step_pyramid_kernel<<<b->total[K_PYR], kPyrThreads, 0, st>>>(b->ds, J, b->din, pf[K_PYR], b->nstreams);
   }));
   if (b->total[K_BLUR]) CUIMG_TRY(launch(ctx, "gaussian_sep", [&] { launch_blur(b); }));
   CUIMG_TRY(launch(ctx, "fills", [&] { step_fill_kernel<<<b->total[K_FILL], 256, 0, st>>>(J, pf[K_FILL], nj); }));

@@ -76,6 +76,11 @@ int cuimg_hip_profile_reset(cuimg_hip_ctx* ctx);
  * synchronises the context. */
 int cuimg_hip_profile_query(cuimg_hip_ctx* ctx, const char* kernel, int64_t* launches,
                             double* total_ms);
+/* Span timer: milliseconds between the two events recorded on the context
+ * stream by start and stop (stop synchronises); no counterpart in the
+ * reference, used to time back-to-back launches without per-launch events. */
+int cuimg_hip_timer_start(cuimg_hip_ctx* ctx);
+int cuimg_hip_timer_stop(cuimg_hip_ctx* ctx, double* ms);
 
 /* --------------------------------------------------------- memory / images */
 int cuimg_hip_malloc(cuimg_hip_ctx* ctx, size_t bytes, void** out);

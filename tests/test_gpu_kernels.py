@@ -117,7 +117,8 @@ def clamp_pad(a, b):
     return np.pad(a, b, mode="edge")
 
 
-@pytest.mark.parametrize("shape,nlev", [((1080, 1920), 4), ((240, 320), 3), ((135, 241), 3), ((67, 131), 5)])
+@pytest.mark.parametrize("shape,nlev", [((1080, 1920), 4), ((240, 320), 3), ((135, 241), 3), ((67, 131), 5),
+                                        ((640, 960), 6), ((1000, 1300), 7), ((6000, 6000), 3)])
 def test_pyramid_u8_bit_exact(ctx, orc, shape, nlev):
     rng = np.random.default_rng(shape[1])
     frame = rng.integers(0, 256, shape, dtype=np.uint8)
@@ -146,6 +147,26 @@ def test_pyramid_f32_in_place(ctx, orc):
         ref.append(orc.pyr_down(ref[-1]))
     ctx.pyramid(levels[0], levels)
     for l in range(5):
+        assert np.array_equal(levels[l].download(), ref[l]), l
+
+
+@pytest.mark.parametrize("shape,nlev,in_place", [((3072, 3072), 4, False), ((1024, 1536), 7, True), ((96, 4000), 5, False),
+                                                 ((300, 1000), 6, True)])
+def test_pyramid_f32_tilings(ctx, orc, shape, nlev, in_place):
+    """Tile shapes from 64x64 to 256x16, ragged edges, and (3072^2) the
+    non-temporal stores of images above 32 MiB."""
+    print(f"pyramid f32 {shape} {nlev} levels in_place={in_place}", flush=True)
+    rng = np.random.default_rng(shape[0] + nlev)
+    img = rng.uniform(0, 255, shape).astype(np.float32)
+    frame = ctx.upload(img)
+    levels = [frame if in_place else ctx.image(shape[0], shape[1], cu.F32)]
+    ref = [img]
+    for l in range(1, nlev):
+        r, c = levels[-1].shape
+        levels.append(ctx.image(r // 2, c // 2, cu.F32))
+        ref.append(orc.pyr_down(ref[-1]))
+    ctx.pyramid(frame, levels)
+    for l in range(nlev):
         assert np.array_equal(levels[l].download(), ref[l]), l
 
 
