@@ -6,9 +6,14 @@
 // recursive coarse-to-fine prediction (predictions.h:60-74,
 // pyrlk_tracker.hpp:1025-1063, transformations.h:77-80).
 //
-// Design: one wavefront per particle; lane l owns window pixels l, l+64, ...
-// (raster order, NPL per lane).  A level solve is latency-bound, so it is built
-// around few memory round trips:
+// Design: a particle is matched by a group of GL lanes: 16 lanes (four
+// particles per wave, one DPP row each) for windows up to 7x7, a whole wave for
+// larger windows.  Lane l of a group owns window pixels l, l+GL, ... (raster
+// order, NS per lane).  The per-particle scalar work (the 2x2 solve, the
+// update, the tests) then runs once for four particles, and a window sum is a
+// 4-step DPP reduction inside the row.  Groups of one wave diverge only as
+// whole groups.  A level solve is latency-bound, so it is built around few
+// memory round trips:
 //  * the previous frame's (ws+5)^2 patch around the particle and the current
 //    frame's (ws+1+2*kMB)^2 patch around the prediction are loaded into LDS in
 //    one batch of clamped reads;
@@ -17,9 +22,10 @@
 //    gradient image exists; template samples and gradients come from LDS;
 //  * Gauss-Newton steps sample the current-frame patch; it is reloaded only when
 //    the estimate leaves its kMB-pixel margin;
-//  * window sums are wave reductions (DPP within rows, two cross-row shuffles),
-//    bit-identical in every lane; the 2x2 solve runs redundantly per lane in
-//    FP32 registers, so all control flow is wave-uniform.
+//  * window sums are group reductions (DPP within rows, and for 64-lane groups
+//    two cross-row broadcasts), bit-identical in every lane of the group; the
+//    2x2 solve runs redundantly per lane in FP32 registers, so all control
+//    flow is uniform over a group.
 // Sample rows/cols satisfy |tz(x + r) - (tz(x) + r)| <= 1 for float x and
 // integer r, which sizes the patches: every corner a lane reads lies inside.
 // No MFMA: nothing here is a dense contraction.  A particle at scale s first
@@ -64,6 +70,34 @@ __device__ __forceinline__ float wave_sum(float x) {
   x += upd_dpp<0x142, 0xA>(x);    // row_bcast:15 into rows 1, 3
   x += upd_dpp<0x143, 0xC>(x);    // row_bcast:31 into rows 2, 3
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+}
+
+// Sum over a group of GL lanes (16: one DPP row; 64: the wave), the same bits
+// in every lane of the group.
+template <int GL>
+__device__ __forceinline__ float group_sum(float x) {
+  if constexpr (GL == 64) {
+    return wave_sum(x);
+  } else {
+    static_assert(GL == 16, "16- or 64-lane groups");
+    x += mov_dpp<0xB1>(x);   // quad_perm [1,0,3,2]
+    x += mov_dpp<0x4E>(x);   // quad_perm [2,3,0,1]
+    x += mov_dpp<0x141>(x);  // row_half_mirror
+    x += mov_dpp<0x140>(x);  // row_mirror
+    return x;
+  }
+}
+
+// The bits of a wave ballot that belong to this lane's group.
+template <int GL>
+__device__ __forceinline__ uint64_t group_ballot(bool v) {
+  const uint64_t b = __ballot(v);
+  if constexpr (GL == 64) {
+    return b;
+  } else {
+    const int sh = (int)(threadIdx.x & 63) & ~(GL - 1);
+    return (b >> sh) & ((1ull << GL) - 1);
+  }
 }
 
 __device__ __forceinline__ int tz(float v) { return (int)v; }
@@ -117,41 +151,61 @@ struct match_out {
 
 constexpr int kMB = 4;  // motion margin of the current-frame patch
 
-// LDS of one wave, sized for the largest window of its NPL class.  The
+// Window class NPL (1, 2, 4: windows up to 7x7, 11x11, 15x15) and its group
+// layout: GL lanes per particle, NS window pixels per lane.
+template <int NPL>
+struct lk_cfg {
+  static constexpr int WSMAX = NPL == 1 ? 7 : (NPL == 2 ? 11 : 15);
+  static constexpr int GL = NPL == 1 ? 16 : 64;
+  static constexpr int NS = (WSMAX * WSMAX + GL - 1) / GL;
+  static constexpr int GPW = 64 / GL;  // particles per wave in flight
+};
+
+// LDS of one group, sized for the largest window of its class.  The
 // previous- and current-frame patches have a row stride of PW (16 or 32
-// floats) so a wave loads 64/PW whole rows per pass with constant offsets.
+// floats) so a group loads GL/PW whole rows per pass with constant offsets.
 template <int NPL>
 struct lk_lds {
-  static constexpr int WSMAX = NPL == 1 ? 7 : (NPL == 2 ? 11 : 15);
+  using C = lk_cfg<NPL>;
+  static constexpr int WSMAX = C::WSMAX;
   static constexpr int QA = WSMAX + 5;            // previous-frame patch side
   static constexpr int QG = WSMAX + 3;            // gradient patch side
   static constexpr int QB = WSMAX + 1 + 2 * kMB;  // current-frame patch side
   static constexpr int PW = QB <= 16 ? 16 : 32;   // patch row stride (>= QA, QB)
-  static constexpr int RPP = 64 / PW;             // rows per load pass
+  static constexpr int RPP = C::GL / PW;          // rows per load pass
   static constexpr int AROWS = (QA + RPP - 1) / RPP * RPP;
   static constexpr int BROWS = (QB + RPP - 1) / RPP * RPP;
-  static constexpr int GPASS = (QG * QG + 63) / 64;
-  static constexpr int FPASS = ((WSMAX + 1) * (WSMAX + 1) + 63) / 64;
+  static constexpr int GPASS = (QG * QG + C::GL - 1) / C::GL;
+  static constexpr int FPASS = ((WSMAX + 1) * (WSMAX + 1) + C::GL - 1) / C::GL;
   float2 G[QG * QG];
   float A[AROWS * PW];
   float B[BROWS * PW];
-  float tpl[64 * NPL];
-  float mag[64 * NPL];
+  float tpl[C::GL * C::NS];
+  float mag[C::GL * C::NS];
 };
 
-// dst[i*PW + j] = img[clamp(r0 + i), clamp(c0 + j)] for i, j < q: indexing the
-// patch with an unclamped coordinate reproduces the reference's clamped read.
-// Lane l loads column l % PW of rows l / PW + m * (64 / PW).
-template <typename TI, int PW>
+// dst[i*PW + j] = img[clamp(r0 + i), clamp(c0 + j)] for i, j < q <= MAXQ:
+// indexing the patch with an unclamped coordinate reproduces the reference's
+// clamped read.  Group lane l loads column l % PW of rows l / PW + m * (GL / PW);
+// all of a lane's loads are issued before its first LDS write, so a patch
+// costs one memory round trip.
+template <typename TI, int PW, int GL, int MAXQ>
 __device__ __forceinline__ void load_patch(float* dst, int q, const void* img, int pitch, int nr, int nc, int r0,
                                            int c0, int lane) {
-  constexpr int RPP = 64 / PW;
+  constexpr int RPP = GL / PW;
+  constexpr int MP = (MAXQ + RPP - 1) / RPP;
   const int col = lane % PW, row = lane / PW;
   const int c = clampi(c0 + col, 0, nc - 1);
-  const int passes = (q + RPP - 1) / RPP;
-  for (int m = 0; m < passes; m++) {
+  float v[MP];
+#pragma unroll
+  for (int m = 0; m < MP; m++) {
     const int i = row + m * RPP;
-    if (col < q && i < q) dst[lane + 64 * m] = ld<TI>(img, pitch, clampi(r0 + i, 0, nr - 1), c);
+    if (col < q && i < q) v[m] = ld<TI>(img, pitch, clampi(r0 + i, 0, nr - 1), c);
+  }
+#pragma unroll
+  for (int m = 0; m < MP; m++) {
+    const int i = row + m * RPP;
+    if (col < q && i < q) dst[lane + GL * m] = v[m];
   }
 }
 
@@ -163,25 +217,27 @@ __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Per-lane geometry, fixed for a launch: the window pixels of the lane and its
-// gradient-patch elements (row << 8 | col).
+// Per-lane geometry, fixed for a launch: the window pixels of the group lane
+// and its gradient-patch elements (row << 8 | col).
 template <int NPL>
 struct lane_win {
-  int r[NPL], c[NPL];
-  float fr[NPL], fc[NPL];
-  bool in[NPL];
+  static constexpr int NS = lk_cfg<NPL>::NS;
+  int r[NS], c[NS];
+  float fr[NS], fc[NS];
+  bool in[NS];
   int g[lk_lds<NPL>::GPASS];   // whole gradient patch
   int gf[lk_lds<NPL>::FPASS];  // its (ws+1)^2 core
 };
 
 template <int NPL>
 __device__ __forceinline__ lane_win<NPL> make_lane_win(int ws, int lane) {
+  constexpr int GL = lk_cfg<NPL>::GL, NS = lk_cfg<NPL>::NS;
   lane_win<NPL> W;
   const int hws = ws / 2, QG = ws + 3;
   const float iws = 1.f / (float)ws, iqg = 1.f / (float)QG;
 #pragma unroll
-  for (int k = 0; k < NPL; k++) {
-    const int i = lane + 64 * k;
+  for (int k = 0; k < NS; k++) {
+    const int i = lane + GL * k;
     const int q = divq(i, iws);
     W.in[k] = i < ws * ws;
     W.r[k] = q - hws;
@@ -191,7 +247,7 @@ __device__ __forceinline__ lane_win<NPL> make_lane_win(int ws, int lane) {
   }
 #pragma unroll
   for (int m = 0; m < lk_lds<NPL>::GPASS; m++) {
-    const int k = lane + 64 * m;
+    const int k = lane + GL * m;
     const int i = divq(k, iqg);
     W.g[m] = k < QG * QG ? (i << 8) | (k - i * QG) : -1;
   }
@@ -199,21 +255,22 @@ __device__ __forceinline__ lane_win<NPL> make_lane_win(int ws, int lane) {
   const float iqf = 1.f / (float)QF;
 #pragma unroll
   for (int m = 0; m < lk_lds<NPL>::FPASS; m++) {
-    const int k = lane + 64 * m;
+    const int k = lane + GL * m;
     const int i = divq(k, iqf);
     W.gf[m] = k < QF * QF ? ((i + 1) << 8) | (k - i * QF + 1) : -1;
   }
   return W;
 }
 
-// iterative_lk_match_5x5::match for one wave.
+// iterative_lk_match_5x5::match for one group of lanes.
 template <typename TI, int NPL, bool SEL>
 __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk_params_dev& P,
                                                 const lane_win<NPL>& W, float2 p, float2 pred_tr, lk_lds<NPL>& S) {
   using LD = lk_lds<NPL>;
   constexpr int PW = LD::PW;
   const match_out fail{make_float2(-1.f, -1.f), 1000.f};
-  const int lane = threadIdx.x & 63;
+  constexpr int GL = lk_cfg<NPL>::GL, NS = lk_cfg<NPL>::NS;
+  const int lane = threadIdx.x & (GL - 1);
   const int ws = P.ws, hws = ws / 2, n = ws * ws;
   const int QA = ws + 5, QG = ws + 3, QB = ws + 1 + 2 * kMB;
   const float2 prediction = make_float2(add_rn(p.x, pred_tr.x), add_rn(p.y, pred_tr.y));
@@ -223,8 +280,8 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
   const int oAr = tz(p.x) - hws - 2, oAc = tz(p.y) - hws - 2;
   int oBr = tz(v.x) - hws - kMB, oBc = tz(v.y) - hws - kMB;
   wave_sync_lds();
-  load_patch<TI, PW>(S.A, QA, L.A, L.pitchA, L.nr, L.nc, oAr, oAc, lane);
-  load_patch<TI, PW>(S.B, QB, L.B, L.pitchB, L.nr, L.nc, oBr, oBc, lane);
+  load_patch<TI, PW, GL, LD::QA>(S.A, QA, L.A, L.pitchA, L.nr, L.nc, oAr, oAc, lane);
+  load_patch<TI, PW, GL, LD::QB>(S.B, QB, L.B, L.pitchB, L.nr, L.nc, oBr, oBc, lane);
   wave_sync_lds();
 
   // ---- Sobel/8 of the previous frame at global pixel (oA + 1 + (i, j));
@@ -234,7 +291,7 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
   // tz(p) + (r, c) only the (ws+1)^2 core of the patch is read.
   bool dev = false;
 #pragma unroll
-  for (int k = 0; k < NPL; k++)
+  for (int k = 0; k < NS; k++)
     if (W.in[k]) {
       const int xr = tz(add_rn(p.x, W.fr[k])), xc = tz(add_rn(p.y, W.fc[k]));
       dev |= xr != oAr + hws + 2 + W.r[k] || xc != oAc + hws + 2 + W.c[k];
@@ -263,7 +320,7 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
     }
     S.G[i * QG + j] = g;
   };
-  if (__ballot(dev)) {
+  if (group_ballot<GL>(dev)) {
 #pragma unroll
     for (int m = 0; m < LD::GPASS; m++) sobel(W.g[m]);
   } else {
@@ -274,12 +331,12 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
 
   // ---- template: bilinear samples of A and of its gradient (pixels whose
   // truncated coordinate is inside the image)
-  float2 gs[NPL];
-  float as[NPL];
-  bool sel[NPL];
+  float2 gs[NS];
+  float as[NS];
+  bool sel[NS];
 #pragma unroll
-  for (int k = 0; k < NPL; k++) {
-    const int i = lane + 64 * k;
+  for (int k = 0; k < NS; k++) {
+    const int i = lane + GL * k;
     gs[k] = make_float2(0.f, 0.f);
     as[k] = 0.f;
     sel[k] = false;
@@ -303,8 +360,8 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
   if (SEL) {
     // rank among valid pixels by (|g| desc, raster index asc); keep the top_n
 #pragma unroll
-    for (int k = 0; k < NPL; k++) {
-      const int i = lane + 64 * k;
+    for (int k = 0; k < NS; k++) {
+      const int i = lane + GL * k;
       if (sel[k]) {
         const float m = S.mag[i];
         int rank = 0;
@@ -320,15 +377,15 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
   float g00 = 0.f, g01 = 0.f, g11 = 0.f;
   int cnt = 0;
 #pragma unroll
-  for (int k = 0; k < NPL; k++) {
+  for (int k = 0; k < NS; k++) {
     if (sel[k]) {
       g00 += gs[k].x * gs[k].x;
       g01 += gs[k].x * gs[k].y;
       g11 += gs[k].y * gs[k].y;
     }
-    cnt += __popcll(__ballot(sel[k]));
+    cnt += __popcll(group_ballot<GL>(sel[k]));
   }
-  const float G00 = wave_sum(g00), G01 = wave_sum(g01), G11 = wave_sum(g11);
+  const float G00 = group_sum<GL>(g00), G01 = group_sum<GL>(g01), G11 = group_sum<GL>(g11);
   const float nsel = (float)cnt;
   const float G10 = G01;
   {
@@ -358,7 +415,7 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
     oBr = vr - hws - kMB;
     oBc = vc - hws - kMB;
     wave_sync_lds();
-    load_patch<TI, PW>(S.B, QB, L.B, L.pitchB, L.nr, L.nc, oBr, oBc, lane);
+    load_patch<TI, PW, GL, LD::QB>(S.B, QB, L.B, L.pitchB, L.nr, L.nc, oBr, oBc, lane);
     wave_sync_lds();
   };
 
@@ -368,14 +425,14 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
     ensure_covered();
     float b0 = 0.f, b1 = 0.f;
 #pragma unroll
-    for (int k = 0; k < NPL; k++)
+    for (int k = 0; k < NS; k++)
       if (sel[k]) {
         const float dt = sub_rn(as[k], sampleB(k));
         b0 += gs[k].x * dt;
         b1 += gs[k].y * dt;
       }
-    b0 = wave_sum(b0);
-    b1 = wave_sum(b1);
+    b0 = group_sum<GL>(b0);
+    b1 = group_sum<GL>(b1);
     nk0 = add_rn(mul_rn(I00, b0), mul_rn(I01, b1));
     nk1 = add_rn(mul_rn(I10, b0), mul_rn(I11, b1));
     v.x = add_rn(v.x, nk0);
@@ -389,19 +446,19 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
   ensure_covered();
   float e = 0.f;
 #pragma unroll
-  for (int k = 0; k < NPL; k++)
+  for (int k = 0; k < NS; k++)
     if (W.in[k]) {
       const int idx = (W.r[k] + hws) * hws + (W.c[k] + hws);
       e += fabsf(sub_rn(S.tpl[idx], sampleB(k)));
     }
-  const float err = wave_sum(e);
+  const float err = group_sum<GL>(e);
   match_out o;
   o.tr = make_float2(sub_rn(v.x, p.x), sub_rn(v.y, p.y));
   o.dist = err / (nsel + (float)n);
   return o;
 }
 
-// One particle per wave.  The particle at scale s and its synthetic age-1
+// One particle per group of lanes.  The particle at scale s and its synthetic age-1
 // ancestors at p/2, p/4, ... are solved in one loop from the coarsest scale
 // down (recursive_prediction / get_flow_at): an ancestor whose prediction
 // leaves domain - border(3) contributes the identity, so does a match ending
@@ -412,12 +469,14 @@ template <typename TI, int NPL, bool SEL>
 // grid covers the capacity, so the hardware's workgroup dispatch balances the
 // particles' very different costs and workgroups past the count exit at once.
 __device__ inline void lk_body(const lk_args& a, int bid, int ppw) {
-  __shared__ lk_lds<NPL> lds_all[4];
+  constexpr int GL = lk_cfg<NPL>::GL, GPW = lk_cfg<NPL>::GPW;
+  __shared__ lk_lds<NPL> lds_all[4 * GPW];
   __shared__ lk_level_view lvs[kMaxLevels];
   __shared__ uint32_t blk_live, blk_acc;
   const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  lk_lds<NPL>& S = lds_all[wave];
+  const int grp = (threadIdx.x & 63) / GL;
+  const int lane = threadIdx.x & (GL - 1);
+  lk_lds<NPL>& S = lds_all[wave * GPW + grp];
   // Every argument is read once, before this kernel stores anything, so the
   // reads are scalar and the values stay in registers (or LDS for the views)
   // whether `a` is a kernel argument or an entry of a job table.
@@ -444,7 +503,8 @@ __device__ inline void lk_body(const lk_args& a, int bid, int ppw) {
   }
   const lane_win<NPL> W = make_lane_win<NPL>(P.ws, lane);
   __syncthreads();
-  for (uint32_t i = first + wave; i < last; i += 4) {
+  // the workgroup's 4 * GPW groups take consecutive particles
+  for (uint32_t i = first + (uint32_t)(wave * GPW + grp); i < last; i += 4 * GPW) {
     cuimg_particle_f part = parts[i];
     int accept = 0;
     float2 tr = make_float2(0.f, 0.f);
