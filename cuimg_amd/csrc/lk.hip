@@ -9,7 +9,7 @@
 namespace cuimg {
 
 template <typename TI, int NPL, bool SEL>
-__global__ __launch_bounds__(256) void lk_kernel(lk_args a, int ppw) {
+__global__ __launch_bounds__(kLkThreads) __attribute__((amdgpu_waves_per_eu(4))) void lk_kernel(lk_args a, int ppw) {
   lk_body<TI, NPL, SEL>(a, blockIdx.x, ppw);
 }
 
@@ -67,11 +67,11 @@ extern "C" int cuimg_hip_lk_track(cuimg_hip_ctx* ctx, const cuimg_img* prev, con
     rc = set_error(ctx, CUIMG_E_NOMEM, "lk_track: counter upload failed");
   if (!rc) {
     const int ppw = 8;
-    const int blocks = (int)((n + 4 * ppw - 1) / (4 * ppw));
+    const int blocks = (int)((n + kLkWaves * ppw - 1) / (kLkWaves * ppw));
     rc = launch(ctx, "lk_track", [&] {
       lk_variant(a.P.ws, a.P.top_n, dt == CUIMG_U8, [&](auto ti, auto npl, auto sel) {
         using TI = typename decltype(ti)::type;
-        lk_kernel<TI, decltype(npl)::value, decltype(sel)::value><<<blocks, 256, 0, ctx->stream>>>(a, ppw);
+        lk_kernel<TI, decltype(npl)::value, decltype(sel)::value><<<blocks, kLkThreads, 0, ctx->stream>>>(a, ppw);
       });
     });
   }

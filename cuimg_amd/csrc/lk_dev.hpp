@@ -149,7 +149,16 @@ struct match_out {
   float dist;
 };
 
-constexpr int kMB = 4;  // motion margin of the current-frame patch
+constexpr int kMB = 3;  // motion margin of the current-frame patch
+
+// Rows of a patch of q rows with stride ps loaded by gl lanes (whole passes).
+constexpr int patch_rows(int q, int ps, int gl) {
+  return gl % ps == 0 ? (q + gl / ps - 1) / (gl / ps) * (gl / ps) : q;
+}
+
+// Waves per matcher workgroup.
+constexpr int kLkWaves = 4;
+constexpr int kLkThreads = 64 * kLkWaves;
 
 // Window class NPL (1, 2, 4: windows up to 7x7, 11x11, 15x15) and its group
 // layout: GL lanes per particle, NS window pixels per lane.
@@ -162,50 +171,55 @@ struct lk_cfg {
 };
 
 // LDS of one group, sized for the largest window of its class.  The
-// previous- and current-frame patches have a row stride of PW (16 or 32
-// floats) so a group loads GL/PW whole rows per pass with constant offsets.
-template <int NPL>
+// current-frame patch has a row stride of PW (16 or 32 floats) so a group
+// loads GL/PW whole rows per pass with constant offsets; the previous-frame
+// patch of a 16-lane group is stored densely (stride QA, one row per pass).
+template <int NPL, bool SEL>
 struct lk_lds {
   using C = lk_cfg<NPL>;
   static constexpr int WSMAX = C::WSMAX;
   static constexpr int QA = WSMAX + 5;            // previous-frame patch side
   static constexpr int QG = WSMAX + 3;            // gradient patch side
   static constexpr int QB = WSMAX + 1 + 2 * kMB;  // current-frame patch side
-  static constexpr int PW = QB <= 16 ? 16 : 32;   // patch row stride (>= QA, QB)
-  static constexpr int RPP = C::GL / PW;          // rows per load pass
-  static constexpr int AROWS = (QA + RPP - 1) / RPP * RPP;
-  static constexpr int BROWS = (QB + RPP - 1) / RPP * RPP;
+  static constexpr int PW = QB <= 16 ? 16 : 32;   // current-frame patch row stride (>= QB)
+  static constexpr int PWA = C::GL == 16 ? QA : PW;  // previous-frame patch row stride (>= QA)
   static constexpr int GPASS = (QG * QG + C::GL - 1) / C::GL;
   static constexpr int FPASS = ((WSMAX + 1) * (WSMAX + 1) + C::GL - 1) / C::GL;
   float2 G[QG * QG];
-  float A[AROWS * PW];
-  float B[BROWS * PW];
-  float tpl[C::GL * C::NS];
-  float mag[C::GL * C::NS];
+  float A[patch_rows(QA, PWA, C::GL) * PWA];
+  float B[patch_rows(QB, PW, C::GL) * PW];
+  // template samples: the error term reads index (r+h)*h + (c+h), h = ws/2,
+  // at most (ws-1)*h + ws-1, so only that prefix is kept
+  static constexpr int TPL = (WSMAX - 1) * (WSMAX / 2) + WSMAX;
+  float tpl[TPL];
+  float mag[SEL ? C::GL * C::NS : 1];  // |gradient| ranks, top_n selection only
 };
 
-// dst[i*PW + j] = img[clamp(r0 + i), clamp(c0 + j)] for i, j < q <= MAXQ:
+// dst[i*PS + j] = img[clamp(r0 + i), clamp(c0 + j)] for i, j < q <= MAXQ:
 // indexing the patch with an unclamped coordinate reproduces the reference's
-// clamped read.  Group lane l loads column l % PW of rows l / PW + m * (GL / PW);
-// all of a lane's loads are issued before its first LDS write, so a patch
-// costs one memory round trip.
-template <typename TI, int PW, int GL, int MAXQ>
+// clamped read.  With PS dividing GL, group lane l loads column l % PS of rows
+// l / PS + m * (GL / PS); otherwise lane l loads column l of row m.  All of a
+// lane's loads are issued before its first LDS write, so a patch costs one
+// memory round trip.
+template <typename TI, int PS, int GL, int MAXQ>
 __device__ __forceinline__ void load_patch(float* dst, int q, const void* img, int pitch, int nr, int nc, int r0,
                                            int c0, int lane) {
-  constexpr int RPP = GL / PW;
+  constexpr bool WHOLE = GL % PS == 0;
+  constexpr int RPP = WHOLE ? GL / PS : 1;
   constexpr int MP = (MAXQ + RPP - 1) / RPP;
-  const int col = lane % PW, row = lane / PW;
+  const int col = WHOLE ? lane % PS : lane, row = WHOLE ? lane / PS : 0;
+  const bool on = col < q && col < PS;
   const int c = clampi(c0 + col, 0, nc - 1);
   float v[MP];
 #pragma unroll
   for (int m = 0; m < MP; m++) {
     const int i = row + m * RPP;
-    if (col < q && i < q) v[m] = ld<TI>(img, pitch, clampi(r0 + i, 0, nr - 1), c);
+    if (on && i < q) v[m] = ld<TI>(img, pitch, clampi(r0 + i, 0, nr - 1), c);
   }
 #pragma unroll
   for (int m = 0; m < MP; m++) {
     const int i = row + m * RPP;
-    if (col < q && i < q) dst[lane + GL * m] = v[m];
+    if (on && i < q) dst[i * PS + col] = v[m];
   }
 }
 
@@ -225,8 +239,8 @@ struct lane_win {
   int r[NS], c[NS];
   float fr[NS], fc[NS];
   bool in[NS];
-  int g[lk_lds<NPL>::GPASS];   // whole gradient patch
-  int gf[lk_lds<NPL>::FPASS];  // its (ws+1)^2 core
+  int g[lk_lds<NPL, false>::GPASS];   // whole gradient patch
+  int gf[lk_lds<NPL, false>::FPASS];  // its (ws+1)^2 core
 };
 
 template <int NPL>
@@ -246,7 +260,7 @@ __device__ __forceinline__ lane_win<NPL> make_lane_win(int ws, int lane) {
     W.fc[k] = (float)W.c[k];
   }
 #pragma unroll
-  for (int m = 0; m < lk_lds<NPL>::GPASS; m++) {
+  for (int m = 0; m < lk_lds<NPL, false>::GPASS; m++) {
     const int k = lane + GL * m;
     const int i = divq(k, iqg);
     W.g[m] = k < QG * QG ? (i << 8) | (k - i * QG) : -1;
@@ -254,7 +268,7 @@ __device__ __forceinline__ lane_win<NPL> make_lane_win(int ws, int lane) {
   const int QF = ws + 1;
   const float iqf = 1.f / (float)QF;
 #pragma unroll
-  for (int m = 0; m < lk_lds<NPL>::FPASS; m++) {
+  for (int m = 0; m < lk_lds<NPL, false>::FPASS; m++) {
     const int k = lane + GL * m;
     const int i = divq(k, iqf);
     W.gf[m] = k < QF * QF ? ((i + 1) << 8) | (k - i * QF + 1) : -1;
@@ -265,9 +279,10 @@ __device__ __forceinline__ lane_win<NPL> make_lane_win(int ws, int lane) {
 // iterative_lk_match_5x5::match for one group of lanes.
 template <typename TI, int NPL, bool SEL>
 __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk_params_dev& P,
-                                                const lane_win<NPL>& W, float2 p, float2 pred_tr, lk_lds<NPL>& S) {
-  using LD = lk_lds<NPL>;
-  constexpr int PW = LD::PW;
+                                                const lane_win<NPL>& W, float2 p, float2 pred_tr,
+                                                lk_lds<NPL, SEL>& S) {
+  using LD = lk_lds<NPL, SEL>;
+  constexpr int PW = LD::PW, PWA = LD::PWA;
   const match_out fail{make_float2(-1.f, -1.f), 1000.f};
   constexpr int GL = lk_cfg<NPL>::GL, NS = lk_cfg<NPL>::NS;
   const int lane = threadIdx.x & (GL - 1);
@@ -280,7 +295,7 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
   const int oAr = tz(p.x) - hws - 2, oAc = tz(p.y) - hws - 2;
   int oBr = tz(v.x) - hws - kMB, oBc = tz(v.y) - hws - kMB;
   wave_sync_lds();
-  load_patch<TI, PW, GL, LD::QA>(S.A, QA, L.A, L.pitchA, L.nr, L.nc, oAr, oAc, lane);
+  load_patch<TI, PWA, GL, LD::QA>(S.A, QA, L.A, L.pitchA, L.nr, L.nc, oAr, oAc, lane);
   load_patch<TI, PW, GL, LD::QB>(S.B, QB, L.B, L.pitchB, L.nr, L.nc, oBr, oBc, lane);
   wave_sync_lds();
 
@@ -302,10 +317,10 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
     const int yr = oAr + 1 + i, yc = oAc + 1 + j;
     float2 g = make_float2(0.f, 0.f);
     if (yr >= 1 && yr <= L.nr - 2 && yc >= 1 && yc <= L.nc - 2) {
-      const float* a = &S.A[(i + 1) * PW + (j + 1)];
-      const float pmm = a[-PW - 1], pm0 = a[-PW], pmp = a[-PW + 1];
+      const float* a = &S.A[(i + 1) * PWA + (j + 1)];
+      const float pmm = a[-PWA - 1], pm0 = a[-PWA], pmp = a[-PWA + 1];
       const float p0m = a[-1], p0p = a[1];
-      const float ppm = a[PW - 1], pp0 = a[PW], ppp = a[PW + 1];
+      const float ppm = a[PWA - 1], pp0 = a[PWA], ppp = a[PWA + 1];
       float gr = add_rn(ppm, mul_rn(2.f, pp0));
       gr = add_rn(gr, ppp);
       gr = sub_rn(gr, pmm);
@@ -345,15 +360,15 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
       if (w.xr >= 0 && w.xr < L.nr && w.xc >= 0 && w.xc < L.nc) {
         sel[k] = true;
         const int ar = w.xr - oAr, ac = w.xc - oAc;  // in [1, ws+2]
-        const float* a = &S.A[ar * PW + ac];
-        const float av = blend(w, a[0], a[PW], a[1], a[PW + 1]);
+        const float* a = &S.A[ar * PWA + ac];
+        const float av = blend(w, a[0], a[PWA], a[1], a[PWA + 1]);
         as[k] = P.template_u8 ? (float)(uint8_t)(int)av : av;  // float -> uint8 through int
         const float2* g = &S.G[(ar - 1) * QG + (ac - 1)];
         const float2 g00 = g[0], g10 = g[QG], g01 = g[1], g11 = g[QG + 1];
         gs[k] = make_float2(blend(w, g00.x, g10.x, g01.x, g11.x), blend(w, g00.y, g10.y, g01.y, g11.y));
       }
-      S.tpl[i] = as[k];
-      if (SEL) S.mag[i] = sel[k] ? norml2(gs[k].x, gs[k].y) : -1.f;
+      if (i < LD::TPL) S.tpl[i] = as[k];
+      if constexpr (SEL) S.mag[i] = sel[k] ? norml2(gs[k].x, gs[k].y) : -1.f;
     }
   }
   wave_sync_lds();
@@ -465,18 +480,19 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
 // outside its level's domain, and a failed match (-1,-1) is kept; without a
 // coarser scale the prediction is the speed (zero at age 1).
 template <typename TI, int NPL, bool SEL>
-// Workgroup `bid` matches particles [bid * 4 * ppw, (bid + 1) * 4 * ppw); the
+// Workgroup `bid` matches particles [bid * W * ppw, (bid + 1) * W * ppw), W =
+// kLkWaves; the
 // grid covers the capacity, so the hardware's workgroup dispatch balances the
 // particles' very different costs and workgroups past the count exit at once.
 __device__ inline void lk_body(const lk_args& a, int bid, int ppw) {
   constexpr int GL = lk_cfg<NPL>::GL, GPW = lk_cfg<NPL>::GPW;
-  __shared__ lk_lds<NPL> lds_all[4 * GPW];
+  __shared__ lk_lds<NPL, SEL> lds_all[kLkWaves * GPW];
   __shared__ lk_level_view lvs[kMaxLevels];
   __shared__ uint32_t blk_live, blk_acc;
   const int wave = threadIdx.x >> 6;
   const int grp = (threadIdx.x & 63) / GL;
   const int lane = threadIdx.x & (GL - 1);
-  lk_lds<NPL>& S = lds_all[wave * GPW + grp];
+  lk_lds<NPL, SEL>& S = lds_all[wave * GPW + grp];
   // Every argument is read once, before this kernel stores anything, so the
   // reads are scalar and the values stay in registers (or LDS for the views)
   // whether `a` is a kernel argument or an entry of a job table.
@@ -491,10 +507,10 @@ __device__ inline void lk_body(const lk_args& a, int bid, int ppw) {
   float2* const dec_tr = uni(a.dec_tr);
   float* const dec_dist = uni(a.dec_dist);
   const uint32_t n = uni(cnt->n);
-  const uint32_t first = (uint32_t)bid * 4u * (uint32_t)ppw;
+  const uint32_t first = (uint32_t)bid * (uint32_t)kLkWaves * (uint32_t)ppw;
   if (bid == 0 && threadIdx.x == 0) cnt->n_matched = n;
   if (first >= n) return;
-  const uint32_t last = min(n, first + 4u * (uint32_t)ppw);
+  const uint32_t last = min(n, first + (uint32_t)kLkWaves * (uint32_t)ppw);
   if (threadIdx.x < kMaxLevels * (int)(sizeof(lk_level_view) / 4))
     reinterpret_cast<int*>(lvs)[threadIdx.x] = reinterpret_cast<const int*>(a.lv)[threadIdx.x];
   if (threadIdx.x == 0) {
@@ -503,8 +519,8 @@ __device__ inline void lk_body(const lk_args& a, int bid, int ppw) {
   }
   const lane_win<NPL> W = make_lane_win<NPL>(P.ws, lane);
   __syncthreads();
-  // the workgroup's 4 * GPW groups take consecutive particles
-  for (uint32_t i = first + (uint32_t)(wave * GPW + grp); i < last; i += 4 * GPW) {
+  // the workgroup's kLkWaves * GPW groups take consecutive particles
+  for (uint32_t i = first + (uint32_t)(wave * GPW + grp); i < last; i += kLkWaves * GPW) {
     cuimg_particle_f part = parts[i];
     int accept = 0;
     float2 tr = make_float2(0.f, 0.f);
@@ -587,10 +603,10 @@ template <typename F>
 inline void lk_variant(int ws, int top_n, int image_u8, F&& f) {
   const bool sel = top_n < ws * ws;
   auto by_npl = [&](auto ti) {
-    if (ws <= lk_lds<1>::WSMAX) {
+    if (ws <= lk_cfg<1>::WSMAX) {
       if (sel) f(ti, std::integral_constant<int, 1>{}, std::true_type{});
       else f(ti, std::integral_constant<int, 1>{}, std::false_type{});
-    } else if (ws <= lk_lds<2>::WSMAX) {
+    } else if (ws <= lk_cfg<2>::WSMAX) {
       if (sel) f(ti, std::integral_constant<int, 2>{}, std::true_type{});
       else f(ti, std::integral_constant<int, 2>{}, std::false_type{});
     } else {

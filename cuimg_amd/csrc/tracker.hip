@@ -221,7 +221,7 @@ __global__ __launch_bounds__(256) void step_fill_kernel(const level_job* jobs, c
 }
 
 template <typename TI, int NPL, bool SEL>
-__global__ __launch_bounds__(256) void step_lk_kernel(const level_job* jobs, const uint32_t* prefix, int njobs,
+__global__ __launch_bounds__(kLkThreads) __attribute__((amdgpu_waves_per_eu(4))) void step_lk_kernel(const level_job* jobs, const uint32_t* prefix, int njobs,
                                                       int ppw) {
   CUIMG_JOB(prefix, njobs);
   const step_flags f = flags_of(J);
@@ -558,9 +558,9 @@ static int build_maps(cuimg_hip_batch* b) {
     cnt[K_SCATI].push_back((uint32_t)(R.nblocks + ceil_div(R.nbr * b->P[J.stream].box_size, 4)));
     cnt[K_MSCAN].push_back((uint32_t)std::max<long>(1, std::min<long>(512, (cap + 255) / 256)));
   }
-  // Matcher: 4 * ppw particles of capacity per workgroup (see lk_body).
+  // Matcher: kLkWaves * ppw particles of capacity per workgroup (see lk_body).
   for (int j = 0; j < b->njobs; j++)
-    cnt[K_LK].push_back((uint32_t)ceil_div((int)b->lv[j].cap, 4 * b->lk_ppw));
+    cnt[K_LK].push_back((uint32_t)ceil_div((int)b->lv[j].cap, kLkWaves * b->lk_ppw));
   for (int k = 0; k < K_NKINDS; k++) {
     std::vector<uint32_t>& p = b->prefix_h[k];
     p.assign(cnt[k].size() + 1, 0);
@@ -679,7 +679,7 @@ static int enqueue_step(cuimg_hip_batch* b) {
   CUIMG_TRY(launch(ctx, "lk_track", [&] {
     lk_variant(b->P0.lk_ws, b->P0.lk_top_n, b->P0.profile == 0, [&](auto ti, auto npl, auto sel) {
       using TI = typename decltype(ti)::type;
-      step_lk_kernel<TI, decltype(npl)::value, decltype(sel)::value><<<b->total[K_LK], 256, 0, st>>>(J, pf[K_LK], nj,
+      step_lk_kernel<TI, decltype(npl)::value, decltype(sel)::value><<<b->total[K_LK], kLkThreads, 0, st>>>(J, pf[K_LK], nj,
                                                                                                 b->lk_ppw);
     });
   }));
