@@ -247,9 +247,20 @@ def gaussian_microbench(cu, ctx, reps=40):
     """Separable Gaussian on f32 4096x4096 and f32x4 3840x2160 (sigma 1, 2, 4;
     2*ceil(3 sigma)+1 taps), 4 distinct buffer pairs rotated so the working set
     (>= 512 MiB) exceeds the 256 MB Infinity Cache; and the 5-level 2x pyramid
-    of the 4096x4096 image."""
+    of the 4096x4096 image.  First, a 256 MiB device-to-device copy (the
+    runtime's copy kernel) over 4 rotated buffer pairs: the achievable
+    read+write rate, and the reference reading for the HBM traffic counters."""
     rng = np.random.default_rng(3)
     out = []
+    nb = 256 << 20
+    pairs = [(cu.DeviceBuffer(ctx, nb), cu.DeviceBuffer(ctx, nb)) for _ in range(4)]
+    us = span_us(ctx, lambda i: pairs[i % 4][1].copy_from(pairs[i % 4][0]), reps)
+    gbs = 2 * nb / us / 1e3
+    out.append({"kernel": "copy_d2d", "image": "256 MiB", "avg_us": round(us, 2), "bytes": 2 * nb,
+                "GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
+    for a, b in pairs:
+        a.free()
+        b.free()
     for kind, shape in (("f32", (4096, 4096)), ("f32x4", (2160, 3840))):
         if kind == "f32":
             img = rng.uniform(0, 255, shape).astype(np.float32)
@@ -312,6 +323,25 @@ def cpu_baseline(video, budget_s, fast_th):
                       f"({spent:.1f} s, {updates} particle updates, {frames / spent:.2f} frames/s)"}
 
 
+# HBM traffic of the matcher launch, measured by rocprofv3 counter runs of
+# this benchmark's workload (FETCH_SIZE, WRITE_SIZE, scaled by the readings of
+# the 256 MiB copy of the microbenchmark); the summary names the command.
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "lk_track_1080p_th8_hbm_traffic.md")
+
+
+def measured_traffic(path=TRAFFIC_PROFILE):
+    """Bytes per launch from the `traffic_bytes_per_launch = N` line of the
+    counter summary, None when there is none."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch"):
+                    return int(line.split("=")[1].strip())
+    except OSError:
+        pass
+    return None
+
+
 def lk_roofline(tr):
     """Roofline entry of the matcher: algorithmic bytes per launch (one launch
     per frame step covers every scale) over its average measured duration."""
@@ -322,10 +352,10 @@ def lk_roofline(tr):
     bytes_per_launch = lk_algorithmic_bytes(tr["shapes"], avg_parts)
     achieved = bytes_per_launch / (k["avg_us"] * 1e-6) / 1e9
     return {"kernel": "lk_track", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(),
             "bytes_per_launch": int(bytes_per_launch), "avg_us": k["avg_us"],
-            "note": "matcher is gather/latency bound; bytes = both FP32 frames of every scale once "
-                    "+ 40 B per particle matched"}
+            "note": "the matcher is bound by instruction latency, not HBM: bytes = both FP32 frames of every "
+                    "scale once + 40 B per particle matched; traffic from rocprofv3 FETCH_SIZE + WRITE_SIZE"}
 
 
 def main():
