@@ -230,6 +230,42 @@ def batch_bench(cu, ctx):
     return out
 
 
+ADJ_SEED, ADJ_NR, ADJ_NC, ADJ_N = 0xC0FFEE05, 720, 1280, 50000
+
+
+def adjoint_bench(cu, ctx, reps=20):
+    """Loss + gradients (dL/dd and dL/dI1) of the blurred photometric LK loss:
+    1280x720 FP32 frames of the synthetic video, 50k particles uniform over
+    the interior, d ~ U[-2, 2]^2, sigma 1 / 7 taps, 7x7 windows; device
+    buffers allocated once, span-timed back-to-back calls."""
+    import ctypes
+    rng = np.random.default_rng(ADJ_SEED)
+    v = cu.synth_video(ADJ_SEED, ADJ_NR, ADJ_NC, 2, 1).astype(np.float32)
+    I0, I1 = ctx.upload(v[0]), ctx.upload(v[1])
+    pos = np.stack([rng.uniform(4, ADJ_NR - 5, ADJ_N), rng.uniform(4, ADJ_NC - 5, ADJ_N)], 1).astype(np.float32)
+    d = rng.uniform(-2, 2, (ADJ_N, 2)).astype(np.float32)
+    bp, bd = cu.DeviceBuffer.from_array(ctx, pos), cu.DeviceBuffer.from_array(ctx, d)
+    tot, lp, gd = cu.DeviceBuffer(ctx, 8), cu.DeviceBuffer(ctx, 4 * ADJ_N), cu.DeviceBuffer(ctx, 8 * ADJ_N)
+    scratch = cu.DeviceBuffer(ctx, 8 * ADJ_NR * ADJ_NC)
+    gI = ctx.image(ADJ_NR, ADJ_NC, cu.F32)
+    L = ctx.L
+
+    def call(_):
+        ctx.check(L.cuimg_hip_lk_loss_adjoint(ctx.h, ctypes.byref(I0.d), ctypes.byref(I1.d), bp.p, bd.p, ADJ_N, 1, 3,
+                                              7, tot.p, lp.p, gd.p, ctypes.byref(gI.d), scratch.p,
+                                              2 * ADJ_NR * ADJ_NC), "lk_loss_adjoint")
+
+    us = span_us(ctx, call, reps)
+    out = {"workload": f"{ADJ_NC}x{ADJ_NR} f32, {ADJ_N} particles, d~U[-2,2]^2, sigma 1 / 7 taps, 7x7 windows, "
+                       "loss + dL/dd + dL/dI1", "avg_us": round(us, 2),
+           "particles_per_s": round(ADJ_N / (us * 1e-6), 1)}
+    for b in (bp, bd, tot, lp, gd, scratch):
+        b.free()
+    for im in (I0, I1, gI):
+        im.free()
+    return out
+
+
 def span_us(ctx, launch, reps):
     """Average microseconds per launch of `reps` back-to-back launches, timed by
     two events on the context stream (after one untimed pass over the 4 buffer
@@ -399,6 +435,9 @@ def main():
         log(f"  batch {batch['batch_ms_per_step']} ms/step vs sequential {batch['sequential_ms_per_step']} ms/step "
             f"({batch['speedup']}x), identical {batch['streams_identical_to_own_tracker']}")
     micro = [] if args.no_micro else gaussian_microbench(cu, ctx)
+    adjoint = None if args.no_micro else adjoint_bench(cu, ctx)
+    if adjoint:
+        log(f"adjoint: {adjoint['avg_us']} us per call ({adjoint['particles_per_s']:.4g} particles/s)")
     ctx.close()
 
     cpu = None
@@ -432,6 +471,7 @@ def main():
         "fast_th_20": sparse,
         "batch_64_streams": batch,
         "gaussian_pyramid_roofline": micro,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
