@@ -10,7 +10,7 @@ namespace cuimg {
 
 template <typename TI, int NPL, bool SEL>
 __global__ __launch_bounds__(kLkThreads) __attribute__((amdgpu_waves_per_eu(4))) void lk_kernel(lk_args a, int ppw) {
-  lk_body<TI, NPL, SEL>(a, blockIdx.x, ppw);
+  lk_body<TI, NPL, SEL>(a, blockIdx.x, gridDim.x, ppw);
 }
 
 }  // namespace cuimg

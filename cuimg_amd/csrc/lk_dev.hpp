@@ -158,6 +158,11 @@ constexpr int patch_rows(int q, int ps, int gl) {
 
 // Waves per matcher workgroup.
 constexpr int kLkWaves = 4;
+// Matcher workgroups per job of a frame step: the grid cannot follow the
+// device-side particle count, and every workgroup without particles still
+// holds its LDS while it reads the count, so the grid stops well short of the
+// capacity.
+constexpr int kLkMaxWgs = 1024;
 constexpr int kLkThreads = 64 * kLkWaves;
 
 // Window class NPL (1, 2, 4: windows up to 7x7, 11x11, 15x15) and its group
@@ -484,7 +489,7 @@ template <typename TI, int NPL, bool SEL>
 // kLkWaves; the
 // grid covers the capacity, so the hardware's workgroup dispatch balances the
 // particles' very different costs and workgroups past the count exit at once.
-__device__ inline void lk_body(const lk_args& a, int bid, int ppw) {
+__device__ inline void lk_body(const lk_args& a, int bid, int nbid, int ppw) {
   constexpr int GL = lk_cfg<NPL>::GL, GPW = lk_cfg<NPL>::GPW;
   __shared__ lk_lds<NPL, SEL> lds_all[kLkWaves * GPW];
   __shared__ lk_level_view lvs[kMaxLevels];
@@ -507,10 +512,10 @@ __device__ inline void lk_body(const lk_args& a, int bid, int ppw) {
   float2* const dec_tr = uni(a.dec_tr);
   float* const dec_dist = uni(a.dec_dist);
   const uint32_t n = uni(cnt->n);
-  const uint32_t first = (uint32_t)bid * (uint32_t)kLkWaves * (uint32_t)ppw;
+  const uint32_t per_wg = (uint32_t)kLkWaves * (uint32_t)ppw;
+  const uint32_t nchunks = (n + per_wg - 1) / per_wg;
   if (bid == 0 && threadIdx.x == 0) cnt->n_matched = n;
-  if (first >= n) return;
-  const uint32_t last = min(n, first + (uint32_t)kLkWaves * (uint32_t)ppw);
+  if ((uint32_t)bid >= nchunks) return;
   if (threadIdx.x < kMaxLevels * (int)(sizeof(lk_level_view) / 4))
     reinterpret_cast<int*>(lvs)[threadIdx.x] = reinterpret_cast<const int*>(a.lv)[threadIdx.x];
   if (threadIdx.x == 0) {
@@ -519,7 +524,10 @@ __device__ inline void lk_body(const lk_args& a, int bid, int ppw) {
   }
   const lane_win<NPL> W = make_lane_win<NPL>(P.ws, lane);
   __syncthreads();
-  // the workgroup's kLkWaves * GPW groups take consecutive particles
+  // chunks bid, bid + nbid, ... of per_wg consecutive particles; the
+  // workgroup's kLkWaves * GPW groups take consecutive particles of a chunk
+  for (uint32_t chunk = (uint32_t)bid; chunk < nchunks; chunk += (uint32_t)nbid) {
+  const uint32_t first = chunk * per_wg, last = min(n, first + per_wg);
   for (uint32_t i = first + (uint32_t)(wave * GPW + grp); i < last; i += kLkWaves * GPW) {
     cuimg_particle_f part = parts[i];
     int accept = 0;
@@ -584,6 +592,7 @@ __device__ inline void lk_body(const lk_args& a, int bid, int ppw) {
       }
       parts[i] = part;
     }
+  }
   }
   __syncthreads();
   if (threadIdx.x == 0 && (blk_live | blk_acc)) {

@@ -226,7 +226,7 @@ __global__ __launch_bounds__(kLkThreads) __attribute__((amdgpu_waves_per_eu(4)))
   CUIMG_JOB(prefix, njobs);
   const step_flags f = flags_of(J);
   if (!f.match) return;
-  lk_body<TI, NPL, SEL>(J.lk[f.slot * 2 + f.cur], lb, ppw);
+  lk_body<TI, NPL, SEL>(J.lk[f.slot * 2 + f.cur], lb, nb, ppw);
 }
 
 __global__ __launch_bounds__(256) void step_pmask_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
@@ -558,9 +558,10 @@ static int build_maps(cuimg_hip_batch* b) {
     cnt[K_SCATI].push_back((uint32_t)(R.nblocks + ceil_div(R.nbr * b->P[J.stream].box_size, 4)));
     cnt[K_MSCAN].push_back((uint32_t)std::max<long>(1, std::min<long>(512, (cap + 255) / 256)));
   }
-  // Matcher: kLkWaves * ppw particles of capacity per workgroup (see lk_body).
+  // Matcher: one workgroup per kLkWaves * ppw particles of capacity, at most
+  // kLkMaxWgs per job (the workgroups loop over the chunks, see lk_body).
   for (int j = 0; j < b->njobs; j++)
-    cnt[K_LK].push_back((uint32_t)ceil_div((int)b->lv[j].cap, kLkWaves * b->lk_ppw));
+    cnt[K_LK].push_back((uint32_t)std::min(ceil_div((int)b->lv[j].cap, kLkWaves * b->lk_ppw), kLkMaxWgs));
   for (int k = 0; k < K_NKINDS; k++) {
     std::vector<uint32_t>& p = b->prefix_h[k];
     p.assign(cnt[k].size() + 1, 0);
