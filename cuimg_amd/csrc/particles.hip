@@ -11,7 +11,8 @@ __global__ void fill_u32_kernel(uint32_t* begin, int pitch, int r0, int r1, int 
 int launch_fill_u32(cuimg_hip_ctx* ctx, uint32_t* begin, int pitch, int r0, int r1, int c0, int c1, uint32_t v) {
   const long total = (long)(r1 - r0) * (c1 - c0);
   if (total <= 0) return 0;
-  const int blocks = (int)std::min<long>((total + 255) / 256, 8192);
+  // one workgroup per row, or (a single row) one per 1024 16-byte stores
+  const int blocks = r1 - r0 > 1 ? std::min(r1 - r0, 8192) : (int)std::min<long>((total + 4095) / 4096, 8192);
   return launch(ctx, "fill_u32", [&] {
     fill_u32_kernel<<<blocks, 256, 0, ctx->stream>>>(begin, pitch, r0, r1, c0, c1, v);
   });

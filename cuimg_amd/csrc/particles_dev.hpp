@@ -73,26 +73,45 @@ __device__ inline uint32_t block_excl_scan(uint32_t x, uint32_t* tmp, uint32_t* 
 }
 
 // ------------------------------------------------------------------ fills
-__device__ inline void fill_u32_body(uint32_t* begin, int pitch, int r0, int r1, int c0, int c1, uint32_t v,
-                                     int bid, int nb) {
-  const int w = c1 - c0;
-  const long total = (long)(r1 - r0) * w;
-  for (long i = bid * (long)blockDim.x + threadIdx.x; i < total; i += (long)nb * blockDim.x)
-    *px<uint32_t>(begin, pitch, r0 + (int)(i / w), c0 + (int)(i % w)) = v;
+// Fills one row segment [p, p + n) of 4-byte words with v: unaligned head and
+// tail words one by one, the aligned body with 16-byte stores; workgroup
+// `part` of `nparts` takes every nparts-th block of the body.
+__device__ __forceinline__ void fill_row_u32(uint32_t* p, int n, uint32_t v, int part, int nparts) {
+  const int head = min(n, (int)(((16 - ((uintptr_t)p & 15)) & 15) >> 2));
+  const int body = (n - head) >> 2;
+  const int tail = n - head - 4 * body;
+  if (part == 0 && (int)threadIdx.x < head) p[threadIdx.x] = v;
+  uint4* q = reinterpret_cast<uint4*>(p + head);
+  const uint4 v4 = make_uint4(v, v, v, v);
+  for (int j = part * blockDim.x + threadIdx.x; j < body; j += nparts * blockDim.x) q[j] = v4;
+  if (part == 0 && (int)threadIdx.x < tail) p[head + 4 * body + threadIdx.x] = v;
 }
 
-// u8 domain fill with 4-byte stores (rows are 4-byte aligned at column 0).
+// 32-bit fill of rows [r0, r1) x cols [c0, c1) (coordinates relative to
+// begin; negative ones reach into the border): one row per workgroup at a
+// time, or a single row split over all workgroups.
+__device__ inline void fill_u32_body(uint32_t* begin, int pitch, int r0, int r1, int c0, int c1, uint32_t v,
+                                     int bid, int nb) {
+  if (r1 - r0 == 1) {
+    fill_row_u32(px<uint32_t>(begin, pitch, r0, c0), c1 - c0, v, bid, nb);
+    return;
+  }
+  for (int r = r0 + bid; r < r1; r += nb) fill_row_u32(px<uint32_t>(begin, pitch, r, c0), c1 - c0, v, 0, 1);
+}
+
+// u8 domain fill (rows 4-byte aligned at column 0 or not): head bytes, then
+// 16-byte stores, then tail bytes, one row per workgroup at a time.
 __device__ inline void fill_u8_body(uint8_t* begin, int pitch, int nr, int nc, uint32_t v4, int bid, int nb) {
-  const int wpr = (nc + 3) / 4;
-  const long total = (long)nr * wpr;
-  for (long i = bid * (long)blockDim.x + threadIdx.x; i < total; i += (long)nb * blockDim.x) {
-    const int r = (int)(i / wpr), w = (int)(i % wpr);
-    uint8_t* p = px<uint8_t>(begin, pitch, r, 4 * w);
-    if (4 * w + 4 <= nc) {
-      *reinterpret_cast<uint32_t*>(p) = v4;
-    } else {
-      for (int k = 0; 4 * w + k < nc; k++) p[k] = (uint8_t)v4;
-    }
+  for (int r = bid; r < nr; r += nb) {
+    uint8_t* p = px<uint8_t>(begin, pitch, r, 0);
+    const int head = min(nc, (int)((16 - ((uintptr_t)p & 15)) & 15));
+    const int body = (nc - head) >> 4;
+    const int tail = nc - head - 16 * body;
+    if ((int)threadIdx.x < head) p[threadIdx.x] = (uint8_t)v4;
+    uint4* q = reinterpret_cast<uint4*>(p + head);
+    const uint4 w = make_uint4(v4, v4, v4, v4);
+    for (int j = threadIdx.x; j < body; j += blockDim.x) q[j] = w;
+    if ((int)threadIdx.x < tail) p[head + 16 * body + threadIdx.x] = (uint8_t)v4;
   }
 }
 

@@ -53,7 +53,8 @@ int pyramid_build(cuimg_hip_ctx* ctx, const cuimg_img* frame, const cuimg_img* l
                        L.nrows, L.ncols, levels[l - 1].nrows / 2, levels[l - 1].ncols / 2);
     a.lv[l] = pyr_level{L.begin, L.pitch, L.nrows, L.ncols, L.border};
   }
-  a.frame_vec_ok = aligned16(frame->begin, frame->pitch) && aligned16(levels[0].begin, levels[0].pitch);
+  a.frame_vec_ok = aligned16(frame->begin, frame->pitch);
+  a.l0_vec_ok = aligned16(levels[0].begin, levels[0].pitch);
   const int es = dt == CUIMG_U8 ? 1 : 4;
   a.stream_out = (size_t)levels[0].nrows * levels[0].ncols * es > kStreamOutBytes;
   const int blocks = pyr_tiling(a, levels[0].nrows, levels[0].ncols, es);

@@ -36,7 +36,8 @@ struct pyr_level {
 struct pyr_args {
   const void* frame;
   int frame_pitch;
-  int frame_vec_ok;   // frame and level 0 rows 16-byte aligned
+  int frame_vec_ok;   // frame rows 16-byte aligned
+  int l0_vec_ok;      // level-0 rows 16-byte aligned
   int copy_level0;    // frame is not level 0 itself
   int n;
   int tiles_x;
@@ -136,13 +137,22 @@ __device__ inline void pyramid_tile(const pyr_args& a, int tile) {
     }
     __builtin_amdgcn_s_waitcnt(0 | (7 << 4) | (15 << 8));  // vmcnt(0)
     if (a.copy_level0) {
+      const bool l0_vec = a.l0_vec_ok;
 #pragma unroll
       for (int k = 0; k < NPT; k++) {
         const int i = threadIdx.x + kPyrThreads * k;
         if (i < nchunks && r0 + (i >> cprl) < r1) {
           // own chunk: written by this lane's DMA, complete after the wait
           const uint4 v = *reinterpret_cast<const uint4*>(t0 + (i << CH_LOG2));
-          pyr_store16(px<T>(L0.begin, L0.pitch, r0 + (i >> cprl), c0 + ((i & ((1 << cprl) - 1)) << CH_LOG2)), v, nt);
+          T* dst = px<T>(L0.begin, L0.pitch, r0 + (i >> cprl), c0 + ((i & ((1 << cprl) - 1)) << CH_LOG2));
+          if (l0_vec) {
+            pyr_store16(dst, v, nt);
+          } else {
+            // level 0 with an odd border: element stores
+            const T* e = reinterpret_cast<const T*>(&v);
+#pragma unroll
+            for (int m = 0; m < CH; m++) pyr_store(dst + m, e[m], nt);
+          }
         }
       }
     }

@@ -444,8 +444,8 @@ static void build_job(cuimg_hip_batch* b, int i, int s) {
   J.sparse_pitch = R.sparse.pitch;
   J.sparse_border = R.sparse.border;
   J.fast_tiles_x = ceil_div(R.nc, kFTW);
-  const long sp_px = (long)(R.nr + 2 * R.sparse.border) * (R.nc + 2 * R.sparse.border);
-  J.fill_sparse_blocks = (int)std::max<long>(1, std::min<long>(1024, (sp_px + 1023) / 1024));
+  // fills: one workgroup per row (the fill bodies loop over rows)
+  J.fill_sparse_blocks = std::max(1, std::min(1024, R.nr + 2 * R.sparse.border));
   for (int slot = 0; slot < 2; slot++)
     for (int cur = 0; cur < 2; cur++) {
       lk_args& a = J.lk[slot * 2 + cur];
@@ -531,7 +531,8 @@ static void build_stream(cuimg_hip_batch* b, int i) {
       a.lv[s] = pyr_level{L.begin, L.pitch, L.nrows, L.ncols, L.border};
     }
     const cuimg_img& L0 = b->lv[S.first].u8[slot];
-    a.frame_vec_ok = ((uintptr_t)L0.begin % 16 == 0) && (L0.pitch % 16 == 0);
+    a.frame_vec_ok = 1;  // and the input table's own flag, at the step
+    a.l0_vec_ok = ((uintptr_t)L0.begin % 16 == 0) && (L0.pitch % 16 == 0);
     pyr_tiling(a, L0.nrows, L0.ncols, 1);
   }
 }
@@ -549,8 +550,7 @@ static int build_maps(cuimg_hip_batch* b) {
     const level_job& J = b->hj[j];
     const long cap = R.cap;
     cnt[K_BLUR].push_back(b->P0.profile == 1 ? (uint32_t)(ceil_div(R.nc, 256) * ceil_div(R.nr, th)) : 0u);
-    const long mask_words = (long)R.nr * ((R.nc + 3) / 4);
-    const int G = (int)std::max<long>(1, std::min<long>(1024, (mask_words + 1023) / 1024));
+    const int G = std::max(1, std::min(1024, R.nr));
     cnt[K_FILL].push_back((uint32_t)(J.fill_sparse_blocks + G));
     cnt[K_PMASK].push_back((uint32_t)std::max<long>(1, std::min<long>(512, (cap + 255) / 256)));
     cnt[K_FAST].push_back((uint32_t)(J.fast_tiles_x * ceil_div(R.nr, kFTH)));
