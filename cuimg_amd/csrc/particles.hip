@@ -44,11 +44,15 @@ int launch_merge_reset(cuimg_hip_ctx* ctx, const merge_args& a) {
 }
 
 __global__ __launch_bounds__(256) void merge_scan_kernel(merge_args a) { merge_scan_body(a, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(256) void merge_round1_kernel(merge_args a) {
+  merge_round1_body(a, blockIdx.x, gridDim.x);
+}
 __global__ __launch_bounds__(1024) void merge_resolve_kernel(merge_args a) { merge_resolve_body(a); }
 
 int launch_merge(cuimg_hip_ctx* ctx, const merge_args& a) {
   const int blocks = (int)std::min<uint32_t>((a.cap + 255) / 256, 2048);
   CUIMG_TRY(launch(ctx, "merge_scan", [&] { merge_scan_kernel<<<blocks, 256, 0, ctx->stream>>>(a); }));
+  CUIMG_TRY(launch(ctx, "merge_round1", [&] { merge_round1_kernel<<<blocks, 256, 0, ctx->stream>>>(a); }));
   return launch(ctx, "merge_resolve", [&] { merge_resolve_kernel<<<1, 1024, 0, ctx->stream>>>(a); });
 }
 

@@ -525,33 +525,44 @@ __device__ inline void replay_chunk(const merge_args& a, unsigned long long key)
   __threadfence();
 }
 
-// One workgroup: round 1 over all actors, later rounds over the open ones,
-// serial replay of the rest, restoration of the scratch.
+// Round 1 for every actor, over many workgroups (the decisions of one round
+// do not depend on each other's order): open actors are appended to the
+// pending list a.pend, counted in cnt->pend1.
+__device__ inline void merge_round1_body(const merge_args& a, int bid, int nb) {
+  const uint32_t m = a.cnt->actors;
+  const int lane = threadIdx.x & 63;
+  const uint32_t stride = nb * blockDim.x;
+  for (uint32_t k0 = bid * blockDim.x + (threadIdx.x & ~63u); k0 < m; k0 += stride) {
+    const uint32_t k = k0 + lane;
+    bool open = false;
+    unsigned long long key = 0;
+    if (k < m) {
+      const merge_actor A = a.actors[k];
+      open = !merge_decide(a, A, 1);
+      key = ((unsigned long long)(uint32_t)A.i << 32) | k;
+    }
+    const uint64_t bal = __ballot(open);
+    if (bal == 0) continue;
+    const int leader = __ffsll((unsigned long long)bal) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&a.cnt->pend1, (uint32_t)__popcll(bal));
+    base = __shfl(base, leader);
+    if (open) st64(&a.pend[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1))], key);
+  }
+}
+
+// One workgroup: later rounds over the actors round 1 left open, serial replay
+// of the rest, restoration of the scratch.
 __device__ inline void merge_resolve_body(const merge_args& a) {
   __shared__ unsigned long long keys_lds[kPendLds];
   __shared__ unsigned long long small[64];
-  __shared__ uint32_t npend, nopen[2], nsmall;
+  __shared__ uint32_t nopen[2], nsmall;
   const uint32_t m = a.cnt->actors;
-  if (threadIdx.x == 0) npend = 0;
-  __syncthreads();
-  for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
-    const merge_actor A = a.actors[k];
-    if (!merge_decide(a, A, 1)) {
-      const uint32_t slot = atomicAdd(&npend, 1u);
-      const unsigned long long key = ((unsigned long long)(uint32_t)A.i << 32) | k;
-      if (slot < kPendLds) keys_lds[slot] = key;
-      else st64(&a.pend[slot], key);
-    }
-  }
-  __threadfence();
-  __syncthreads();
-  const uint32_t np = npend;
+  const uint32_t np = ld32(&a.cnt->pend1);
   const bool in_lds = np <= kPendLds;
-  if (!in_lds) {
-    for (uint32_t k = threadIdx.x; k < kPendLds; k += blockDim.x) st64(&a.pend[k], keys_lds[k]);
-    __threadfence();
-    __syncthreads();
-  }
+  if (in_lds)
+    for (uint32_t k = threadIdx.x; k < np; k += blockDim.x) keys_lds[k] = ld64(&a.pend[k]);
+  __syncthreads();
   unsigned long long* keys = in_lds ? keys_lds : a.pend;
   auto kget = [&](uint32_t s) { return in_lds ? keys_lds[s] : ld64(&a.pend[s]); };
   auto kset = [&](uint32_t s, unsigned long long v) {
@@ -619,7 +630,10 @@ __device__ inline void merge_resolve_body(const merge_args& a) {
         st32(&a.zeroer[A.b[j]], kNoIdx);
       }
   }
-  if (threadIdx.x == 0) a.cnt->actors = 0;
+  if (threadIdx.x == 0) {
+    a.cnt->actors = 0;
+    a.cnt->pend1 = 0;
+  }
 }
 
 }  // namespace cuimg

@@ -16,7 +16,7 @@
 // every ping-pong combination (frame slot, current particle vector).  Which
 // combination applies, and whether this frame matches / detects / merges, is
 // derived on the device from the job's level_counters and frequencies, so a
-// frame step is the same fixed sequence of 11 launches for any number of
+// frame step is the same fixed sequence of 12 launches for any number of
 // streams and any frame: it is captured once into a hipGraph and replayed.
 // Each batched launch maps its blocks to jobs through a per-kind prefix array
 // of block counts.  The per-stream input frame pointers are the only per-step
@@ -96,7 +96,7 @@ struct frame_in {
   int vec_ok;
 };
 
-enum { K_PYR, K_BLUR, K_FILL, K_LK, K_PMASK, K_FAST, K_DETC, K_SCATI, K_MSCAN, K_NKINDS };
+enum { K_PYR, K_BLUR, K_FILL, K_LK, K_PMASK, K_FAST, K_DETC, K_SCATI, K_MSCAN, K_MR1, K_NKINDS };
 constexpr int kRing = 8;
 
 }  // namespace cuimg
@@ -279,6 +279,14 @@ __global__ __launch_bounds__(256) void step_mscan_kernel(const level_job* jobs, 
   if (!f.merge) return;
   const merge_args m = uni(J.mg[f.cur ^ (f.detect ? 1u : 0u)]);
   merge_scan_body(m, lb, nb);
+}
+
+__global__ __launch_bounds__(256) void step_mr1_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
+  CUIMG_JOB(prefix, njobs);
+  const step_flags f = flags_of(J);
+  if (!f.merge) return;
+  const merge_args m = uni(J.mg[f.cur ^ (f.detect ? 1u : 0u)]);
+  merge_round1_body(m, lb, nb);
 }
 
 // Merge resolution, then the end of the job's frame step.
@@ -557,6 +565,7 @@ static int build_maps(cuimg_hip_batch* b) {
     cnt[K_DETC].push_back((uint32_t)(R.nbr + R.nblocks));
     cnt[K_SCATI].push_back((uint32_t)(R.nblocks + ceil_div(R.nbr * b->P[J.stream].box_size, 4)));
     cnt[K_MSCAN].push_back((uint32_t)std::max<long>(1, std::min<long>(512, (cap + 255) / 256)));
+    cnt[K_MR1].push_back(cnt[K_MSCAN].back());
   }
   // Matcher: one workgroup per kLkWaves * ppw particles of capacity, at most
   // kLkMaxWgs per job (the workgroups loop over the chunks, see lk_body).
@@ -692,6 +701,7 @@ static int enqueue_step(cuimg_hip_batch* b) {
   CUIMG_TRY(launch(ctx, "scatter_insert",
                    [&] { step_scati_kernel<<<b->total[K_SCATI], 256, 0, st>>>(J, pf[K_SCATI], nj); }));
   CUIMG_TRY(launch(ctx, "merge_scan", [&] { step_mscan_kernel<<<b->total[K_MSCAN], 256, 0, st>>>(J, pf[K_MSCAN], nj); }));
+  CUIMG_TRY(launch(ctx, "merge_round1", [&] { step_mr1_kernel<<<b->total[K_MR1], 256, 0, st>>>(J, pf[K_MR1], nj); }));
   return launch(ctx, "merge_resolve", [&] { step_mres_kernel<<<nj, 1024, 0, st>>>(J); });
 }
 

@@ -22,7 +22,8 @@ struct level_counters {
   uint32_t frame;    // frame steps this scale has completed
   uint32_t cur;      // index (0/1) of the current dense vector
   uint32_t compact_has_run;  // the last frame step ran the detector
-  uint32_t pad[2];
+  uint32_t pend1;    // merge: actors left open by round 1 (in merge_args::pend)
+  uint32_t pad;
   unsigned long long live_total;      // particles (age > 0) run through the matcher
   unsigned long long accepted_total;  // of which accepted (moved)
 };
@@ -111,6 +112,7 @@ int launch_fill_u32(cuimg_hip_ctx* ctx, uint32_t* begin, int pitch, int r0, int 
 size_t merge_scratch_bytes(uint32_t cap);
 merge_args merge_scratch_carve(void* base, uint32_t cap);
 int launch_merge_reset(cuimg_hip_ctx* ctx, const merge_args& a);
+// Scan, round 1 and resolution of one merge (standalone launches).
 int launch_merge(cuimg_hip_ctx* ctx, const merge_args& a);
 
 }  // namespace cuimg
