@@ -362,7 +362,7 @@ def cpu_baseline(video, budget_s, fast_th):
 # HBM traffic of the matcher launch, measured by rocprofv3 counter runs of
 # this benchmark's workload (FETCH_SIZE, WRITE_SIZE, scaled by the readings of
 # the 256 MiB copy of the microbenchmark); the summary names the command.
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "lk_track_1080p_th8_hbm_traffic.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "lk_track_1080p_th8_hbm_traffic.txt")
 
 
 def measured_traffic(path=TRAFFIC_PROFILE):
