@@ -93,13 +93,15 @@ __device__ float blurred_patch_value(const adj_args& a, const float* Ipatch, flo
 __global__ __launch_bounds__(256) void lk_adjoint_kernel(adj_args a) {
   constexpr int PMAX = kAdjMaxWs + 1;
   constexpr int QMAX = PMAX + 2 * kAdjMaxK;
-  __shared__ float s_ip[4][QMAX * QMAX];
+  __shared__ float s_ip[4][2][QMAX * QMAX];
+  __shared__ double s_loss[4];
   __shared__ float s_rt[4][QMAX * PMAX];
   __shared__ float s_b1[4][PMAX * PMAX];
   __shared__ float s_b0[4][PMAX * PMAX];
   __shared__ float s_t[4][PMAX * PMAX];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ws = a.ws, hws = ws / 2, K = a.K, P = ws + 1, Q = P + 2 * K;
+  double wave_loss = 0.0;  // lane 0: this wave's particles, added once per workgroup at the end
   for (uint32_t p = blockIdx.x * 4 + wave; p < a.n; p += gridDim.x * 4) {
     const float2 x = a.pos[p], d = a.d[p];
     const float q1r = x.x + d.x, q1c = x.y + d.y;
@@ -147,20 +149,36 @@ __global__ __launch_bounds__(256) void lk_adjoint_kernel(adj_args a) {
       if (lane == 0) {
         a.loss_p[p] = lp;
         a.dL_dd[p] = make_float2(gr, gc);
-        atomicAdd(a.loss_total, (double)lp);
+        wave_loss += (double)lp;
       }
       continue;
     }
     {
+      // both raw patches in one batch of loads, all issued before the first LDS write
+      constexpr int NQ = (QMAX * QMAX + 63) / 64;
+      float raw[2][NQ];
+#pragma unroll
       for (int img = 0; img < 2; img++) {
         const float* I = img ? a.I1 : a.I0;
         const int pitch = img ? a.pitch1 : a.pitch0;
         const int br = img ? b1r : b0r, bc = img ? b1c : b0c;
-        for (int k = lane; k < Q * Q; k += 64)
-          s_ip[wave][k] = ldI(I, pitch, a.nr, a.nc, br - K + k / Q, bc - K + k % Q);
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const float v = blurred_patch_value(a, s_ip[wave], s_rt[wave], P, Q, lane);
+#pragma unroll
+        for (int m = 0; m < NQ; m++) {
+          const int k = lane + 64 * m;
+          if (k < Q * Q) raw[img][m] = ldI(I, pitch, a.nr, a.nc, br - K + k / Q, bc - K + k % Q);
+        }
+      }
+#pragma unroll
+      for (int img = 0; img < 2; img++)
+#pragma unroll
+        for (int m = 0; m < NQ; m++) {
+          const int k = lane + 64 * m;
+          if (k < Q * Q) s_ip[wave][img][k] = raw[img][m];
+        }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      for (int img = 0; img < 2; img++) {
+        const float v = blurred_patch_value(a, s_ip[wave][img], s_rt[wave], P, Q, lane);
         if (lane < P * P) (img ? B1 : B0)[lane] = v;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -189,7 +207,7 @@ __global__ __launch_bounds__(256) void lk_adjoint_kernel(adj_args a) {
     if (lane == 0) {
       a.loss_p[p] = lp;
       a.dL_dd[p] = make_float2(gr, gc);
-      atomicAdd(a.loss_total, (double)lp);
+      wave_loss += (double)lp;
     }
     // splat 2r with the bilinear weights onto the blurred pixels
     const float w00 = (1 - a1r) * (1 - a1c), w10 = a1r * (1 - a1c), w01 = (1 - a1r) * a1c, w11 = a1r * a1c;
@@ -210,15 +228,22 @@ __global__ __launch_bounds__(256) void lk_adjoint_kernel(adj_args a) {
       __builtin_amdgcn_wave_barrier();
     }
   }
+  // one FP64 atomic per workgroup for the total loss
+  if (lane == 0) s_loss[wave] = wave_loss;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double t = (s_loss[0] + s_loss[1]) + (s_loss[2] + s_loss[3]);
+    if (t != 0.0) atomicAdd(a.loss_total, t);
+  }
 }
 
 // One pass of G^T along columns (axis 1) or rows (axis 0) with the clamp
 // adjoint; in and out are dense nr x nc (out with an explicit pitch).
 __global__ __launch_bounds__(256) void blur_adjoint_pass(const float* in, int in_pitch, float* out, int out_pitch,
                                                          int nr, int nc, int axis, int K, adj_args a) {
-  const long total = (long)nr * nc;
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
-    const int r = (int)(idx / nc), c = (int)(idx % nc);
+  const int total = nr * nc;  // < 2^31 (checked by the host)
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x) {
+    const int r = idx / nc, c = idx - r * nc;
     const int y = axis ? c : r, N = axis ? nc : nr;
     auto at = [&](int x) { return axis ? *px<float>(in, in_pitch, r, x) : *px<float>(in, in_pitch, x, c); };
     float s = 0.f;
@@ -251,6 +276,7 @@ int lk_loss_adjoint(cuimg_hip_ctx* ctx, const cuimg_img* I0, const cuimg_img* I1
     return set_error(ctx, CUIMG_E_INVALID, "lk_loss_adjoint: dL_dI1 must be F32 of the frame domain");
   const int nr = I1->nrows, nc = I1->ncols;
   const size_t npx = (size_t)nr * nc;
+  if (npx >= (size_t)1 << 31) return set_error(ctx, CUIMG_E_UNSUPPORTED, "lk_loss_adjoint: frame too large");
   if (scratch_floats < 2 * npx)
     return set_error(ctx, CUIMG_E_INVALID, "lk_loss_adjoint: scratch holds %zu floats, needs %zu", scratch_floats,
                      2 * npx);
