@@ -118,7 +118,8 @@ struct cuimg_hip_batch {
   hipEvent_t ring_ev[cuimg::kRing]{};
   uint64_t steps = 0;
   std::vector<uint32_t> prefix_h[cuimg::K_NKINDS];
-  uint32_t* prefix_d[cuimg::K_NKINDS]{};
+  uint32_t* prefix_d[cuimg::K_NKINDS]{};   // prefix + block -> job map per kind
+  size_t table_cap[cuimg::K_NKINDS]{};
   uint32_t total[cuimg::K_NKINDS]{};
   cuimg::coef_pack cp{};
   int lk_ppw = 8;     // particles per matcher wave (tuning)
@@ -139,14 +140,20 @@ struct cuimg_hip_tracker {
 namespace cuimg {
 
 // ------------------------------------------------------------ step kernels
-__device__ __forceinline__ int find_job(const uint32_t* prefix, int njobs, uint32_t b) {
-  int lo = 0, hi = njobs - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (prefix[mid] <= b) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
+// Block -> job map of a batched launch.  The per-kind table holds the
+// prefix of block counts (njobs + 1 words) followed by one word per block:
+// (job << kJobShift) | block index within the job, so a workgroup finds its
+// job with one load instead of a binary search over the prefix.
+constexpr int kJobShift = 20;
+constexpr uint32_t kJobBlockMask = (1u << kJobShift) - 1;
+
+struct job_block {
+  int j, lb;
+};
+
+__device__ __forceinline__ job_block job_of_block(const uint32_t* table, int njobs) {
+  const uint32_t e = uni(table[njobs + 1 + blockIdx.x]);
+  return job_block{(int)(e >> kJobShift), (int)(e & kJobBlockMask)};
 }
 
 struct step_flags {
@@ -167,10 +174,11 @@ __device__ __forceinline__ step_flags flags_of(const level_job& J) {
 }
 
 #define CUIMG_JOB(prefix, njobs)                               \
-  const int j = find_job(prefix, njobs, blockIdx.x);           \
+  const job_block jb_ = job_of_block(prefix, njobs);           \
+  const int j = jb_.j;                                         \
   const level_job& J = jobs[j];                                \
-  const int lb = (int)(blockIdx.x - prefix[j]);                \
-  const int nb = (int)(prefix[j + 1] - prefix[j]);             \
+  const int lb = jb_.lb;                                       \
+  const int nb = (int)(uni(prefix[j + 1]) - uni(prefix[j]));   \
   (void)lb;                                                    \
   (void)nb
 
@@ -179,7 +187,8 @@ __device__ __forceinline__ step_flags flags_of(const level_job& J) {
 __global__ __launch_bounds__(kPyrThreads) void step_pyramid_kernel(const stream_job* S, const level_job* jobs,
                                                            const frame_in* in, const uint32_t* prefix, int nstreams) {
   __shared__ pyr_args a;
-  const int st = find_job(prefix, nstreams, blockIdx.x);
+  const job_block jb = job_of_block(prefix, nstreams);
+  const int st = jb.j;
   const stream_job& SJ = S[st];
   const uint32_t slot = jobs[SJ.first].cnt->frame & 1u;
   const int* src = reinterpret_cast<const int*>(&SJ.pyr[slot]);
@@ -192,7 +201,7 @@ __global__ __launch_bounds__(kPyrThreads) void step_pyramid_kernel(const stream_
     a.frame_vec_ok = a.frame_vec_ok && in[st].vec_ok;
   }
   __syncthreads();
-  pyramid_tile<uint8_t>(a, (int)(blockIdx.x - prefix[st]));
+  pyramid_tile<uint8_t>(a, jb.lb);
 }
 
 // Blurred-frame profile: sigma / K Gaussian of every level of the new frame.
@@ -573,11 +582,29 @@ static int build_maps(cuimg_hip_batch* b) {
   for (int j = 0; j < b->njobs; j++)
     cnt[K_LK].push_back((uint32_t)std::min(ceil_div((int)b->lv[j].cap, kLkWaves * b->lk_ppw), kLkMaxWgs));
   for (int k = 0; k < K_NKINDS; k++) {
+    // prefix of block counts, then the block -> (job, block in job) map
     std::vector<uint32_t>& p = b->prefix_h[k];
-    p.assign(cnt[k].size() + 1, 0);
-    for (size_t x = 0; x < cnt[k].size(); x++) p[x + 1] = p[x] + cnt[k][x];
+    const size_t nj = cnt[k].size();
+    p.assign(nj + 1, 0);
+    for (size_t x = 0; x < nj; x++) p[x + 1] = p[x] + cnt[k][x];
     b->total[k] = p.back();
-    if (!b->prefix_d[k]) CUIMG_TRY(dev_alloc_n(b, p.size(), &b->prefix_d[k]));
+    if (nj >= (1u << (32 - kJobShift)))
+      return set_error(b->ctx, CUIMG_E_UNSUPPORTED, "batch: %zu jobs (at most %u)", nj, 1u << (32 - kJobShift));
+    for (size_t x = 0; x < nj; x++) {
+      if (cnt[k][x] > kJobBlockMask + 1)
+        return set_error(b->ctx, CUIMG_E_UNSUPPORTED, "batch: %u blocks in one job", cnt[k][x]);
+      for (uint32_t lb = 0; lb < cnt[k][x]; lb++) p.push_back(((uint32_t)x << kJobShift) | lb);
+    }
+    if (p.size() > b->table_cap[k]) {
+      if (b->prefix_d[k]) {
+        (void)hipStreamSynchronize(b->ctx->stream);
+        (void)hipFree(b->prefix_d[k]);
+        b->allocs.erase(std::find(b->allocs.begin(), b->allocs.end(), (void*)b->prefix_d[k]));
+        b->prefix_d[k] = nullptr;
+      }
+      CUIMG_TRY(dev_alloc_n(b, p.size(), &b->prefix_d[k]));
+      b->table_cap[k] = p.size();
+    }
     CUIMG_HIP(b->ctx, hipMemcpyAsync(b->prefix_d[k], p.data(), p.size() * 4, hipMemcpyHostToDevice, b->ctx->stream));
   }
   CUIMG_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
