@@ -122,7 +122,7 @@ struct cuimg_hip_batch {
   size_t table_cap[cuimg::K_NKINDS]{};
   uint32_t total[cuimg::K_NKINDS]{};
   cuimg::coef_pack cp{};
-  int lk_ppw = 8;     // particles per matcher wave (tuning)
+  int lk_ppw = 4;     // particles per matcher wave (tuning)
   int blur_th = 0;    // blur tile height, 0 = heuristic (tuning)
   bool use_graph = true;
   hipGraphExec_t exec = nullptr;

@@ -280,7 +280,7 @@ int cuimg_hip_batch_set_graph(cuimg_hip_batch* b, int enable);
  * tracker_run on it is refused, tracker_destroy does nothing). */
 int cuimg_hip_batch_tracker(cuimg_hip_batch* b, int stream, cuimg_hip_tracker** out);
 /* Launch shape: lk_ppw = particles per matcher wave (a workgroup of 4 waves
- * takes 4 * lk_ppw consecutive particles; default 8), blur_th = blur tile
+ * takes 4 * lk_ppw consecutive particles; default 4), blur_th = blur tile
  * height in rows (0 = chosen from the total work).  Reading reports the
  * values in use. */
 int cuimg_hip_batch_set_tuning(cuimg_hip_batch* b, int lk_ppw, int blur_th);
