@@ -34,6 +34,10 @@ extern "C" int cuimg_hip_lk_track(cuimg_hip_ctx* ctx, const cuimg_img* prev, con
     if (A.dtype != dt || B.dtype != dt || A.nrows != B.nrows || A.ncols != B.ncols || !A.begin || !B.begin ||
         A.nrows < 1 || A.ncols < 1)
       return set_error(ctx, CUIMG_E_INVALID, "lk_track: level %d images differ or are empty", j);
+    // patches are addressed with 32-bit byte offsets from the image start
+    if (A.pitch < 1 || B.pitch < 1 || A.pitch >= (1 << 24) || B.pitch >= (1 << 24) || A.nrows >= (1 << 24) ||
+        (uint64_t)A.nrows * (uint64_t)std::max(A.pitch, B.pitch) >= (1ull << 32))
+      return set_error(ctx, CUIMG_E_UNSUPPORTED, "lk_track: level %d too large for 32-bit offsets", j);
     a.lv[j] = lk_level_view{A.begin, B.begin, A.pitch, B.pitch, A.nrows, A.ncols};
   }
   const cuimg_img& S = prev[scale];

@@ -111,10 +111,6 @@ __device__ __forceinline__ float norml2(float a, float b) {
 // an integer, far more than the float error of the product.
 __device__ __forceinline__ int divq(int k, float invq) { return (int)(((float)k + 0.5f) * invq); }
 
-template <typename TI>
-__device__ __forceinline__ float ld(const void* img, int pitch, int r, int c) {
-  return (float)*px<TI>(img, pitch, r, c);
-}
 
 // Bilinear weights of li: base = coordinates truncated toward zero.
 struct bw {
@@ -214,12 +210,18 @@ __device__ __forceinline__ void load_patch(float* dst, int q, const void* img, i
   constexpr int MP = (MAXQ + RPP - 1) / RPP;
   const int col = WHOLE ? lane % PS : lane, row = WHOLE ? lane / PS : 0;
   const bool on = col < q && col < PS;
-  const int c = clampi(c0 + col, 0, nc - 1);
+  // 32-bit byte offsets from the (wave-uniform) image base: one 24-bit
+  // multiply-add per load (images are below 2^24 rows and 2^24 pitch bytes)
+  const uint32_t cb = (uint32_t)clampi(c0 + col, 0, nc - 1) * (uint32_t)sizeof(TI);
+  const char* base = static_cast<const char*>(img);
   float v[MP];
 #pragma unroll
   for (int m = 0; m < MP; m++) {
     const int i = row + m * RPP;
-    if (on && i < q) v[m] = ld<TI>(img, pitch, clampi(r0 + i, 0, nr - 1), c);
+    if (on && i < q) {
+      const uint32_t off = __umul24((uint32_t)clampi(r0 + i, 0, nr - 1), (uint32_t)pitch) + cb;
+      v[m] = (float)*reinterpret_cast<const TI*>(base + off);
+    }
   }
 #pragma unroll
   for (int m = 0; m < MP; m++) {

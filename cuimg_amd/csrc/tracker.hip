@@ -363,6 +363,9 @@ static int fill_sparse(cuimg_hip_ctx* ctx, level_res& R) {
 }
 
 static int check_params(cuimg_hip_ctx* ctx, int nrows, int ncols, const cuimg_tracker_params* P) {
+  // the matcher addresses images with 32-bit byte offsets (24-bit rows and pitch)
+  if (nrows < 1 || ncols < 1 || nrows > 16384 || ncols > 16384)
+    return set_error(ctx, CUIMG_E_UNSUPPORTED, "tracker: frame %dx%d (each side 1..16384)", nrows, ncols);
   if (P->nscales < 1 || P->nscales > 7)
     return set_error(ctx, CUIMG_E_INVALID, "tracker: nscales %d not in 1..7", P->nscales);
   if (P->lk_ws < 1 || P->lk_ws > 15 || !(P->lk_ws & 1))
