@@ -155,6 +155,8 @@ def test_bad_arguments_raise(ctx):
         cu.Tracker(ctx, 100, 100, cu.profile_params("B", 9))
     with pytest.raises(cu.CuimgError):
         cu.Tracker(ctx, 100, 100, cu.profile_params("B", 2, lk_ws=8))
+    with pytest.raises(cu.CuimgError):  # the matcher's 32-bit offsets: at most 16384 per side
+        cu.Tracker(ctx, 64, 20000, cu.profile_params("B", 2))
     g = cu.Tracker(ctx, 64, 64, cu.profile_params("B", 2))
     with pytest.raises(cu.CuimgError):
         g.run(ctx.image(32, 32, cu.U8))
