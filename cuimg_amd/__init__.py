@@ -129,6 +129,9 @@ def lib():
         L.cuimg_hip_tracker_matcher_images.argtypes = [vp, c_int, P(Img), P(Img), P(Img)]
         L.cuimg_hip_tracker_frame_count.argtypes = [vp, P(c_int)]
         L.cuimg_hip_particles_merge.argtypes = [vp, vp, ctypes.c_uint32, P(Img)]
+        L.cuimg_hip_particle_mask.argtypes = [vp, vp, ctypes.c_uint32, P(Img)]
+        L.cuimg_hip_fast_select.argtypes = [vp, P(Img), P(Img), c_int, ctypes.c_float, vp, ctypes.c_uint32,
+                                            P(ctypes.c_uint32)]
         L.cuimg_hip_tracker_work.argtypes = [vp, c_int, P(ctypes.c_uint64), P(ctypes.c_uint64)]
         L.cuimg_hip_tracker_set_graph.argtypes = [vp, c_int]
         L.cuimg_hip_tracker_set_tuning.argtypes = [vp, c_int, c_int]
@@ -592,6 +595,46 @@ def particles_merge(ctx, parts, sparse):
     finally:
         L.cuimg_hip_free(ctx.h, dev)
     return out
+
+
+def particle_mask(ctx, parts, nrows, ncols):
+    """Detector mask of a dense particle vector on the device: (nrows, ncols)
+    uint8, 255 except 0 on the 3x3 block of every particle."""
+    L = ctx.L
+    parts = np.ascontiguousarray(parts, PARTICLE_DTYPE)
+    img = ctx.image(nrows, ncols, U8)
+    dev = DeviceBuffer(ctx, parts.nbytes)
+    if len(parts):
+        ctx.check(L.cuimg_hip_memcpy_h2d(ctx.h, dev.p, parts.ctypes.data, parts.nbytes), "h2d")
+    ctx.check(L.cuimg_hip_particle_mask(ctx.h, dev.p, len(parts), ctypes.byref(img.d)), "particle_mask")
+    out = img.download()
+    dev.free()
+    return out
+
+
+def fast_select(ctx, sal, sparse, box, th, cap=None):
+    """New-point selection on the device: int32 (n, 2) array of (row, col) in
+    raster order.  `sparse` is the (nrows, ncols) index image (-1 = empty)."""
+    L = ctx.L
+    sal = np.ascontiguousarray(sal, np.uint8)
+    sp = np.ascontiguousarray(sparse).astype(np.int32).view(np.uint32)
+    nr, nc = sal.shape
+    s_img = ctx.image(nr, nc, U8)
+    s_img.upload(sal)
+    p_img = ctx.image(nr, nc, U32)
+    p_img.upload(sp)
+    if cap is None:
+        cap = max(1, (nr // max(1, box)) * (nc // max(1, box)))
+    dev = DeviceBuffer(ctx, 8 * cap)
+    n = ctypes.c_uint32()
+    ctx.check(L.cuimg_hip_fast_select(ctx.h, ctypes.byref(s_img.d), ctypes.byref(p_img.d), box, float(th),
+                                      dev.p, cap, ctypes.byref(n)), "fast_select")
+    k = min(n.value, cap)
+    out = np.zeros((k, 2), np.int32)
+    if k:
+        ctx.check(L.cuimg_hip_memcpy_d2h(ctx.h, out.ctypes.data, dev.p, out.nbytes), "d2h")
+    dev.free()
+    return out, n.value
 
 
 class DeviceBuffer:

@@ -116,16 +116,21 @@ __device__ inline void fill_u8_body(uint8_t* begin, int pitch, int nr, int nc, u
 }
 
 // ------------------------------------------------------------ detector mask
-__device__ inline void particle_mask_body(const detect_args& a, int bid, int nb) {
-  const uint32_t n = a.cnt->n;
+// Clears the 3x3 block of each of the n particles (the mask already holds 255).
+__device__ inline void particle_mask_core(const cuimg_particle_f* parts, uint32_t n, uint8_t* mask, int pitch,
+                                          int nr, int nc, int bid, int nb) {
   for (uint32_t i = bid * blockDim.x + threadIdx.x; i < n; i += nb * blockDim.x) {
-    const int r = tzf(a.parts[i].pos_r), c = tzf(a.parts[i].pos_c);
+    const int r = tzf(parts[i].pos_r), c = tzf(parts[i].pos_c);
     for (int dr = -1; dr <= 1; dr++)
       for (int dc = -1; dc <= 1; dc++) {
         const int rr = r + dr, cc = c + dc;
-        if (rr >= 0 && rr < a.nr && cc >= 0 && cc < a.nc) *px<uint8_t>(a.mask, a.mask_pitch, rr, cc) = 0;
+        if (rr >= 0 && rr < nr && cc >= 0 && cc < nc) *px<uint8_t>(mask, pitch, rr, cc) = 0;
       }
   }
+}
+
+__device__ inline void particle_mask_body(const detect_args& a, int bid, int nb) {
+  particle_mask_core(a.parts, a.cnt->n, a.mask, a.mask_pitch, a.nr, a.nc, bid, nb);
 }
 
 // ---------------------------------------------------------- box selection
@@ -270,43 +275,58 @@ __device__ inline void compact_scatter_body(const detect_args& a, int bid) {
   }
 }
 
-// One wave per sub-row (4 per 256-thread block): appends its points in column
-// order.
-__device__ inline void insert_new_body(const detect_args& a, int bid) {
+// One wave per sub-row (4 per 256-thread block) visits its selected points in
+// column order: emit(k, r, c) with k the point's rank among all new points
+// (subcnt holds the scanned sub-row offsets).
+template <class Emit>
+__device__ inline void for_new_points(const detect_args& a, int bid, Emit emit) {
   const int sub = bid * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (sub >= a.nbr * a.box) return;
-  const level_counters& C = *a.cnt;
-  const uint32_t n_live = C.n_live, n_old = C.n_old, n = C.n;
   const int bi = sub / a.box, dr = sub % a.box;
-  uint32_t k = a.subcnt[sub];  // rank among new points
+  uint32_t k = a.subcnt[sub];
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int b0 = 0; b0 < a.nbc; b0 += 64) {
     const int bj = b0 + lane;
     const uint8_t code = bj < a.nbc ? a.boxcode[(size_t)bi * a.nbc + bj] : 0xFF;
     const bool hit = code != 0xFF && code / a.box == dr;
     const uint64_t m = __ballot(hit);
-    if (hit) {
-      const uint32_t kk = k + (uint32_t)__popcll(m & below);
-      const uint32_t j = n_live + kk;
-      if (j < n) {
-        const int r = bi * a.box + 1 + dr, c = bj * a.box + 1 + code % a.box;
-        cuimg_particle_f p;
-        p.speed_r = 0.f;
-        p.speed_c = 0.f;
-        p.pos_r = (float)r;
-        p.pos_c = (float)c;
-        p.age = 1;
-        p.pad = 0;
-        a.parts_out[j] = p;
-        *px<uint32_t>(a.sparse, a.sparse_pitch, r, c) = j;
-        a.matches[n_old + kk] = j;
-      } else {
-        a.matches[n_old + kk] = kEmpty;
-      }
-    }
+    if (hit) emit(k + (uint32_t)__popcll(m & below), bi * a.box + 1 + dr, bj * a.box + 1 + code % a.box);
     k += (uint32_t)__popcll(m);
   }
+}
+
+// Appends the new points after the live particles of the compacted vector.
+__device__ inline void insert_new_body(const detect_args& a, int bid) {
+  const level_counters& C = *a.cnt;
+  const uint32_t n_live = C.n_live, n_old = C.n_old, n = C.n;
+  for_new_points(a, bid, [&](uint32_t kk, int r, int c) {
+    const uint32_t j = n_live + kk;
+    if (j < n) {
+      cuimg_particle_f p;
+      p.speed_r = 0.f;
+      p.speed_c = 0.f;
+      p.pos_r = (float)r;
+      p.pos_c = (float)c;
+      p.age = 1;
+      p.pad = 0;
+      a.parts_out[j] = p;
+      *px<uint32_t>(a.sparse, a.sparse_pitch, r, c) = j;
+      a.matches[n_old + kk] = j;
+    } else {
+      a.matches[n_old + kk] = kEmpty;
+    }
+  });
+}
+
+// Standalone selection output: (row, col) of the first cap new points.
+__device__ inline void select_points_body(const detect_args& a, int32_t* pts, uint32_t cap, int bid) {
+  for_new_points(a, bid, [&](uint32_t kk, int r, int c) {
+    if (kk < cap) {
+      pts[2 * (size_t)kk] = r;
+      pts[2 * (size_t)kk + 1] = c;
+    }
+  });
 }
 
 // ------------------------------------------------------------------ merge

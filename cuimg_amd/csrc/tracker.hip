@@ -799,6 +799,24 @@ static int clear_stream(cuimg_hip_batch* b, int i) {
   return 0;
 }
 
+// ------------------------------------------- standalone detector entry points
+__global__ __launch_bounds__(256) void mask_fill_kernel(uint8_t* m, int pitch, int nr, int nc) {
+  fill_u8_body(m, pitch, nr, nc, 0xFFFFFFFFu, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(256) void mask_particles_kernel(const cuimg_particle_f* parts, uint32_t n, uint8_t* m,
+                                                             int pitch, int nr, int nc) {
+  particle_mask_core(parts, n, m, pitch, nr, nc, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(256) void select_rows_kernel(detect_args d) { nms_row_body(d, d.fast_th, blockIdx.x); }
+
+__global__ __launch_bounds__(1024) void select_scan_kernel(detect_args d) { compact_scan_body(d); }
+
+__global__ __launch_bounds__(256) void select_points_kernel(detect_args d, int32_t* pts, uint32_t cap) {
+  select_points_body(d, pts, cap, blockIdx.x);
+}
+
 }  // namespace cuimg
 
 using namespace cuimg;
@@ -1114,6 +1132,73 @@ int cuimg_hip_particles_merge(cuimg_hip_ctx* ctx, cuimg_particle_f* parts, uint3
   (void)hipStreamSynchronize(ctx->stream);
   (void)hipFree(buf);
   (void)hipFree(cnt);
+  return rc;
+}
+
+int cuimg_hip_particle_mask(cuimg_hip_ctx* ctx, const cuimg_particle_f* parts, uint32_t n, const cuimg_img* mask) {
+  if (!ctx || (!parts && n) || !mask || mask->dtype != CUIMG_U8 || !mask->begin || mask->nrows < 1 ||
+      mask->ncols < 1)
+    return set_error(ctx, CUIMG_E_INVALID, "particle_mask: bad arguments");
+  uint8_t* m = static_cast<uint8_t*>(mask->begin);
+  const int nr = mask->nrows, nc = mask->ncols, pitch = (int)mask->pitch;
+  hipStream_t st = ctx->stream;
+  CUIMG_TRY(launch(ctx, "mask_fill", [&] {
+    mask_fill_kernel<<<std::min(nr, 1024), 256, 0, st>>>(m, pitch, nr, nc);
+  }));
+  if (n)
+    CUIMG_TRY(launch(ctx, "particle_mask", [&] {
+      mask_particles_kernel<<<std::min<uint32_t>(ceil_div((int)std::min<uint32_t>(n, 1u << 30), 256), 4096), 256,
+                              0, st>>>(parts, n, m, pitch, nr, nc);
+    }));
+  return 0;
+}
+
+int cuimg_hip_fast_select(cuimg_hip_ctx* ctx, const cuimg_img* sal, const cuimg_img* sparse, int box, float th,
+                          int32_t* pts, uint32_t cap, uint32_t* count) {
+  if (!ctx || !sal || !sparse || !count || (!pts && cap) || sal->dtype != CUIMG_U8 || sparse->dtype != CUIMG_U32 ||
+      !sal->begin || !sparse->begin || sal->nrows != sparse->nrows || sal->ncols != sparse->ncols ||
+      sal->nrows < 1 || sal->ncols < 1)
+    return set_error(ctx, CUIMG_E_INVALID, "fast_select: bad arguments");
+  if (box < 1 || box > kMaxBox)
+    return set_error(ctx, CUIMG_E_UNSUPPORTED, "fast_select: box %d not in 1..%d", box, kMaxBox);
+  detect_args d{};
+  d.nr = sal->nrows;
+  d.nc = sal->ncols;
+  d.box = box;
+  d.nbr = (int)((float)d.nr / (float)box);
+  d.nbc = (int)((float)d.nc / (float)box);
+  d.fast_th = th;
+  *count = 0;
+  if (d.nbr < 1 || d.nbc < 1) return 0;
+  d.sal = static_cast<const uint8_t*>(sal->begin);
+  d.sal_pitch = (int)sal->pitch;
+  d.sparse = static_cast<uint32_t*>(sparse->begin);
+  d.sparse_pitch = (int)sparse->pitch;
+  d.cap = cap;
+  d.nblocks = 0;  // the scan's particle-block pass is empty
+  const size_t nboxes = (size_t)d.nbr * d.nbc, nsub = (size_t)d.nbr * box;
+  const size_t off_sub = (nboxes + 255) & ~(size_t)255, off_cnt = off_sub + ((nsub * 4 + 255) & ~(size_t)255);
+  void* buf = nullptr;
+  CUIMG_HIP(ctx, hipMalloc(&buf, off_cnt + sizeof(level_counters)));
+  char* base = static_cast<char*>(buf);
+  d.boxcode = reinterpret_cast<uint8_t*>(base);
+  d.subcnt = reinterpret_cast<uint32_t*>(base + off_sub);
+  d.cnt = reinterpret_cast<level_counters*>(base + off_cnt);
+  hipStream_t st = ctx->stream;
+  const hipError_t e = hipMemsetAsync(d.cnt, 0, sizeof(level_counters), st);
+  int rc = e == hipSuccess ? 0 : hip_error(ctx, e, "hipMemsetAsync", __FILE__, __LINE__);
+  if (!rc) rc = launch(ctx, "select_rows", [&] { select_rows_kernel<<<d.nbr, 256, 0, st>>>(d); });
+  if (!rc) rc = launch(ctx, "select_scan", [&] { select_scan_kernel<<<1, 1024, 0, st>>>(d); });
+  if (!rc && cap)
+    rc = launch(ctx, "select_points", [&] { select_points_kernel<<<ceil_div((int)nsub, 4), 256, 0, st>>>(d, pts, cap); });
+  level_counters c{};
+  if (!rc) {
+    const hipError_t ec = hipMemcpyAsync(&c, d.cnt, sizeof c, hipMemcpyDeviceToHost, st);
+    if (ec != hipSuccess) rc = hip_error(ctx, ec, "hipMemcpyAsync", __FILE__, __LINE__);
+  }
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(buf);
+  if (!rc) *count = c.n_new;
   return rc;
 }
 

@@ -142,6 +142,26 @@ int cuimg_hip_pyramid_build(cuimg_hip_ctx* ctx, const cuimg_img* frame, const cu
 int cuimg_hip_fast_saliency(cuimg_hip_ctx* ctx, const cuimg_img* in, const cuimg_img* mask,
                             const cuimg_img* out, int n, int th);
 
+/* Detector mask (create_detector_mask + particle_mask_kernel,
+ * tracking2/pyrlk_tracker.hpp:879-888, tracking2/tracking_strategies.hpp:201-225):
+ * mask <- 255, then 0 on the 3x3 block around the truncated position of each of
+ * the n particles (device array, dead particles included); pixels outside the
+ * mask's domain are left alone.  Ordered on the context's stream. */
+int cuimg_hip_particle_mask(cuimg_hip_ctx* ctx, const cuimg_particle_f* parts, uint32_t n,
+                            const cuimg_img* mask);
+
+/* Box selection of new points (fast_detector::new_particles,
+ * tracking2/fast_detector.hpp:217-262): one box of box x box pixels at
+ * (i*box + 1, j*box + 1) per cell of domain/box; a box holding a particle
+ * (sparse != 0xFFFFFFFF) is skipped; its first strict saliency maximum in raster
+ * order is kept unless below th; the kept points inside domain - border(3) are
+ * written as (row, col) int32 pairs to the device array pts in raster order
+ * (the order particle_container::add receives them).  *count = number of
+ * points (only the first cap are written).  sal: U8, sparse: U32, same domain;
+ * box in 1..15.  Synchronises the context. */
+int cuimg_hip_fast_select(cuimg_hip_ctx* ctx, const cuimg_img* sal, const cuimg_img* sparse, int box,
+                          float th, int32_t* pts, uint32_t cap, uint32_t* count);
+
 /* ---------------------------------------------------------------- tracker */
 typedef struct {
   int32_t nscales;

@@ -125,6 +125,10 @@ class Oracle:
         L.orc_tracker_get_decisions.argtypes = [vp, c_int, P(ctypes.c_int32), P(c_float), P(c_float), c_int]
         L.orc_tracker_lk_decide.argtypes = [vp, c_int, P(ctypes.c_int32), P(c_float), P(c_float), c_int]
         L.orc_merge.argtypes = [vp, c_int, P(ctypes.c_int32), c_int, c_int]
+        L.orc_particle_mask.argtypes = [vp, c_int, c_int, c_int, P(ctypes.c_uint8)]
+        L.orc_fast_select.restype = c_int
+        L.orc_fast_select.argtypes = [P(ctypes.c_uint8), P(ctypes.c_int32), c_int, c_int, c_int, c_float,
+                                      P(ctypes.c_int32), c_int]
         L.orc_lk_loss_grad_f64.restype = c_double
         L.orc_lk_loss_grad_f64.argtypes = [P(c_float), P(c_float), c_int, c_int, P(c_float), c_int,
                                            P(c_float), P(c_float), c_int, c_int, vp, vp]
@@ -220,6 +224,25 @@ class Oracle:
         self.L.orc_merge(parts.ctypes.data, len(parts), _ptr(sparse, ctypes.c_int32), sparse.shape[0],
                          sparse.shape[1])
         return parts
+
+    def particle_mask(self, parts, nr, nc):
+        """Detector mask of a dense particle vector (uint8 nr x nc)."""
+        parts = np.ascontiguousarray(parts, PARTICLE_DTYPE)
+        mask = np.zeros((nr, nc), np.uint8)
+        self.L.orc_particle_mask(parts.ctypes.data if len(parts) else None, len(parts), nr, nc,
+                                 _ptr(mask, ctypes.c_uint8))
+        return mask
+
+    def fast_select(self, sal, sparse, box, th):
+        """New-point selection: int32 (n, 2) array of (row, col) in raster order."""
+        sal = np.ascontiguousarray(sal, np.uint8)
+        sparse = np.ascontiguousarray(sparse, np.int32)
+        nr, nc = sal.shape
+        cap = max(1, (nr // max(1, box) + 1) * (nc // max(1, box) + 1))
+        pts = np.zeros((cap, 2), np.int32)
+        n = self.L.orc_fast_select(_ptr(sal, ctypes.c_uint8), _ptr(sparse, ctypes.c_int32), nr, nc, box,
+                                   float(th), _ptr(pts, ctypes.c_int32), cap)
+        return pts[:min(n, cap)].copy()
 
     # ---------------------------------------------------------------- tracker
     def tracker(self, nr, nc, params):

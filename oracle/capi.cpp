@@ -212,6 +212,23 @@ int orc_tracker_lk_decide(void* t, int s, int32_t* accept, float* tr2, float* di
   return copy_decisions(d, accept, tr2, dist, cap);
 }
 
+void orc_particle_mask(const orc_particle* parts, int n, int nr, int nc, uint8_t* mask) {
+  std::vector<particle> v(n);
+  if (n) std::memcpy(v.data(), parts, sizeof(particle) * n);
+  const image<uint8_t> m = particle_mask(v, nr, nc);
+  std::memcpy(mask, m.px.data(), m.px.size());
+}
+
+int orc_fast_select(const uint8_t* sal, const int32_t* sparse, int nr, int nc, int box, float th, int32_t* pts,
+                    int cap) {
+  const std::vector<std::pair<int, int>> v = fast_select(wrap(sal, nr, nc), wrap(sparse, nr, nc), box, th);
+  for (size_t k = 0; k < v.size() && (int)k < cap; k++) {
+    pts[2 * k] = v[k].first;
+    pts[2 * k + 1] = v[k].second;
+  }
+  return (int)v.size();
+}
+
 void orc_merge(orc_particle* parts, int n, const int32_t* sparse, int nr, int nc) {
   std::vector<particle> v(n);
   std::memcpy(v.data(), parts, sizeof(particle) * n);

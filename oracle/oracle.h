@@ -84,6 +84,13 @@ int orc_tracker_lk_decide(void* t, int s, int32_t* accept, float* tr2, float* di
 
 /* merge_trajectories over a container state (in place); sparse: nr*nc, -1 empty. */
 void orc_merge(orc_particle* parts, int n, const int32_t* sparse, int nr, int nc);
+/* Detector mask of a dense vector: 255, 0 on the 3x3 block of every particle
+ * (tracking_strategies.hpp:201-225). */
+void orc_particle_mask(const orc_particle* parts, int n, int nr, int nc, uint8_t* mask);
+/* Box selection of new points (fast_detector.hpp:217-262), (row, col) pairs in
+ * raster order; returns the number of points (at most cap written). */
+int orc_fast_select(const uint8_t* sal, const int32_t* sparse, int nr, int nc, int box, float th, int32_t* pts,
+                    int cap);
 
 /* FP64 photometric LK loss and its gradient (blurred frames, bilinear samples). */
 double orc_lk_loss_grad_f64(const float* I0, const float* I1, int nr, int nc, const float* coef,

@@ -1,5 +1,7 @@
 // Oracle tracker state.  TEST INFRASTRUCTURE ONLY (see oracle_core.hpp).
 #pragma once
+#include <utility>
+
 #include "oracle_core.hpp"
 
 namespace orc {
@@ -40,6 +42,12 @@ struct level_state {
   int frame_cpt = 0;
   box pdomain{0, 0, -1, -1};
 };
+
+// Detector mask of a dense vector (255, 0 on the 3x3 block of every particle).
+image<uint8_t> particle_mask(const std::vector<particle>& parts, int nr, int nc);
+// Box selection of new points, in the raster order they are added.
+std::vector<std::pair<int, int>> fast_select(const image<uint8_t>& sal, const image<int32_t>& sparse, int box,
+                                             float th);
 
 struct tracker {
   tracker_params P;

@@ -157,25 +157,30 @@ void tracker::match_particles(int s) {
 // pyrlk_cpu::create_detector_mask + particle_mask_kernel
 // (tracking2/pyrlk_tracker.hpp:879-888, tracking_strategies.hpp:201-225): every
 // particle of the dense vector, dead ones included, clears its 3x3 block.
-void tracker::detector_mask(level_state& L) {
-  L.mask = image<uint8_t>(L.nr, L.nc, 255);
-  for (const particle& p : L.parts) {
+image<uint8_t> particle_mask(const std::vector<particle>& parts, int nr, int nc) {
+  image<uint8_t> mask(nr, nc, 255);
+  for (const particle& p : parts) {
     int r = tz(p.pos_r), c = tz(p.pos_c);
-    if (L.mask.has(r, c)) L.mask(r, c) = 0;
+    if (mask.has(r, c)) mask(r, c) = 0;
     for (int k = 0; k < 8; k++)
-      if (L.mask.has(r + kC8[k][0], c + kC8[k][1])) L.mask(r + kC8[k][0], c + kC8[k][1]) = 0;
+      if (mask.has(r + kC8[k][0], c + kC8[k][1])) mask(r + kC8[k][0], c + kC8[k][1]) = 0;
   }
+  return mask;
 }
 
-// fast_detector::new_particles(cpu) (tracking2/fast_detector.hpp:217-262):
-// one box of box_size^2 pixels at p*box + (1,1) per cell of domain/box; a box
-// holding any particle is skipped; the first strict maximum in raster order
-// wins; saliency below the float threshold is rejected; then particles are
-// added in raster order over domain - border(3).
-void tracker::new_particles(level_state& L) {
-  const int b = P.box_size;
-  const int br = (int)((float)L.nr / (float)b), bc = (int)((float)L.nc / (float)b);
-  image<uint8_t> np(L.nr, L.nc, 0);
+void tracker::detector_mask(level_state& L) { L.mask = particle_mask(L.parts, L.nr, L.nc); }
+
+// fast_detector::new_particles(cpu) (tracking2/fast_detector.hpp:217-262),
+// selection part: one box of box_size^2 pixels at p*box + (1,1) per cell of
+// domain/box; a box holding any particle (sparse index != -1) is skipped; the
+// first strict maximum in raster order wins; saliency below the float
+// threshold is rejected; the points come out in raster order over
+// domain - border(3), the order in which new_particles adds them.
+std::vector<std::pair<int, int>> fast_select(const image<uint8_t>& sal, const image<int32_t>& sparse, int b,
+                                             float th) {
+  const int nr = sal.nr, nc = sal.nc;
+  const int br = (int)((float)nr / (float)b), bc = (int)((float)nc / (float)b);
+  image<uint8_t> np(nr, nc, 0);
 #pragma omp parallel for schedule(static, 2)
   for (int i = 0; i < br; i++)
     for (int j = 0; j < bc; j++) {
@@ -183,38 +188,45 @@ void tracker::new_particles(level_state& L) {
       bool skip = false;
       for (int r = 0; r < b && !skip; r++)
         for (int c = 0; c < b; c++)
-          if (L.cur_u8.has(pr + r, pc + c) && sparse_at(L.sparse, pr + r, pc + c) != -1) {
+          if (sal.has(pr + r, pc + c) && sparse_at(sparse, pr + r, pc + c) != -1) {
             skip = true;
             break;
           }
       if (skip) continue;
-      float vmax = (float)L.sal(pr, pc);
+      // only box == 1 reaches past the domain here, where the saliency border is 0
+      float vmax = sal.has(pr, pc) ? (float)sal(pr, pc) : 0.f;
       int mr = pr, mc = pc;
       for (int r = 0; r < b; r++)
         for (int c = 0; c < b; c++)
-          if (L.cur_u8.has(pr + r, pc + c)) {
-            float sv = (float)L.sal(pr + r, pc + c);
+          if (sal.has(pr + r, pc + c)) {
+            float sv = (float)sal(pr + r, pc + c);
             if (vmax < sv) {
               vmax = sv;
               mr = pr + r;
               mc = pc + c;
             }
           }
-      if (vmax < P.fast_th) continue;
+      if (vmax < th || !np.has(mr, mc)) continue;
       np(mr, mc) = 1;
     }
-  const box in3 = inner(L.nr, L.nc, 3);
+  std::vector<std::pair<int, int>> out;
+  const box in3 = inner(nr, nc, 3);
   for (int r = in3.r1; r <= in3.r2; r++)
     for (int c = in3.c1; c <= in3.c2; c++)
-      if (np(r, c)) {
-        // particle_container::add (tracking2/particle_container.hpp:370-387)
-        particle p{};
-        p.age = 1;
-        p.pos_r = (float)r;
-        p.pos_c = (float)c;
-        L.sparse(r, c) = (int32_t)L.parts.size();
-        L.parts.push_back(p);
-      }
+      if (np(r, c)) out.emplace_back(r, c);
+  return out;
+}
+
+void tracker::new_particles(level_state& L) {
+  for (const auto& rc : fast_select(L.sal, L.sparse, P.box_size, P.fast_th)) {
+    // particle_container::add (tracking2/particle_container.hpp:370-387)
+    particle p{};
+    p.age = 1;
+    p.pos_r = (float)rc.first;
+    p.pos_c = (float)rc.second;
+    L.sparse(rc.first, rc.second) = (int32_t)L.parts.size();
+    L.parts.push_back(p);
+  }
 }
 
 // particle_container::compact(cpu) (tracking2/particle_container.hpp:216-252).
