@@ -130,6 +130,9 @@ def lib():
         L.cuimg_hip_tracker_frame_count.argtypes = [vp, P(c_int)]
         L.cuimg_hip_particles_merge.argtypes = [vp, vp, ctypes.c_uint32, P(Img)]
         L.cuimg_hip_particle_mask.argtypes = [vp, vp, ctypes.c_uint32, P(Img)]
+        L.cuimg_hip_particles_add.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, ctypes.c_uint32,
+                                              P(Img), P(ctypes.c_uint32)]
+        L.cuimg_hip_particles_compact.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, P(Img), P(ctypes.c_uint32)]
         L.cuimg_hip_fast_select.argtypes = [vp, P(Img), P(Img), c_int, ctypes.c_float, vp, ctypes.c_uint32,
                                             P(ctypes.c_uint32)]
         L.cuimg_hip_tracker_work.argtypes = [vp, c_int, P(ctypes.c_uint64), P(ctypes.c_uint64)]
@@ -635,6 +638,55 @@ def fast_select(ctx, sal, sparse, box, th, cap=None):
         ctx.check(L.cuimg_hip_memcpy_d2h(ctx.h, out.ctypes.data, dev.p, out.nbytes), "d2h")
     dev.free()
     return out, n.value
+
+
+def _sparse_image(ctx, sparse):
+    sp = np.ascontiguousarray(sparse).astype(np.int32).view(np.uint32)
+    img = ctx.image(sp.shape[0], sp.shape[1], U32)
+    img.upload(sp)
+    return img
+
+
+def particles_add(ctx, parts, sparse, pts, cap=None):
+    """particle_container::add of each (row, col) of pts on the device while
+    below cap: returns (particles, sparse) with sparse as int32 (-1 = empty)."""
+    L = ctx.L
+    parts = np.ascontiguousarray(parts, PARTICLE_DTYPE)
+    pts = np.ascontiguousarray(np.asarray(pts, np.int32).reshape(-1, 2))
+    cap = len(parts) + len(pts) if cap is None else cap
+    img = _sparse_image(ctx, sparse)
+    dev = DeviceBuffer(ctx, max(cap, len(parts)) * PARTICLE_DTYPE.itemsize)
+    if len(parts):
+        ctx.check(L.cuimg_hip_memcpy_h2d(ctx.h, dev.p, parts.ctypes.data, parts.nbytes), "h2d")
+    dpts = DeviceBuffer.from_array(ctx, pts)
+    n = ctypes.c_uint32()
+    ctx.check(L.cuimg_hip_particles_add(ctx.h, dev.p, len(parts), cap, dpts.p, len(pts), ctypes.byref(img.d),
+                                        ctypes.byref(n)), "particles_add")
+    out = dev.download(PARTICLE_DTYPE, (n.value,))
+    sp = img.download().view(np.int32)
+    dev.free()
+    dpts.free()
+    return out, sp
+
+
+def particles_compact(ctx, parts, sparse):
+    """particle_container::compact on the device: returns (particles, matches,
+    sparse) with matches as uint32 (0xFFFFFFFF = removed) and sparse as int32."""
+    L = ctx.L
+    parts = np.ascontiguousarray(parts, PARTICLE_DTYPE)
+    img = _sparse_image(ctx, sparse)
+    src = DeviceBuffer.from_array(ctx, parts)
+    dst = DeviceBuffer(ctx, parts.nbytes)
+    mt = DeviceBuffer(ctx, 4 * len(parts))
+    n = ctypes.c_uint32()
+    ctx.check(L.cuimg_hip_particles_compact(ctx.h, src.p, len(parts), dst.p, mt.p, ctypes.byref(img.d),
+                                            ctypes.byref(n)), "particles_compact")
+    out = dst.download(PARTICLE_DTYPE, (n.value,))
+    matches = mt.download(np.uint32, (len(parts),))
+    sp = img.download().view(np.int32)
+    for b in (src, dst, mt):
+        b.free()
+    return out, matches, sp
 
 
 class DeviceBuffer:

@@ -817,6 +817,62 @@ __global__ __launch_bounds__(256) void select_points_kernel(detect_args d, int32
   select_points_body(d, pts, cap, blockIdx.x);
 }
 
+// Pixel claims with the serial last-writer rule of the container: after
+// claim_zero_kernel and then claim_max_kernel over the same particles [i0, i1)
+// (live ones only if live_only), each of their pixels holds the largest index
+// of those particles at that pixel.  Positions outside the domain claim nothing.
+__global__ __launch_bounds__(256) void claim_zero_kernel(const cuimg_particle_f* parts, uint32_t i0, uint32_t i1,
+                                                         int live_only, uint32_t* sp, int pitch, int nr, int nc) {
+  for (uint32_t i = i0 + blockIdx.x * blockDim.x + threadIdx.x; i < i1; i += gridDim.x * blockDim.x) {
+    const cuimg_particle_f p = parts[i];
+    const int r = tzf(p.pos_r), c = tzf(p.pos_c);
+    if ((p.age != 0 || !live_only) && r >= 0 && r < nr && c >= 0 && c < nc) *px<uint32_t>(sp, pitch, r, c) = 0;
+  }
+}
+
+__global__ __launch_bounds__(256) void claim_max_kernel(const cuimg_particle_f* parts, uint32_t i0, uint32_t i1,
+                                                        int live_only, uint32_t* sp, int pitch, int nr, int nc) {
+  for (uint32_t i = i0 + blockIdx.x * blockDim.x + threadIdx.x; i < i1; i += gridDim.x * blockDim.x) {
+    const cuimg_particle_f p = parts[i];
+    const int r = tzf(p.pos_r), c = tzf(p.pos_c);
+    if ((p.age != 0 || !live_only) && r >= 0 && r < nr && c >= 0 && c < nc) atomicMax(px<uint32_t>(sp, pitch, r, c), i);
+  }
+}
+
+// particle_container::add: particle n + k of age 1 and zero speed at point k.
+__global__ __launch_bounds__(256) void add_points_kernel(cuimg_particle_f* parts, uint32_t n, uint32_t m,
+                                                         const int32_t* pts) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+    cuimg_particle_f p;
+    p.speed_r = 0.f;
+    p.speed_c = 0.f;
+    p.pos_r = (float)pts[2 * (size_t)k];
+    p.pos_c = (float)pts[2 * (size_t)k + 1];
+    p.age = 1;
+    p.pad = 0;
+    parts[n + k] = p;
+  }
+}
+
+__global__ __launch_bounds__(256) void compact_count_kernel(detect_args d) { compact_count_body(d, blockIdx.x); }
+__global__ __launch_bounds__(1024) void compact_scan_kernel(detect_args d) { compact_scan_body(d); }
+__global__ __launch_bounds__(256) void compact_scatter_kernel(detect_args d) { compact_scatter_body(d, blockIdx.x); }
+
+static int launch_claims(cuimg_hip_ctx* ctx, const cuimg_particle_f* parts, uint32_t i0, uint32_t i1, int live_only,
+                         const cuimg_img* sparse) {
+  if (i1 <= i0) return 0;
+  const int grid = std::min(ceil_div((int)std::min<uint32_t>(i1 - i0, 1u << 30), 256), 4096);
+  uint32_t* sp = static_cast<uint32_t*>(sparse->begin);
+  const int pitch = (int)sparse->pitch, nr = sparse->nrows, nc = sparse->ncols;
+  hipStream_t st = ctx->stream;
+  CUIMG_TRY(launch(ctx, "claim_zero", [&] {
+    claim_zero_kernel<<<grid, 256, 0, st>>>(parts, i0, i1, live_only, sp, pitch, nr, nc);
+  }));
+  return launch(ctx, "claim_max", [&] {
+    claim_max_kernel<<<grid, 256, 0, st>>>(parts, i0, i1, live_only, sp, pitch, nr, nc);
+  });
+}
+
 }  // namespace cuimg
 
 using namespace cuimg;
@@ -1199,6 +1255,72 @@ int cuimg_hip_fast_select(cuimg_hip_ctx* ctx, const cuimg_img* sal, const cuimg_
   (void)hipStreamSynchronize(st);
   (void)hipFree(buf);
   if (!rc) *count = c.n_new;
+  return rc;
+}
+
+int cuimg_hip_particles_add(cuimg_hip_ctx* ctx, cuimg_particle_f* parts, uint32_t n, uint32_t cap,
+                            const int32_t* pts, uint32_t npts, const cuimg_img* sparse, uint32_t* n_out) {
+  if (!ctx || !parts || !n_out || (!pts && npts) || !sparse || sparse->dtype != CUIMG_U32 || !sparse->begin ||
+      n > cap || cap > kMaxParticles)
+    return set_error(ctx, CUIMG_E_INVALID, "particles_add: bad arguments");
+  const uint32_t m = std::min(npts, cap - n);
+  *n_out = n;
+  if (m) {
+    hipStream_t st = ctx->stream;
+    CUIMG_TRY(launch(ctx, "add_points", [&] {
+      add_points_kernel<<<std::min(ceil_div((int)m, 256), 4096), 256, 0, st>>>(parts, n, m, pts);
+    }));
+    CUIMG_TRY(launch_claims(ctx, parts, n, n + m, 0, sparse));
+  }
+  *n_out = n + m;
+  return 0;
+}
+
+int cuimg_hip_particles_compact(cuimg_hip_ctx* ctx, const cuimg_particle_f* parts, uint32_t n,
+                                cuimg_particle_f* out, uint32_t* matches, const cuimg_img* sparse, uint32_t* n_out) {
+  if (!ctx || !n_out || !sparse || sparse->dtype != CUIMG_U32 || !sparse->begin ||
+      (n && (!parts || !out || !matches)) || n > kMaxParticles)
+    return set_error(ctx, CUIMG_E_INVALID, "particles_compact: bad arguments");
+  const uintptr_t a0 = (uintptr_t)parts, a1 = a0 + (uintptr_t)n * sizeof(cuimg_particle_f);
+  const uintptr_t b0 = (uintptr_t)out, b1 = b0 + (uintptr_t)n * sizeof(cuimg_particle_f);
+  if (n && a0 < b1 && b0 < a1) return set_error(ctx, CUIMG_E_INVALID, "particles_compact: out overlaps the input");
+  *n_out = 0;
+  if (!n) return 0;
+  CUIMG_TRY(launch_claims(ctx, parts, 0, n, 1, sparse));
+  detect_args d{};
+  d.nr = sparse->nrows;
+  d.nc = sparse->ncols;
+  d.sparse = static_cast<uint32_t*>(sparse->begin);
+  d.sparse_pitch = (int)sparse->pitch;
+  d.parts = const_cast<cuimg_particle_f*>(parts);  // read only by the count and scatter passes
+  d.parts_out = out;
+  d.matches = matches;
+  d.cap = n;
+  d.nblocks = ceil_div((int)n, kCompactBlock);  // nbr = 0: the scan's sub-row pass is empty
+  const size_t off_blk = ((size_t)n + 255) & ~(size_t)255;
+  const size_t off_cnt = off_blk + (((size_t)d.nblocks * 4 + 255) & ~(size_t)255);
+  void* buf = nullptr;
+  CUIMG_HIP(ctx, hipMalloc(&buf, off_cnt + sizeof(level_counters)));
+  char* base = static_cast<char*>(buf);
+  d.flags = reinterpret_cast<uint8_t*>(base);
+  d.blockcnt = reinterpret_cast<uint32_t*>(base + off_blk);
+  d.cnt = reinterpret_cast<level_counters*>(base + off_cnt);
+  hipStream_t st = ctx->stream;
+  level_counters c{};
+  c.n = n;
+  int rc = 0;
+  const hipError_t e = hipMemcpyAsync(d.cnt, &c, sizeof c, hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) rc = hip_error(ctx, e, "hipMemcpyAsync", __FILE__, __LINE__);
+  if (!rc) rc = launch(ctx, "compact_count", [&] { compact_count_kernel<<<d.nblocks, 256, 0, st>>>(d); });
+  if (!rc) rc = launch(ctx, "compact_scan", [&] { compact_scan_kernel<<<1, 1024, 0, st>>>(d); });
+  if (!rc) rc = launch(ctx, "compact_scatter", [&] { compact_scatter_kernel<<<d.nblocks, 256, 0, st>>>(d); });
+  if (!rc) {
+    const hipError_t ec = hipMemcpyAsync(&c, d.cnt, sizeof c, hipMemcpyDeviceToHost, st);
+    if (ec != hipSuccess) rc = hip_error(ctx, ec, "hipMemcpyAsync", __FILE__, __LINE__);
+  }
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(buf);
+  if (!rc) *n_out = c.n_live;
   return rc;
 }
 

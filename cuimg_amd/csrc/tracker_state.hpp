@@ -101,6 +101,7 @@ struct merge_args {
   uint32_t cap, pend_cap;    // pend_cap: power of two >= cap; cap < 2^24
 };
 constexpr uint32_t kMergeMaxCap = (1u << 24) - 1;
+constexpr uint32_t kMaxParticles = 1u << 30;  // standalone container calls
 
 constexpr int kCompactBlock = 1024;  // particles per compaction block
 constexpr int kMaxBox = 15;          // largest detector box

@@ -252,6 +252,25 @@ int cuimg_hip_tracker_work(cuimg_hip_tracker* t, int scale, uint64_t* live, uint
 int cuimg_hip_particles_merge(cuimg_hip_ctx* ctx, cuimg_particle_f* parts, uint32_t n,
                               const cuimg_img* sparse);
 
+/* particle_container::add (tracking2/particle_container.hpp:370-387) for each
+ * point of pts (device array of npts (row, col) int32 pairs) in order, while the
+ * count stays below cap: particle n + k gets age 1, zero speed and the point as
+ * position, and sparse(point) = n + k (the last of equal points wins; points
+ * outside the sparse domain set no index).  parts: device array of cap entries;
+ * *n_out = n + number added.  Ordered on the context's stream. */
+int cuimg_hip_particles_add(cuimg_hip_ctx* ctx, cuimg_particle_f* parts, uint32_t n, uint32_t cap,
+                            const int32_t* pts, uint32_t npts, const cuimg_img* sparse, uint32_t* n_out);
+
+/* particle_container::compact(cpu) (tracking2/particle_container.hpp:216-252):
+ * the particles of `parts` with age != 0, in order, are written to `out` (a
+ * separate device array of n entries); matches[i] = new index of particle i or
+ * 0xFFFFFFFF; every kept particle's truncated position gets its new index in the
+ * U32 `sparse` image (the last kept particle of a pixel wins; other pixels are
+ * unchanged).  *n_out = kept count.  Synchronises the context. */
+int cuimg_hip_particles_compact(cuimg_hip_ctx* ctx, const cuimg_particle_f* parts, uint32_t n,
+                                cuimg_particle_f* out, uint32_t* matches, const cuimg_img* sparse,
+                                uint32_t* n_out);
+
 /* Frame steps are captured once into a hipGraph and replayed (default on);
  * disabled, every step launches its kernels (per-kernel timing needs this). */
 int cuimg_hip_tracker_set_graph(cuimg_hip_tracker* t, int enable);

@@ -127,6 +127,10 @@ class Oracle:
         L.orc_merge.argtypes = [vp, c_int, P(ctypes.c_int32), c_int, c_int]
         L.orc_particle_mask.argtypes = [vp, c_int, c_int, c_int, P(ctypes.c_uint8)]
         L.orc_fast_select.restype = c_int
+        L.orc_particles_add.restype = c_int
+        L.orc_particles_add.argtypes = [vp, c_int, c_int, P(ctypes.c_int32), c_int, c_int, P(ctypes.c_int32), c_int]
+        L.orc_particles_compact.restype = c_int
+        L.orc_particles_compact.argtypes = [vp, c_int, P(ctypes.c_int32), c_int, c_int, P(ctypes.c_uint32)]
         L.orc_fast_select.argtypes = [P(ctypes.c_uint8), P(ctypes.c_int32), c_int, c_int, c_int, c_float,
                                       P(ctypes.c_int32), c_int]
         L.orc_lk_loss_grad_f64.restype = c_double
@@ -243,6 +247,29 @@ class Oracle:
         n = self.L.orc_fast_select(_ptr(sal, ctypes.c_uint8), _ptr(sparse, ctypes.c_int32), nr, nc, box,
                                    float(th), _ptr(pts, ctypes.c_int32), cap)
         return pts[:min(n, cap)].copy()
+
+    def particles_add(self, parts, sparse, pts, cap=None):
+        """particle_container::add for each (row, col) of pts while below cap:
+        returns (particles, sparse)."""
+        parts = np.ascontiguousarray(parts, PARTICLE_DTYPE)
+        pts = np.ascontiguousarray(np.asarray(pts, np.int32).reshape(-1, 2))
+        cap = len(parts) + len(pts) if cap is None else cap
+        buf = np.zeros(max(cap, len(parts), 1), PARTICLE_DTYPE)
+        buf[:len(parts)] = parts
+        sp = np.array(sparse, np.int32, copy=True)
+        n = self.L.orc_particles_add(buf.ctypes.data, len(parts), cap, _ptr(sp, ctypes.c_int32), sp.shape[0],
+                                     sp.shape[1], _ptr(pts, ctypes.c_int32), len(pts))
+        return buf[:n].copy(), sp
+
+    def particles_compact(self, parts, sparse):
+        """particle_container::compact(cpu): returns (particles, matches, sparse)."""
+        buf = np.array(parts, PARTICLE_DTYPE, copy=True)
+        sp = np.array(sparse, np.int32, copy=True)
+        m = np.zeros(max(len(buf), 1), np.uint32)
+        n = self.L.orc_particles_compact(buf.ctypes.data if len(buf) else None, len(buf),
+                                         _ptr(sp, ctypes.c_int32), sp.shape[0], sp.shape[1],
+                                         _ptr(m, ctypes.c_uint32))
+        return buf[:n].copy(), m[:len(buf)].copy(), sp
 
     # ---------------------------------------------------------------- tracker
     def tracker(self, nr, nc, params):

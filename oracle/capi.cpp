@@ -229,6 +229,31 @@ int orc_fast_select(const uint8_t* sal, const int32_t* sparse, int nr, int nc, i
   return (int)v.size();
 }
 
+int orc_particles_add(orc_particle* parts, int n, int cap, int32_t* sparse, int nr, int nc, const int32_t* pts,
+                      int npts) {
+  std::vector<particle> v(n);
+  if (n) std::memcpy(v.data(), parts, sizeof(particle) * n);
+  image<int32_t> sp = wrap(sparse, nr, nc);
+  std::vector<std::pair<int, int>> q;
+  for (int k = 0; k < npts && n + k < cap; k++) q.emplace_back(pts[2 * k], pts[2 * k + 1]);
+  container_add(v, sp, q);
+  if (!v.empty()) std::memcpy(parts, v.data(), sizeof(particle) * v.size());
+  std::memcpy(sparse, sp.px.data(), sizeof(int32_t) * sp.px.size());
+  return (int)v.size();
+}
+
+int orc_particles_compact(orc_particle* parts, int n, int32_t* sparse, int nr, int nc, uint32_t* matches) {
+  std::vector<particle> v(n);
+  if (n) std::memcpy(v.data(), parts, sizeof(particle) * n);
+  image<int32_t> sp = wrap(sparse, nr, nc);
+  std::vector<uint32_t> m;
+  container_compact(v, sp, m);
+  if (!v.empty()) std::memcpy(parts, v.data(), sizeof(particle) * v.size());
+  if (n) std::memcpy(matches, m.data(), sizeof(uint32_t) * n);
+  std::memcpy(sparse, sp.px.data(), sizeof(int32_t) * sp.px.size());
+  return (int)v.size();
+}
+
 void orc_merge(orc_particle* parts, int n, const int32_t* sparse, int nr, int nc) {
   std::vector<particle> v(n);
   std::memcpy(v.data(), parts, sizeof(particle) * n);

@@ -92,6 +92,14 @@ void orc_particle_mask(const orc_particle* parts, int n, int nr, int nc, uint8_t
 int orc_fast_select(const uint8_t* sal, const int32_t* sparse, int nr, int nc, int box, float th, int32_t* pts,
                     int cap);
 
+/* particle_container::add for up to cap - n points (pts: npts (row, col)
+ * pairs), in place; returns the new particle count (particle_container.hpp:370-387). */
+int orc_particles_add(orc_particle* parts, int n, int cap, int32_t* sparse, int nr, int nc, const int32_t* pts,
+                      int npts);
+/* particle_container::compact(cpu) in place (particle_container.hpp:216-252);
+ * matches: n entries; returns the kept count. */
+int orc_particles_compact(orc_particle* parts, int n, int32_t* sparse, int nr, int nc, uint32_t* matches);
+
 /* FP64 photometric LK loss and its gradient (blurred frames, bilinear samples). */
 double orc_lk_loss_grad_f64(const float* I0, const float* I1, int nr, int nc, const float* coef,
                             int K, const float* pos2, const float* d2, int np, int ws,

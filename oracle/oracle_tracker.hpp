@@ -48,6 +48,10 @@ image<uint8_t> particle_mask(const std::vector<particle>& parts, int nr, int nc)
 // Box selection of new points, in the raster order they are added.
 std::vector<std::pair<int, int>> fast_select(const image<uint8_t>& sal, const image<int32_t>& sparse, int box,
                                              float th);
+// particle_container::add over a point list, and compact(cpu).
+void container_add(std::vector<particle>& parts, image<int32_t>& sparse,
+                   const std::vector<std::pair<int, int>>& pts);
+void container_compact(std::vector<particle>& parts, image<int32_t>& sparse, std::vector<uint32_t>& matches);
 
 struct tracker {
   tracker_params P;

@@ -217,33 +217,47 @@ std::vector<std::pair<int, int>> fast_select(const image<uint8_t>& sal, const im
   return out;
 }
 
-void tracker::new_particles(level_state& L) {
-  for (const auto& rc : fast_select(L.sal, L.sparse, P.box_size, P.fast_th)) {
-    // particle_container::add (tracking2/particle_container.hpp:370-387)
+// particle_container::add (tracking2/particle_container.hpp:370-387) for each
+// point in order: a particle of age 1 and zero speed at the point, sparse(point)
+// = its index.  Points outside the sparse image's domain write no index.
+void container_add(std::vector<particle>& parts, image<int32_t>& sparse,
+                   const std::vector<std::pair<int, int>>& pts) {
+  for (const auto& rc : pts) {
     particle p{};
     p.age = 1;
     p.pos_r = (float)rc.first;
     p.pos_c = (float)rc.second;
-    L.sparse(rc.first, rc.second) = (int32_t)L.parts.size();
-    L.parts.push_back(p);
+    if (sparse.has(rc.first, rc.second)) sparse(rc.first, rc.second) = (int32_t)parts.size();
+    parts.push_back(p);
   }
 }
 
-// particle_container::compact(cpu) (tracking2/particle_container.hpp:216-252).
-void tracker::compact(level_state& L) {
-  L.compact_has_run = true;
-  L.matches.assign(L.parts.size(), 0xFFFFFFFFu);
+void tracker::new_particles(level_state& L) {
+  container_add(L.parts, L.sparse, fast_select(L.sal, L.sparse, P.box_size, P.fast_th));
+}
+
+// particle_container::compact(cpu) (tracking2/particle_container.hpp:216-252):
+// particles with age != 0 keep their order; matches[old] = new index or
+// 0xFFFFFFFF; each kept particle writes its new index at its truncated position
+// (the last one of a pixel wins; positions outside the domain write nothing).
+void container_compact(std::vector<particle>& parts, image<int32_t>& sparse, std::vector<uint32_t>& matches) {
+  matches.assign(parts.size(), 0xFFFFFFFFu);
   size_t j = 0;
-  for (size_t i = 0; i < L.parts.size(); i++) {
-    if (L.parts[i].age != 0) {
-      particle p = L.parts[i];
-      L.parts[j] = p;
-      L.sparse(tz(p.pos_r), tz(p.pos_c)) = (int32_t)j;
-      L.matches[i] = (uint32_t)j;
+  for (size_t i = 0; i < parts.size(); i++) {
+    if (parts[i].age != 0) {
+      particle p = parts[i];
+      parts[j] = p;
+      if (sparse.has(tz(p.pos_r), tz(p.pos_c))) sparse(tz(p.pos_r), tz(p.pos_c)) = (int32_t)j;
+      matches[i] = (uint32_t)j;
       j++;
     }
   }
-  L.parts.resize(j);
+  parts.resize(j);
+}
+
+void tracker::compact(level_state& L) {
+  L.compact_has_run = true;
+  container_compact(L.parts, L.sparse, L.matches);
 }
 
 // merge_trajectories (tracking2/merge_trajectories.h:10-35) over the dense
