@@ -330,7 +330,14 @@ def gaussian_microbench(cu, ctx, reps=40):
     gbs = nbytes / us / 1e3
     out.append({"kernel": "pyramid_build", "image": "f32 4096x4096, 5 levels", "avg_us": round(us, 2),
                 "bytes": int(nbytes), "GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
-    for lv in pyrs:
+    # blur-then-subsample level (sigma 1, 13 taps) of the same image
+    decs = [ctx.image(2048, 2048, cu.F32) for _ in range(4)]
+    nbytes = 4 * 4096 * 4096 + 4 * 2048 * 2048
+    us = span_us(ctx, lambda i: ctx.pyr_down(pyrs[i % 4][0], decs[i % 4], cu.PYR_GAUSS_DECIMATE), reps)
+    gbs = nbytes / us / 1e3
+    out.append({"kernel": "pyr_gauss_decimate", "image": "f32 4096x4096 -> 2048x2048", "avg_us": round(us, 2),
+                "bytes": int(nbytes), "GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
+    for lv in pyrs + [decs]:
         for im in lv:
             im.free()
     return out

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libcuimg_hip.so")
 HEADER = os.path.join(os.path.dirname(_HERE), "include", "cuimg_hip.h")
 
 U8, F32, F32X4, U32, F32X2 = 1, 2, 3, 4, 5
+PYR_BOX2X2, PYR_GAUSS_DECIMATE = 0, 1
 _NP = {U8: (np.uint8, ()), F32: (np.float32, ()), F32X4: (np.float32, (4,)),
        U32: (np.uint32, ()), F32X2: (np.float32, (2,))}
 _ESIZE = {U8: 1, F32: 4, F32X4: 16, U32: 4, F32X2: 8}
@@ -108,7 +109,7 @@ def lib():
         L.cuimg_hip_convolve_sep.argtypes = [vp, P(Img), P(Img), P(ctypes.c_float), c_int]
         L.cuimg_hip_convolve_sep_th.argtypes = [vp, P(Img), P(Img), P(ctypes.c_float), c_int, c_int]
         L.cuimg_hip_gaussian_sep_multi.argtypes = [vp, P(Img), P(Img), c_int, c_int, c_int]
-        L.cuimg_hip_pyr_down.argtypes = [vp, P(Img), P(Img)]
+        L.cuimg_hip_pyr_down.argtypes = [vp, P(Img), P(Img), c_int]
         L.cuimg_hip_pyramid_build.argtypes = [vp, P(Img), P(Img), c_int]
         L.cuimg_hip_fast_saliency.argtypes = [vp, P(Img), P(Img), P(Img), c_int, c_int]
         L.cuimg_synth_video.argtypes = [ctypes.c_uint64, c_int, c_int, c_int, c_int, vp]
@@ -258,8 +259,9 @@ class Context:
         b = (Img * len(dsts))(*[d.d for d in dsts])
         self.check(self.L.cuimg_hip_gaussian_sep_multi(self.h, a, b, len(srcs), sigma, K), "gaussian_sep_multi")
 
-    def pyr_down(self, src, dst):
-        self.check(self.L.cuimg_hip_pyr_down(self.h, ctypes.byref(src.d), ctypes.byref(dst.d)), "pyr_down")
+    def pyr_down(self, src, dst, mode=0):
+        """One pyramid level: mode PYR_BOX2X2 (subsample) or PYR_GAUSS_DECIMATE."""
+        self.check(self.L.cuimg_hip_pyr_down(self.h, ctypes.byref(src.d), ctypes.byref(dst.d), mode), "pyr_down")
 
     def pyramid(self, frame, levels):
         a = (Img * len(levels))(*[l.d for l in levels])

@@ -3,6 +3,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+#include <utility>
+
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -122,6 +125,17 @@ __device__ __forceinline__ void store_nt(void* p, v2f_t v) {
 }
 __device__ __forceinline__ void store_nt(void* p, v4f_t v) {
   asm volatile("global_store_dwordx4 %0, %1, off nt\n\ts_nop 1" : : "v"(p), "v"(v) : "memory");
+}
+
+// Calls f(integral_constant<int, 0>) ... f(integral_constant<int, N-1>) in order,
+// so a loop body can use its index as a compile-time register index.
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on XCD

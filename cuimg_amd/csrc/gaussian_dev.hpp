@@ -49,17 +49,6 @@ struct gauss_jobs {
   int n;
 };
 
-// Calls f(integral_constant<int, 0>) ... f(integral_constant<int, N-1>) in order,
-// so a loop body can use its index as a compile-time register index.
-template <typename F, int... I>
-__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
-  (f(std::integral_constant<int, I>{}), ...);
-}
-template <int N, typename F>
-__device__ __forceinline__ void static_for(F&& f) {
-  static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
-
 // ---------------------------------------------------------------- elements
 __device__ __forceinline__ float fma_e(float a, float x, float s) { return fmaf(a, x, s); }
 __device__ __forceinline__ float4 fma_e(float a, float4 x, float4 s) {

@@ -3,6 +3,7 @@
 #include "pyramid_dev.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace cuimg {
 
@@ -65,6 +66,211 @@ int pyramid_build(cuimg_hip_ctx* ctx, const cuimg_img* frame, const cuimg_img* l
   });
 }
 
+// ------------------------------------------------- Gaussian then decimate
+// The blur-and-subsample pyramid of the reference's disabled build_mipmap
+// (cuimg/gpu/mipmap.h:94-109): local_jet_static<..., 0, 0, 1, 6> (sigma 1,
+// K = 6, clamp-to-edge, row pass then column pass, each sum from tap +K down to
+// tap -K) and the sample at 2p, out domain = in / 2 (floor).
+//
+// Fused: a tile is 16 x 64 output pixels.  Its (2*16 - 1 + 12)-row input
+// block is staged in LDS (columns 2*c0 - 8 .. 2*c0 + 136: whole 16-byte chunks
+// for aligned F32 images), rows are filtered at the even columns only, then
+// the even rows are filtered: no full-resolution intermediate reaches memory.
+// A workgroup walks down a run of kDecRun tiles of one tile column, loading the
+// next tile's block into registers while it filters the current one, so HBM
+// stays busy through the LDS phases; consecutive runs of a row of runs go to
+// one XCD, so horizontally adjacent tiles read their shared halo columns from
+// one L2, and a run's vertical halo rows were read by the run itself.
+constexpr int kDecK = 6, kDecTH = 16, kDecTW = 64, kDecRun = 4;
+constexpr int kDecInR = 2 * kDecTH - 1 + 2 * kDecK;  // input rows of a tile
+constexpr int kDecInC = 2 * kDecTW + 16;              // input columns, 16-byte aligned span
+constexpr int kDecQ = kDecInC / 4;                    // 16-byte chunks per input row
+constexpr int kDecInStride = kDecInC + 4;
+constexpr int kDecChunks = (kDecInR * kDecQ + 255) / 256;  // per thread
+
+struct dec_coefs {
+  float c[2 * kDecK + 1];
+};
+
+struct dec_args {
+  const void* in;
+  int in_pitch, nr, nc;
+  float* out;
+  int out_pitch, onr, onc;
+  int tiles_x, tiles_y, runs_y;
+  int vec_ok;
+  dec_coefs cf;
+};
+
+// Chunk k of this thread in the staged block of the tile whose input origin is
+// (ir0, ic0); rows and columns clamp to the domain.  Addresses are 32-bit byte
+// offsets from the image start (the image spans less than 4 GiB), so a load
+// holds one offset register next to the scalar base.
+//
+// dec_chunk_vec (aligned F32 image, ic0 a multiple of 4): one 16-byte load of
+// the chunk, or for a chunk reaching past either side, of the in-domain chunk
+// holding its clamped columns, from which each element is picked.
+__device__ __forceinline__ float4 dec_chunk_vec(const dec_args& a, int ir0, int ic0, int k, bool edge) {
+  const int i = k * 256 + threadIdx.x;
+  const int rr = i / kDecQ, q = i % kDecQ;
+  const uint32_t row = (uint32_t)clampi(ir0 + rr, 0, a.nr - 1) * (uint32_t)a.in_pitch;
+  const int c = ic0 + 4 * q;
+  const char* base = static_cast<const char*>(a.in);
+  if (!edge) return *reinterpret_cast<const float4*>(base + row + 4u * (uint32_t)c);
+  const int m = a.nc - 1, cl = clampi(c, 0, m & ~3);
+  const float4 v = *reinterpret_cast<const float4*>(base + row + 4u * (uint32_t)cl);
+  auto pick = [&](int j) {
+    const int e = clampi(c + j, 0, m) - cl;
+    return e == 0 ? v.x : (e == 1 ? v.y : (e == 2 ? v.z : v.w));
+  };
+  return make_float4(pick(0), pick(1), pick(2), pick(3));
+}
+
+// dec_chunk_clamped (any image): four column-clamped loads.
+template <typename TI>
+__device__ __forceinline__ float4 dec_chunk_clamped(const dec_args& a, int ir0, int ic0, int k) {
+  const int i = k * 256 + threadIdx.x;
+  const int rr = i / kDecQ, q = i % kDecQ;
+  const int c = ic0 + 4 * q, m = a.nc - 1;
+  const uint32_t row = (uint32_t)clampi(ir0 + rr, 0, a.nr - 1) * (uint32_t)a.in_pitch;
+  const char* base = static_cast<const char*>(a.in);
+  auto at = [&](int cc) {
+    return (float)*reinterpret_cast<const TI*>(base + row + (uint32_t)clampi(cc, 0, m) * (uint32_t)sizeof(TI));
+  };
+  return make_float4(at(c), at(c + 1), at(c + 2), at(c + 3));
+}
+
+__device__ __forceinline__ float4* dec_dst(float* tin, int k) {
+  const int i = k * 256 + threadIdx.x;
+  return reinterpret_cast<float4*>(&tin[(i / kDecQ) * kDecInStride + 4 * (i % kDecQ)]);
+}
+
+// The next tile's block is held in seven float4 registers p0..p6 of the
+// kernel, written out chunk by chunk (kept in an array or a struct, the block
+// went to scratch memory).
+#define CUIMG_DEC_CHUNKS(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6)
+static_assert(kDecChunks == 7, "seven chunks per thread");
+
+template <typename TI, bool VEC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void gauss_decimate_kernel(dec_args a) {
+  __shared__ __align__(16) float tin[kDecInR * kDecInStride];
+  __shared__ float trow[kDecInR * kDecTW];
+  const int run = xcd_order(blockIdx.x, gridDim.x);
+  const int tx = run % a.tiles_x, ty0 = (run / a.tiles_x) * kDecRun;
+  const int ty1 = min(ty0 + kDecRun, a.tiles_y);
+  const int c0 = tx * kDecTW;
+  float4 p0, p1, p2, p3, p4, p5, p6;
+  const bool last_ok = 6 * 256 + (int)threadIdx.x < kDecInR * kDecQ;
+  // VEC (aligned F32 image): 16-byte chunks, and the next tile's block is
+  // loaded during the filters; otherwise clamped loads when a block is due
+  const bool edge = 2 * c0 - 8 < 0 || 2 * c0 - 8 + kDecInC > a.nc;  // the same for the whole run
+#define CUIMG_DEC_LOAD_VEC(k) \
+  if (k < 6 || last_ok) p##k = dec_chunk_vec(a, 2 * (r0n) - kDecK, 2 * c0 - 8, k, edge);
+#define CUIMG_DEC_LOAD_CLAMPED(k) \
+  if (k < 6 || last_ok) p##k = dec_chunk_clamped<TI>(a, 2 * (r0n) - kDecK, 2 * c0 - 8, k);
+#define CUIMG_DEC_PUT(k) \
+  if (k < 6 || last_ok) *dec_dst(tin, k) = p##k;
+  if constexpr (VEC) {
+    const int r0n = ty0 * kDecTH;
+    CUIMG_DEC_CHUNKS(CUIMG_DEC_LOAD_VEC)
+  }
+  for (int ty = ty0; ty < ty1; ty++) {
+    const int r0 = ty * kDecTH;
+    if constexpr (!VEC) {
+      const int r0n = r0;
+      CUIMG_DEC_CHUNKS(CUIMG_DEC_LOAD_CLAMPED)
+    }
+    CUIMG_DEC_CHUNKS(CUIMG_DEC_PUT)
+    __syncthreads();
+    if constexpr (VEC) {
+      if (ty + 1 < ty1) {
+        const int r0n = r0 + kDecTH;
+        CUIMG_DEC_CHUNKS(CUIMG_DEC_LOAD_VEC)
+      }
+    }
+    // row pass: a thread filters two adjacent even columns 2*oc, 2*oc + 2
+    // (staged columns 2*oc + 8 and 2*oc + 10) from five 16-byte LDS reads
+    for (int i = threadIdx.x; i < kDecInR * (kDecTW / 2); i += 256) {
+      const int rr = i / (kDecTW / 2), j = i % (kDecTW / 2);
+      const float4* t4 = reinterpret_cast<const float4*>(&tin[rr * kDecInStride + 4 * j]);
+      float w[20];
+#pragma unroll
+      for (int q = 0; q < 5; q++) {
+        const float4 v = t4[q];
+        w[4 * q] = v.x;
+        w[4 * q + 1] = v.y;
+        w[4 * q + 2] = v.z;
+        w[4 * q + 3] = v.w;
+      }
+      // output column oc = 2j reads staged 4j + 2 .. 4j + 14, oc = 2j + 1 reads 4j + 4 .. 4j + 16
+      float acc0 = a.cf.c[2 * kDecK] * w[2 + 2 * kDecK];
+      float acc1 = a.cf.c[2 * kDecK] * w[4 + 2 * kDecK];
+#pragma unroll
+      for (int k = 2 * kDecK - 1; k >= 0; k--) {
+        acc0 = fmaf(a.cf.c[k], w[2 + k], acc0);
+        acc1 = fmaf(a.cf.c[k], w[4 + k], acc1);
+      }
+      trow[rr * kDecTW + 2 * j] = acc0;
+      trow[rr * kDecTW + 2 * j + 1] = acc1;
+    }
+    __syncthreads();
+    // column pass: a thread filters output rows 2m and 2m + 1 of one column
+    for (int i = threadIdx.x; i < (kDecTH / 2) * kDecTW; i += 256) {
+      const int m = i / kDecTW, oc = i % kDecTW;
+      const float* t = &trow[4 * m * kDecTW + oc];
+      float w[2 * kDecK + 3];
+#pragma unroll
+      for (int k = 0; k < 2 * kDecK + 3; k++) w[k] = t[k * kDecTW];
+      float acc0 = a.cf.c[2 * kDecK] * w[2 * kDecK];
+      float acc1 = a.cf.c[2 * kDecK] * w[2 * kDecK + 2];
+#pragma unroll
+      for (int k = 2 * kDecK - 1; k >= 0; k--) {
+        acc0 = fmaf(a.cf.c[k], w[k], acc0);
+        acc1 = fmaf(a.cf.c[k], w[k + 2], acc1);
+      }
+      const int r = r0 + 2 * m, c = c0 + oc;
+      if (c < a.onc) {
+        if (r < a.onr) *px<float>(a.out, a.out_pitch, r, c) = acc0;
+        if (r + 1 < a.onr) *px<float>(a.out, a.out_pitch, r + 1, c) = acc1;
+      }
+    }
+  }
+}
+#undef CUIMG_DEC_LOAD_VEC
+#undef CUIMG_DEC_LOAD_CLAMPED
+#undef CUIMG_DEC_PUT
+
+static int gauss_decimate(cuimg_hip_ctx* ctx, const cuimg_img* in, const cuimg_img* out) {
+  if ((in->dtype != CUIMG_U8 && in->dtype != CUIMG_F32) || out->dtype != CUIMG_F32)
+    return set_error(ctx, CUIMG_E_UNSUPPORTED, "pyr_down: Gaussian decimation reads U8 or F32 and writes F32");
+  if (out->nrows != in->nrows / 2 || out->ncols != in->ncols / 2)
+    return set_error(ctx, CUIMG_E_INVALID, "pyr_down: output is %dx%d, expected %dx%d", out->nrows, out->ncols,
+                     in->nrows / 2, in->ncols / 2);
+  dec_args a{};
+  CUIMG_TRY(gaussian_coefs(1, kDecK, a.cf.c));
+  a.in = in->begin;
+  a.in_pitch = in->pitch;
+  a.nr = in->nrows;
+  a.nc = in->ncols;
+  a.out = static_cast<float*>(out->begin);
+  a.out_pitch = out->pitch;
+  a.onr = out->nrows;
+  a.onc = out->ncols;
+  if (a.onr == 0 || a.onc == 0) return 0;
+  if ((uint64_t)in->nrows * (uint64_t)in->pitch >= (1ull << 32))
+    return set_error(ctx, CUIMG_E_UNSUPPORTED, "pyr_down: Gaussian decimation needs an input below 4 GiB");
+  a.tiles_x = ceil_div(a.onc, kDecTW);
+  a.tiles_y = ceil_div(a.onr, kDecTH);
+  a.runs_y = ceil_div(a.tiles_y, kDecRun);
+  a.vec_ok = aligned16(in->begin, in->pitch);
+  const int blocks = a.tiles_x * a.runs_y;
+  return launch(ctx, "pyr_gauss_decimate", [&] {
+    if (in->dtype == CUIMG_U8) gauss_decimate_kernel<uint8_t, false><<<blocks, 256, 0, ctx->stream>>>(a);
+    else if (a.vec_ok) gauss_decimate_kernel<float, true><<<blocks, 256, 0, ctx->stream>>>(a);
+    else gauss_decimate_kernel<float, false><<<blocks, 256, 0, ctx->stream>>>(a);
+  });
+}
+
 }  // namespace cuimg
 
 using namespace cuimg;
@@ -76,7 +282,10 @@ int cuimg_hip_pyramid_build(cuimg_hip_ctx* ctx, const cuimg_img* frame, const cu
   return pyramid_build(ctx, frame, levels, nlevels);
 }
 
-int cuimg_hip_pyr_down(cuimg_hip_ctx* ctx, const cuimg_img* in, const cuimg_img* out) {
+int cuimg_hip_pyr_down(cuimg_hip_ctx* ctx, const cuimg_img* in, const cuimg_img* out, int mode) {
+  if (!in || !out || !in->begin || !out->begin) return set_error(ctx, CUIMG_E_INVALID, "pyr_down: bad arguments");
+  if (mode == CUIMG_PYR_GAUSS_DECIMATE) return gauss_decimate(ctx, in, out);
+  if (mode != CUIMG_PYR_BOX2X2) return set_error(ctx, CUIMG_E_INVALID, "pyr_down: mode %d unknown", mode);
   // a two-level pyramid whose level 0 is the input itself
   // subsample writes the output domain only: neither border is touched
   cuimg_img lv[2] = {*in, *out};

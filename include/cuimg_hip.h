@@ -124,9 +124,17 @@ int cuimg_hip_gaussian_sep_multi(cuimg_hip_ctx* ctx, const cuimg_img* ins, const
                                  int n, int sigma, int half_width);
 
 /* ---------------------------------------------------------------- pyramid */
-/* subsample (cuimg/gpu/mipmap.h:206-227, kernel :25-50): 2x2 box mean, integer
- * division for U8; out domain = in / 2 (floor).  U8 or F32. */
-int cuimg_hip_pyr_down(cuimg_hip_ctx* ctx, const cuimg_img* in, const cuimg_img* out);
+enum {
+  /* subsample (cuimg/gpu/mipmap.h:206-227, kernel :25-50): 2x2 box mean, integer
+   * division for U8.  U8 or F32 in and out. */
+  CUIMG_PYR_BOX2X2 = 0,
+  /* build_mipmap's blur-then-subsample level (cuimg/gpu/mipmap.h:94-109,
+   * disabled in the reference): Gaussian sigma 1, K = 6 with clamp-to-edge,
+   * sampled at 2p.  U8 or F32 in, F32 out. */
+  CUIMG_PYR_GAUSS_DECIMATE = 1
+};
+/* One pyramid level: out domain = in / 2 (floor); mode is a CUIMG_PYR_* value. */
+int cuimg_hip_pyr_down(cuimg_hip_ctx* ctx, const cuimg_img* in, const cuimg_img* out, int mode);
 /* tracker::update_input (tracking2/tracker.hpp:53-69) fused: copies `frame`
  * into levels[0], builds levels[1..n-1] with pyr_down and fills every level's
  * border by clamping.  U8 (frame may be levels[0] itself) or F32 (no border

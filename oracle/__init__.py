@@ -187,6 +187,13 @@ class Oracle:
             self.L.orc_pyr_down_f32(_ptr(img, ctypes.c_float), nr, nc, _ptr(out, ctypes.c_float))
         return out
 
+    def pyr_down_gauss(self, img):
+        """build_mipmap's blur-then-subsample level (cuimg/gpu/mipmap.h:94-109):
+        the sigma-1, K-6 Gaussian of the image sampled at 2p, FP32."""
+        nr, nc = img.shape
+        g = self.gaussian(img, self.coefs(1, 6), 6)
+        return np.ascontiguousarray(g[0:2 * (nr // 2):2, 0:2 * (nc // 2):2])
+
     # ---------------------------------------------------------------- FAST
     def fast_score(self, img, r, c, n=9, th=20):
         img = np.ascontiguousarray(img, np.uint8)

@@ -179,6 +179,32 @@ def test_pyr_down_leaves_borders(ctx, orc):
     assert np.array_equal(dst.download(), orc.pyr_down(img))
 
 
+@pytest.mark.parametrize("shape", [(2, 2), (3, 5), (17, 13), (50, 70), (131, 257), (1080, 1920), (4096, 4096)])
+@pytest.mark.parametrize("kind", ["u8", "f32"])
+def test_pyr_down_gauss_decimate(ctx, orc, shape, kind):
+    print(f"gauss decimate {kind} {shape}", flush=True)
+    rng = np.random.default_rng(shape[0] * 31 + shape[1])
+    if kind == "u8":
+        img = rng.integers(0, 256, shape, dtype=np.uint8)
+        src = ctx.upload(img, border=3)
+    else:
+        img = rng.uniform(0, 255, shape).astype(np.float32)
+        src = ctx.upload(img)
+    dst = ctx.image(shape[0] // 2, shape[1] // 2, cu.F32)
+    ctx.pyr_down(src, dst, cu.PYR_GAUSS_DECIMATE)
+    assert rel_err(dst.download(), orc.pyr_down_gauss(img)) < 1e-5
+
+
+def test_pyr_down_mode_errors(ctx):
+    src = ctx.upload(np.zeros((20, 20), np.uint8))
+    with pytest.raises(cu.CuimgError):  # U8 output
+        ctx.pyr_down(src, ctx.image(10, 10, cu.U8), cu.PYR_GAUSS_DECIMATE)
+    with pytest.raises(cu.CuimgError):  # wrong size
+        ctx.pyr_down(src, ctx.image(9, 10, cu.F32), cu.PYR_GAUSS_DECIMATE)
+    with pytest.raises(cu.CuimgError):
+        ctx.pyr_down(src, ctx.image(10, 10, cu.U8), 7)
+
+
 @pytest.mark.parametrize("dtype", [cu.U8, cu.U32, cu.F32X2, cu.F32X4])
 def test_fill_border_clamp(ctx, dtype):
     np_t, tail = {cu.U8: (np.uint8, ()), cu.U32: (np.uint32, ()), cu.F32X2: (np.float32, (2,)),

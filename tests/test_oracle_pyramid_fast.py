@@ -128,3 +128,24 @@ def test_saliency_respects_mask_and_border(orc):
         for c in range(50):
             want = 0 if mask[r, c] == 0 else py_fast(img, r, c, 9, 20)
             assert sal[r, c] == want
+
+
+def test_pyr_down_gauss_sampling(orc):
+    """Blur-then-subsample level: a constant image gives c * sum(coef)^2, and an
+    impulse at (r0, c0) gives coef[2i - r0 + 6] * coef[2j - c0 + 6] (the
+    separable impulse response sampled at even pixels)."""
+    cf = orc.coefs(1, 6).astype(np.float64)
+    out = orc.pyr_down_gauss(np.full((40, 36), 10.0, np.float32))
+    assert out.shape == (20, 18)
+    assert np.allclose(out, 10.0 * cf.sum() ** 2, rtol=1e-6)
+    for r0, c0 in [(20, 18), (21, 17)]:
+        img = np.zeros((41, 37), np.float32)
+        img[r0, c0] = 1.0
+        out = orc.pyr_down_gauss(img).astype(np.float64)
+        want = np.zeros_like(out)
+        for i in range(out.shape[0]):
+            for j in range(out.shape[1]):
+                a, b = 2 * i - r0 + 6, 2 * j - c0 + 6
+                if 0 <= a <= 12 and 0 <= b <= 12:
+                    want[i, j] = cf[a] * cf[b]
+        assert np.allclose(out, want, rtol=1e-6, atol=1e-12)
