@@ -66,30 +66,48 @@ __device__ inline void fast_tile_body(const uint8_t* in, int in_pitch, const uin
   constexpr int SW = kFTW + 2 * kFH, SH = kFTH + 2 * kFH;
   __shared__ uint8_t t[SH * SW];
   const int r0 = (tile / tiles_x) * kFTH, c0 = (tile % tiles_x) * kFTW;
-  for (int i = threadIdx.x; i < SH * SW; i += blockDim.x) {
-    int r = r0 - kFH + i / SW, c = c0 - kFH + i % SW;
+  const int lc = threadIdx.x % kFTW, lr0 = threadIdx.x / kFTW;
+  const int c = c0 + lc;
+  // the mask bytes of this thread's pixels, loaded with the tile (one memory
+  // round trip for both)
+  uint8_t mk[kFTH / 4];
+#pragma unroll
+  for (int k = 0; k < kFTH / 4; k++) {
+    const int r = r0 + lr0 + 4 * k;
+    mk[k] = (!mask || r >= nr || c >= nc) ? 1 : *px<uint8_t>(mask, mask_pitch, r, c);
+  }
+  // the tile with its halo: every load of the thread issued before the first
+  // LDS write (one memory round trip)
+  constexpr int NL = (SH * SW + 255) / 256;
+  uint8_t v[NL];
+#pragma unroll
+  for (int k = 0; k < NL; k++) {
+    const int i = k * 256 + threadIdx.x;
+    int rr = r0 - kFH + i / SW, cc = c0 - kFH + i % SW;
     if (CLAMP) {
-      r = clampi(r, 0, nr - 1);
-      c = clampi(c, 0, nc - 1);
+      rr = clampi(rr, 0, nr - 1);
+      cc = clampi(cc, 0, nc - 1);
     } else {
       // the image border (>= 3, clamp-filled) covers the ring; rows/cols past
       // a partial tile are clamped so they stay inside the allocation
-      r = min(r, nr + kFH - 1);
-      c = min(c, nc + kFH - 1);
+      rr = min(rr, nr + kFH - 1);
+      cc = min(cc, nc + kFH - 1);
     }
-    t[i] = *px<uint8_t>(in, in_pitch, r, c);
+    if (i < SH * SW) v[k] = *px<uint8_t>(in, in_pitch, rr, cc);
+  }
+#pragma unroll
+  for (int k = 0; k < NL; k++) {
+    const int i = k * 256 + threadIdx.x;
+    if (i < SH * SW) t[i] = v[k];
   }
   __syncthreads();
-  const int lc = threadIdx.x % kFTW, lr0 = threadIdx.x / kFTW;
-  const int c = c0 + lc;
   if (c >= nc) return;
 #pragma unroll
   for (int k = 0; k < kFTH / 4; k++) {
     const int lr = lr0 + 4 * k, r = r0 + lr;
     if (r >= nr) break;
     uint8_t s = 0;
-    if (!mask || *px<uint8_t>(mask, mask_pitch, r, c) != 0)
-      s = fast_score_lds(&t[(lr + kFH) * SW + lc + kFH], SW, n, th);
+    if (mk[k] != 0) s = fast_score_lds(&t[(lr + kFH) * SW + lc + kFH], SW, n, th);
     *px<uint8_t>(out, out_pitch, r, c) = s;
   }
 }

@@ -139,9 +139,78 @@ __device__ __forceinline__ uint32_t sparse_get(const uint32_t* sp, int pitch, in
   return *px<uint32_t>(sp, pitch, r, c);
 }
 
-// One workgroup per box row bi; each thread one box at a time.  code =
-// dr*box + dc of the selected point, 0xFF when the box yields nothing.  The
-// row's points are counted per sub-row (row offset dr) into subcnt.
+// Box at (pr, pc): code dr*box + dc of the selected point, 0xFF when the box
+// yields nothing.  Pixels outside the domain read as empty (sparse) and 0
+// (saliency); a 0 never beats a maximum, which starts at the box's first
+// pixel (the saliency border is 0, reached only when box == 1).
+__device__ __forceinline__ uint8_t nms_code(const detect_args& a, float fast_th, int pr, int pc, int B, float vmax,
+                                            int best) {
+  const int mr = pr + best / B, mc = pc + best % B;
+  if (!(vmax < fast_th) && mr >= 3 && mr <= a.nr - 4 && mc >= 3 && mc <= a.nc - 4) return (uint8_t)best;
+  return 0xFF;
+}
+
+// Box side B <= 6: every sparse and saliency read of the box is issued before
+// the first comparison (one memory round trip per box).
+template <int B>
+__device__ __forceinline__ uint8_t nms_box_fixed(const detect_args& a, float fast_th, int pr, int pc) {
+  uint32_t sp[B * B];
+  uint32_t sv[B * B];
+#pragma unroll
+  for (int r = 0; r < B; r++)
+#pragma unroll
+    for (int c = 0; c < B; c++) {
+      const bool in = pr + r < a.nr && pc + c < a.nc;
+      sp[r * B + c] = in ? *px<uint32_t>(a.sparse, a.sparse_pitch, pr + r, pc + c) : kEmpty;
+      sv[r * B + c] = in ? *px<uint8_t>(a.sal, a.sal_pitch, pr + r, pc + c) : 0u;
+    }
+  bool skip = false;
+#pragma unroll
+  for (int k = 0; k < B * B; k++) skip |= sp[k] != kEmpty;
+  if (skip) return 0xFF;
+  float vmax = (float)sv[0];
+  int best = 0;
+#pragma unroll
+  for (int k = 1; k < B * B; k++) {
+    const float v = (float)sv[k];
+    if (vmax < v) {
+      vmax = v;
+      best = k;
+    }
+  }
+  return nms_code(a, fast_th, pr, pc, B, vmax, best);
+}
+
+// Any box side: one row of the box per memory round trip.
+__device__ inline uint8_t nms_box_rows(const detect_args& a, float fast_th, int pr, int pc, int B) {
+  bool skip = false;
+  float vmax = 0.f;
+  int best = 0;
+  for (int r = 0; r < B; r++) {
+    uint32_t sp[kMaxBox], sv[kMaxBox];
+#pragma unroll
+    for (int c = 0; c < kMaxBox; c++) {
+      const bool in = c < B && pr + r < a.nr && pc + c < a.nc;
+      sp[c] = in ? *px<uint32_t>(a.sparse, a.sparse_pitch, pr + r, pc + c) : kEmpty;
+      sv[c] = in ? *px<uint8_t>(a.sal, a.sal_pitch, pr + r, pc + c) : 0u;
+    }
+#pragma unroll
+    for (int c = 0; c < kMaxBox; c++) {
+      skip |= sp[c] != kEmpty;
+      const float v = (float)sv[c];
+      if (r == 0 && c == 0) {
+        vmax = v;
+      } else if (c < B && vmax < v) {
+        vmax = v;
+        best = r * B + c;
+      }
+    }
+  }
+  return skip ? (uint8_t)0xFF : nms_code(a, fast_th, pr, pc, B, vmax, best);
+}
+
+// One workgroup per box row bi; each thread one box at a time.  The row's
+// points are counted per sub-row (row offset dr) into subcnt.
 __device__ inline void nms_row_body(const detect_args& a, float fast_th, int bi) {
   __shared__ uint32_t cnts[kMaxBox];
   const int B = a.box;
@@ -149,31 +218,15 @@ __device__ inline void nms_row_body(const detect_args& a, float fast_th, int bi)
   __syncthreads();
   for (int bj = threadIdx.x; bj < a.nbc; bj += blockDim.x) {
     const int pr = bi * B + 1, pc = bj * B + 1;
-    uint8_t code = 0xFF;
-    bool skip = false;
-    for (int r = 0; r < B && !skip; r++)
-      for (int c = 0; c < B; c++)
-        if (pr + r < a.nr && pc + c < a.nc &&
-            *px<uint32_t>(a.sparse, a.sparse_pitch, pr + r, pc + c) != kEmpty) {
-          skip = true;
-          break;
-        }
-    if (!skip) {
-      // the saliency border is 0 (its memset), reached only when box == 1
-      float vmax = (pr < a.nr && pc < a.nc) ? (float)*px<uint8_t>(a.sal, a.sal_pitch, pr, pc) : 0.f;
-      int mr = pr, mc = pc;
-      for (int r = 0; r < B; r++)
-        for (int c = 0; c < B; c++)
-          if (pr + r < a.nr && pc + c < a.nc) {
-            const float sv = (float)*px<uint8_t>(a.sal, a.sal_pitch, pr + r, pc + c);
-            if (vmax < sv) {
-              vmax = sv;
-              mr = pr + r;
-              mc = pc + c;
-            }
-          }
-      if (!(vmax < fast_th) && mr >= 3 && mr <= a.nr - 4 && mc >= 3 && mc <= a.nc - 4)
-        code = (uint8_t)((mr - pr) * B + (mc - pc));
+    uint8_t code;
+    switch (B) {
+      case 1: code = nms_box_fixed<1>(a, fast_th, pr, pc); break;
+      case 2: code = nms_box_fixed<2>(a, fast_th, pr, pc); break;
+      case 3: code = nms_box_fixed<3>(a, fast_th, pr, pc); break;
+      case 4: code = nms_box_fixed<4>(a, fast_th, pr, pc); break;
+      case 5: code = nms_box_fixed<5>(a, fast_th, pr, pc); break;
+      case 6: code = nms_box_fixed<6>(a, fast_th, pr, pc); break;
+      default: code = nms_box_rows(a, fast_th, pr, pc, B); break;
     }
     a.boxcode[(size_t)bi * a.nbc + bj] = code;
     if (code != 0xFF) atomicAdd(&cnts[code / B], 1u);
