@@ -116,7 +116,8 @@ __device__ inline void fill_u8_body(uint8_t* begin, int pitch, int nr, int nc, u
 }
 
 // ------------------------------------------------------------ detector mask
-// Clears the 3x3 block of each of the n particles (the mask already holds 255).
+// Zeroes the 3x3 block around the truncated position of each of the n
+// particles in a u8 image (a detector mask that holds 255, or a saliency).
 __device__ inline void particle_mask_core(const cuimg_particle_f* parts, uint32_t n, uint8_t* mask, int pitch,
                                           int nr, int nc, int bid, int nb) {
   for (uint32_t i = bid * blockDim.x + threadIdx.x; i < n; i += nb * blockDim.x) {
@@ -127,10 +128,6 @@ __device__ inline void particle_mask_core(const cuimg_particle_f* parts, uint32_
         if (rr >= 0 && rr < nr && cc >= 0 && cc < nc) *px<uint8_t>(mask, pitch, rr, cc) = 0;
       }
   }
-}
-
-__device__ inline void particle_mask_body(const detect_args& a, int bid, int nb) {
-  particle_mask_core(a.parts, a.cnt->n, a.mask, a.mask_pitch, a.nr, a.nc, bid, nb);
 }
 
 // ---------------------------------------------------------- box selection

@@ -43,7 +43,7 @@ struct level_res {
   int nr = 0, nc = 0;
   cuimg_img u8[2]{};
   cuimg_img blur[2]{};
-  cuimg_img sal{}, mask{}, sparse{};
+  cuimg_img sal{}, sparse{};
   uint8_t* boxcode = nullptr;
   uint32_t* subcnt = nullptr;
   cuimg_particle_f* parts[2] = {nullptr, nullptr};
@@ -71,13 +71,11 @@ struct level_job {
   level_counters* cnt;
   uint8_t* u8[2];
   int u8_pitch;
-  uint8_t* mask;
-  int mask_pitch;
   uint8_t* sal;
   int sal_pitch;
   uint32_t* sparse;
   int sparse_pitch, sparse_border;
-  int fill_sparse_blocks;  // first blocks of the fills launch: sparse reset; the rest: mask
+  int fill_sparse_blocks;  // blocks of the sparse reset
   int fast_tiles_x;
   lk_args lk[4];        // [slot * 2 + cur]
   detect_args det[2];   // [cur]
@@ -116,6 +114,10 @@ struct cuimg_hip_batch {
   cuimg::frame_in* din = nullptr;            // device input table
   cuimg::frame_in* hin = nullptr;            // pinned ring of input tables
   hipEvent_t ring_ev[cuimg::kRing]{};
+  // FAST runs on a second stream (default priority) beside the matcher:
+  // forked after the pyramid, joined before the saliency is masked
+  hipStream_t aux = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   uint64_t steps = 0;
   std::vector<uint32_t> prefix_h[cuimg::K_NKINDS];
   uint32_t* prefix_d[cuimg::K_NKINDS]{};   // prefix + block -> job map per kind
@@ -218,15 +220,9 @@ __global__ __launch_bounds__(64) void step_blur_kernel(const level_job* jobs, co
 __global__ __launch_bounds__(256) void step_fill_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
   CUIMG_JOB(prefix, njobs);
   const step_flags f = flags_of(J);
-  const int F = J.fill_sparse_blocks;
-  if (lb < F) {
-    if (f.match) {
-      const int b = J.sparse_border;
-      fill_u32_body(J.sparse, J.sparse_pitch, -b, J.nr + b, -b, J.nc + b, kEmpty, lb, F);
-    }
-  } else if (f.detect) {
-    fill_u8_body(J.mask, J.mask_pitch, J.nr, J.nc, 0xFFFFFFFFu, lb - F, nb - F);
-  }
+  if (!f.match) return;
+  const int b = J.sparse_border;
+  fill_u32_body(J.sparse, J.sparse_pitch, -b, J.nr + b, -b, J.nc + b, kEmpty, lb, nb);
 }
 
 template <typename TI, int NPL, bool SEL>
@@ -242,15 +238,18 @@ __global__ __launch_bounds__(256) void step_pmask_kernel(const level_job* jobs, 
   CUIMG_JOB(prefix, njobs);
   const step_flags f = flags_of(J);
   if (!f.detect) return;
+  // the detector mask applied to the saliency: the score is per pixel, so
+  // scoring every pixel and then zeroing the 3x3 block of every particle gives
+  // the masked saliency
   const detect_args d = uni(J.det[f.cur]);
-  particle_mask_body(d, lb, nb);
+  particle_mask_core(d.parts, d.cnt->n, J.sal, J.sal_pitch, d.nr, d.nc, lb, nb);
 }
 
 __global__ __launch_bounds__(256) void step_fast_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
   CUIMG_JOB(prefix, njobs);
   const step_flags f = flags_of(J);
   if (!f.detect) return;
-  fast_tile_body<false>(J.u8[f.slot], J.u8_pitch, J.mask, J.mask_pitch, J.sal, J.sal_pitch, J.nr, J.nc, lb,
+  fast_tile_body<false>(J.u8[f.slot], J.u8_pitch, nullptr, 0, J.sal, J.sal_pitch, J.nr, J.nc, lb,
                         J.fast_tiles_x, J.prm.fast_n, (int)J.prm.fast_th);
 }
 
@@ -344,7 +343,6 @@ static void destroy_batch(cuimg_hip_batch* b) {
       image_free(b->ctx, &R.blur[k]);
     }
     image_free(b->ctx, &R.sal);
-    image_free(b->ctx, &R.mask);
     image_free(b->ctx, &R.sparse);
   }
   for (auto& f : b->frame_buf) image_free(b->ctx, &f);
@@ -352,6 +350,9 @@ static void destroy_batch(cuimg_hip_batch* b) {
   if (b->hin) (void)hipHostFree(b->hin);
   for (auto& e : b->ring_ev)
     if (e) (void)hipEventDestroy(e);
+  if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
+  if (b->join_ev) (void)hipEventDestroy(b->join_ev);
+  if (b->aux) (void)hipStreamDestroy(b->aux);
   for (auto* v : b->views) delete v;
   delete b;
 }
@@ -401,7 +402,6 @@ static int alloc_level(cuimg_hip_batch* b, const cuimg_tracker_params* P, int r,
     if (P->profile == 1) CUIMG_TRY(image_alloc(ctx, r, c, 0, CUIMG_F32, &R.blur[k]));
   }
   CUIMG_TRY(image_alloc(ctx, r, c, 1, CUIMG_U8, &R.sal));
-  CUIMG_TRY(image_alloc(ctx, r, c, 1, CUIMG_U8, &R.mask));
   CUIMG_TRY(image_alloc(ctx, r, c, 7, CUIMG_U32, &R.sparse));
   R.nbr = (int)((float)r / (float)P->box_size);
   R.nbc = (int)((float)c / (float)P->box_size);
@@ -457,8 +457,6 @@ static void build_job(cuimg_hip_batch* b, int i, int s) {
   J.cnt = b->cnt + j;
   for (int k = 0; k < 2; k++) J.u8[k] = static_cast<uint8_t*>(R.u8[k].begin);
   J.u8_pitch = R.u8[0].pitch;
-  J.mask = static_cast<uint8_t*>(R.mask.begin);
-  J.mask_pitch = R.mask.pitch;
   J.sal = static_cast<uint8_t*>(R.sal.begin);
   J.sal_pitch = R.sal.pitch;
   J.sparse = static_cast<uint32_t*>(R.sparse.begin);
@@ -497,8 +495,6 @@ static void build_job(cuimg_hip_batch* b, int i, int s) {
     d.fast_th = R.sp.fast_th;
     d.sal = J.sal;
     d.sal_pitch = J.sal_pitch;
-    d.mask = J.mask;
-    d.mask_pitch = J.mask_pitch;
     d.sparse = J.sparse;
     d.sparse_pitch = J.sparse_pitch;
     d.parts = R.parts[cur];
@@ -571,8 +567,7 @@ static int build_maps(cuimg_hip_batch* b) {
     const level_job& J = b->hj[j];
     const long cap = R.cap;
     cnt[K_BLUR].push_back(b->P0.profile == 1 ? (uint32_t)(ceil_div(R.nc, 256) * ceil_div(R.nr, th)) : 0u);
-    const int G = std::max(1, std::min(1024, R.nr));
-    cnt[K_FILL].push_back((uint32_t)(J.fill_sparse_blocks + G));
+    cnt[K_FILL].push_back((uint32_t)J.fill_sparse_blocks);
     cnt[K_PMASK].push_back((uint32_t)std::max<long>(1, std::min<long>(512, (cap + 255) / 256)));
     cnt[K_FAST].push_back((uint32_t)(J.fast_tiles_x * ceil_div(R.nr, kFTH)));
     cnt[K_DETC].push_back((uint32_t)(R.nbr + R.nblocks));
@@ -688,6 +683,10 @@ static int create_batch(cuimg_hip_ctx* ctx, int nstreams, const int* nrows, cons
   for (auto& e : b->ring_ev)
     if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
       return fail(set_error(ctx, CUIMG_E_NOMEM, "batch: event creation failed"));
+  if (hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking) != hipSuccess)
+    return fail(set_error(ctx, CUIMG_E_NOMEM, "batch: stream or event creation failed"));
   if ((rc = rebuild(b))) return fail(rc);
   *out = b;
   return 0;
@@ -706,7 +705,7 @@ static void launch_blur(cuimg_hip_batch* b) {
 }
 
 // The fixed launch sequence of one frame step of every stream.
-static int enqueue_step(cuimg_hip_batch* b) {
+static int enqueue_step(cuimg_hip_batch* b, bool capturing) {
   cuimg_hip_ctx* ctx = b->ctx;
   hipStream_t st = ctx->stream;
   const int nj = b->njobs;
@@ -715,6 +714,18 @@ static int enqueue_step(cuimg_hip_batch* b) {
   CUIMG_TRY(launch(ctx, "pyramid_build", [&] {
     step_pyramid_kernel<<<b->total[K_PYR], kPyrThreads, 0, st>>>(b->ds, J, b->din, pf[K_PYR], b->nstreams);
   }));
+  // FAST needs only the pyramid: in a captured step it runs beside the matcher
+  // on the second stream; plain launches (and per-launch timing on the context
+  // stream) keep it in order, where two more stream calls per step would only
+  // add host time
+  const bool fork = capturing;
+  hipStream_t fst = fork ? b->aux : st;
+  if (fork) {
+    CUIMG_HIP(ctx, hipEventRecord(b->fork_ev, st));
+    CUIMG_HIP(ctx, hipStreamWaitEvent(b->aux, b->fork_ev, 0));
+  }
+  CUIMG_TRY(launch(ctx, "fast_saliency", [&] { step_fast_kernel<<<b->total[K_FAST], 256, 0, fst>>>(J, pf[K_FAST], nj); }));
+  if (fork) CUIMG_HIP(ctx, hipEventRecord(b->join_ev, b->aux));
   if (b->total[K_BLUR]) CUIMG_TRY(launch(ctx, "gaussian_sep", [&] { launch_blur(b); }));
   CUIMG_TRY(launch(ctx, "fills", [&] { step_fill_kernel<<<b->total[K_FILL], 256, 0, st>>>(J, pf[K_FILL], nj); }));
   CUIMG_TRY(launch(ctx, "lk_track", [&] {
@@ -724,9 +735,9 @@ static int enqueue_step(cuimg_hip_batch* b) {
                                                                                                 b->lk_ppw);
     });
   }));
+  if (fork) CUIMG_HIP(ctx, hipStreamWaitEvent(st, b->join_ev, 0));
   CUIMG_TRY(launch(ctx, "particle_mask",
                    [&] { step_pmask_kernel<<<b->total[K_PMASK], 256, 0, st>>>(J, pf[K_PMASK], nj); }));
-  CUIMG_TRY(launch(ctx, "fast_saliency", [&] { step_fast_kernel<<<b->total[K_FAST], 256, 0, st>>>(J, pf[K_FAST], nj); }));
   CUIMG_TRY(launch(ctx, "detect_count", [&] { step_detc_kernel<<<b->total[K_DETC], 256, 0, st>>>(J, pf[K_DETC], nj); }));
   CUIMG_TRY(launch(ctx, "compact_scan", [&] { step_cscan_kernel<<<nj, 1024, 0, st>>>(J); }));
   CUIMG_TRY(launch(ctx, "scatter_insert",
@@ -755,7 +766,7 @@ static int batch_step(cuimg_hip_batch* b, const cuimg_img* frames) {
     if (!b->exec) {
       hipGraph_t g = nullptr;
       CUIMG_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      const int rc = enqueue_step(b);
+      const int rc = enqueue_step(b, true);
       const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
       if (rc) {
         if (g) (void)hipGraphDestroy(g);
@@ -768,7 +779,7 @@ static int batch_step(cuimg_hip_batch* b, const cuimg_img* frames) {
     }
     CUIMG_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
   } else {
-    CUIMG_TRY(enqueue_step(b));
+    CUIMG_TRY(enqueue_step(b, false));
   }
   b->steps++;
   return 0;

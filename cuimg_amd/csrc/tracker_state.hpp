@@ -66,7 +66,6 @@ struct detect_args {
   float fast_th;
   // images
   const uint8_t* sal; int sal_pitch;
-  uint8_t* mask; int mask_pitch;
   uint32_t* sparse; int sparse_pitch;
   // particles
   cuimg_particle_f* parts;      // current dense vector
