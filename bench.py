@@ -65,10 +65,12 @@ def upload_video(cu, ctx, nframes):
     return video, frames
 
 
-def tracker_bench(cu, ctx, frames, fast_th, warmup, steps, prof_steps):
+def tracker_bench(cu, ctx, frames, fast_th, warmup, steps, prof_steps, pct_steps=0):
     """warmup untimed frames, `steps` timed frame steps replayed from the
-    captured graph, then `prof_steps` more steps with plain launches and HIP
-    events around every kernel (the per-kernel breakdown)."""
+    captured graph, `pct_steps` more graph steps each between two HIP events
+    (median and p10 / p90 of the step time), then `prof_steps` more steps with
+    plain launches and HIP events around every kernel (the per-kernel
+    breakdown)."""
     params = cu.profile_params("B", NSCALES, fast_th)
     trk = cu.Tracker(ctx, NR, NC, params)
     shapes = [(NR >> s, NC >> s) for s in range(NSCALES)]
@@ -84,13 +86,19 @@ def tracker_bench(cu, ctx, frames, fast_th, warmup, steps, prof_steps):
     live1 = [trk.work(s)[0] for s in range(NSCALES)]
     per_scale = [b - a for a, b in zip(live0, live1)]
     nparts = [trk.nparticles(s) for s in range(NSCALES)]
+    step_ms = []
+    for f in range(warmup + steps, warmup + steps + pct_steps):
+        ctx.timer_start()
+        trk.run(frames[f])
+        step_ms.append(ctx.timer_stop())
+    live_p = [trk.work(s)[0] for s in range(NSCALES)]
     kernels, prof = {}, None
     if prof_steps:
         trk.set_graph(False)
         ctx.profile(True)
         ctx.profile_reset()
         t1 = time.perf_counter()
-        for f in range(warmup + steps, warmup + steps + prof_steps):
+        for f in range(warmup + steps + pct_steps, warmup + steps + pct_steps + prof_steps):
             trk.run(frames[f])
         ctx.sync()
         pdt = time.perf_counter() - t1
@@ -100,14 +108,26 @@ def tracker_bench(cu, ctx, frames, fast_th, warmup, steps, prof_steps):
             if n:
                 kernels[name] = {"launches": n, "total_ms": round(ms, 4), "avg_us": round(ms / n * 1000, 3)}
         ctx.profile(False)
-        prof = dict(frames=prof_steps, seconds=pdt, per_scale=[b - a for a, b in zip(live1, live2)])
+        prof = dict(frames=prof_steps, seconds=pdt, per_scale=[b - a for a, b in zip(live_p, live2)])
     trk.close()
     return dict(seconds=dt, frames=steps, updates=sum(per_scale), per_scale=per_scale, nparts=nparts,
-                kernels=kernels, shapes=shapes, fast_th=fast_th, prof=prof)
+                kernels=kernels, shapes=shapes, fast_th=fast_th, prof=prof, step_ms=step_ms)
+
+
+def percentiles(xs):
+    if not xs:
+        return None
+    p10, p50, p90 = np.percentile(np.asarray(xs, np.float64), [10, 50, 90])
+    return {"n": len(xs), "p10": round(float(p10), 4), "median": round(float(p50), 4), "p90": round(float(p90), 4)}
 
 
 def summary(tr):
+    # level solves: a particle at scale s is matched at every level from the
+    # coarsest (NSCALES - 1) down to s
+    solves = sum(n * (NSCALES - s) for s, n in enumerate(tr["per_scale"]))
     return {"fast_th": tr["fast_th"], "particle_updates_per_s": round(tr["updates"] / tr["seconds"], 1),
+            "level_solves_per_s": round(solves / tr["seconds"], 1),
+            "step_ms_events": percentiles(tr["step_ms"]),
             "frames_per_s": round(tr["frames"] / tr["seconds"], 2),
             "ms_per_step": round(tr["seconds"] / tr["frames"] * 1000, 4),
             "updates_per_frame": round(tr["updates"] / tr["frames"], 1),
@@ -279,6 +299,17 @@ def span_us(ctx, launch, reps):
     return ctx.timer_stop() * 1000 / reps
 
 
+def launch_us(ctx, launch, reps):
+    """Microseconds of each of `reps` launches, each between two events (the
+    spread; span_us gives the back-to-back rate)."""
+    out = []
+    for i in range(reps):
+        ctx.timer_start()
+        launch(i)
+        out.append(ctx.timer_stop() * 1000)
+    return out
+
+
 def gaussian_microbench(cu, ctx, reps=40):
     """Separable Gaussian on f32 4096x4096 and f32x4 3840x2160 (sigma 1, 2, 4;
     2*ceil(3 sigma)+1 taps), 4 distinct buffer pairs rotated so the working set
@@ -309,11 +340,18 @@ def gaussian_microbench(cu, ctx, reps=40):
         nbytes = shape[0] * shape[1] * es * 2
         for sigma in (1, 2, 4):
             K = 3 * sigma
-            us = span_us(ctx, lambda i: ctx.gaussian(ins[i % 4], outs[i % 4], sigma, K), reps)
+            run = lambda i: ctx.gaussian(ins[i % 4], outs[i % 4], sigma, K)  # noqa: E731
+            us = span_us(ctx, run, reps)
             gbs = nbytes / us / 1e3
             out.append({"kernel": "gaussian_sep", "image": f"{kind} {shape[1]}x{shape[0]}", "sigma": sigma,
                         "taps": 2 * K + 1, "avg_us": round(us, 2), "bytes": nbytes, "GBps": round(gbs, 1),
-                        "frac": round(gbs / HBM_PEAK_GBS, 4)})
+                        "frac": round(gbs / HBM_PEAK_GBS, 4), "launch_us": percentiles(launch_us(ctx, run, reps))})
+            if kind == "f32" and sigma == 1:
+                # one buffer pair (128 MiB) stays in the 256 MB Infinity Cache
+                us = span_us(ctx, lambda i: ctx.gaussian(ins[0], outs[0], sigma, K), reps)
+                out.append({"kernel": "gaussian_sep", "image": f"{kind} {shape[1]}x{shape[0]}, one buffer pair "
+                            "(hot cache, not the roofline figure)", "sigma": sigma, "taps": 2 * K + 1,
+                            "avg_us": round(us, 2), "bytes": nbytes, "GBps": round(nbytes / us / 1e3, 1)})
         for im in ins + outs:
             im.free()
     # 5-level pyramid, in place on level 0
@@ -326,10 +364,12 @@ def gaussian_microbench(cu, ctx, reps=40):
             lv.append(ctx.image(r // 2, c // 2, cu.F32))
         pyrs.append(lv)
     nbytes = 4 * 4096 * 4096 * (1 + 1 / 4 + 1 / 16 + 1 / 64 + 1 / 256)
-    us = span_us(ctx, lambda i: ctx.pyramid(pyrs[i % 4][0], pyrs[i % 4]), reps)
+    run = lambda i: ctx.pyramid(pyrs[i % 4][0], pyrs[i % 4])  # noqa: E731
+    us = span_us(ctx, run, reps)
     gbs = nbytes / us / 1e3
     out.append({"kernel": "pyramid_build", "image": "f32 4096x4096, 5 levels", "avg_us": round(us, 2),
-                "bytes": int(nbytes), "GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
+                "bytes": int(nbytes), "GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                "launch_us": percentiles(launch_us(ctx, run, reps))})
     # blur-then-subsample level (sigma 1, 13 taps) of the same image
     decs = [ctx.image(2048, 2048, cu.F32) for _ in range(4)]
     nbytes = 4 * 4096 * 4096 + 4 * 2048 * 2048
@@ -360,10 +400,36 @@ def cpu_baseline(video, budget_s, fast_th):
         spent += time.perf_counter() - t0
         frames += 1
         updates += live
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = orc.max_threads()
+    # the same frames with one thread, on a fresh oracle tracker
+    orc.set_threads(1)
+    t1 = orc.tracker(nr, nc, oracle.profile_params("B", NSCALES, fast_th))
+    t1.run(video[0])
+    f1, u1, s1 = 0, 0, 0.0
+    while s1 < budget_s / 3 and f1 + 1 < len(video):
+        live = sum(int((t1.particles(s)["age"] > 0).sum()) for s in range(NSCALES))
+        t0 = time.perf_counter()
+        t1.run(video[f1 + 1])
+        s1 += time.perf_counter() - t0
+        f1 += 1
+        u1 += live
+    orc.set_threads(threads)
     return {"value": round(updates / spent, 1), "unit": "particle-updates/s", "cores": threads, "kind": "port",
             "sample": f"CPU oracle, frames 1..{frames} of the same 1920x1080 video at FAST th {fast_th:g} "
-                      f"({spent:.1f} s, {updates} particle updates, {frames / spent:.2f} frames/s)"}
+                      f"({spent:.1f} s, {updates} particle updates, {frames / spent:.2f} frames/s)",
+            "host": {"cpu_model": cpu_model(), "logical_cpus": os.cpu_count(), "omp_threads": threads},
+            "one_thread": {"value": round(u1 / s1, 1), "sample": f"frames 1..{f1}, {s1:.1f} s"}}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 # HBM traffic of the matcher launch, measured by rocprofv3 counter runs of
@@ -422,9 +488,10 @@ def main():
 
     import cuimg_amd as cu
     ctx = cu.Context(0)
-    video, frames = upload_video(cu, ctx, args.warmup + args.steps + args.prof_steps)
+    pct_steps = 50 if args.steps >= 50 else 0
+    video, frames = upload_video(cu, ctx, args.warmup + args.steps + pct_steps + args.prof_steps)
     log(f"tracker benchmark, FAST th {args.fast_th:g}")
-    tr = tracker_bench(cu, ctx, frames, args.fast_th, args.warmup, args.steps, args.prof_steps)
+    tr = tracker_bench(cu, ctx, frames, args.fast_th, args.warmup, args.steps, args.prof_steps, pct_steps)
     log(f"  {tr['updates'] / tr['seconds']:.4g} particle-updates/s, {tr['frames'] / tr['seconds']:.1f} frames/s")
     sparse = None
     if not args.no_sparse:
@@ -471,6 +538,8 @@ def main():
                                f"(~{s['updates_per_frame'] / 1000:.0f}k particle updates/frame), 7x7 LK <=5 it",
                    "frames": args.warmup + args.steps, "parallelism": "1 GPU"},
         "frames_per_s": s["frames_per_s"],
+        "level_solves_per_s": s["level_solves_per_s"],
+        "step_ms_events": s["step_ms_events"],
         "particles_per_scale": s["particles_per_scale"],
         "updates_per_frame": s["updates_per_frame"],
         "roofline": lk_roofline(tr),

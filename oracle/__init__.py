@@ -127,6 +127,8 @@ class Oracle:
         L.orc_merge.argtypes = [vp, c_int, P(ctypes.c_int32), c_int, c_int]
         L.orc_particle_mask.argtypes = [vp, c_int, c_int, c_int, P(ctypes.c_uint8)]
         L.orc_fast_select.restype = c_int
+        L.orc_set_threads.argtypes = [c_int]
+        L.orc_max_threads.restype = c_int
         L.orc_particles_add.restype = c_int
         L.orc_particles_add.argtypes = [vp, c_int, c_int, P(ctypes.c_int32), c_int, c_int, P(ctypes.c_int32), c_int]
         L.orc_particles_compact.restype = c_int
@@ -254,6 +256,13 @@ class Oracle:
         n = self.L.orc_fast_select(_ptr(sal, ctypes.c_uint8), _ptr(sparse, ctypes.c_int32), nr, nc, box,
                                    float(th), _ptr(pts, ctypes.c_int32), cap)
         return pts[:min(n, cap)].copy()
+
+    def set_threads(self, n):
+        """OpenMP threads of the oracle's parallel loops."""
+        self.L.orc_set_threads(int(n))
+
+    def max_threads(self):
+        return int(self.L.orc_max_threads())
 
     def particles_add(self, parts, sparse, pts, cap=None):
         """particle_container::add for each (row, col) of pts while below cap:

@@ -1,6 +1,8 @@
 // C entry points of the oracle for ctypes.  TEST INFRASTRUCTURE ONLY.
 #include "oracle.h"
 
+#include <omp.h>
+
 #include "oracle_core.hpp"
 #include "oracle_tracker.hpp"
 
@@ -228,6 +230,9 @@ int orc_fast_select(const uint8_t* sal, const int32_t* sparse, int nr, int nc, i
   }
   return (int)v.size();
 }
+
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_max_threads() { return omp_get_max_threads(); }
 
 int orc_particles_add(orc_particle* parts, int n, int cap, int32_t* sparse, int nr, int nc, const int32_t* pts,
                       int npts) {

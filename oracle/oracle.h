@@ -92,6 +92,10 @@ void orc_particle_mask(const orc_particle* parts, int n, int nr, int nc, uint8_t
 int orc_fast_select(const uint8_t* sal, const int32_t* sparse, int nr, int nc, int box, float th, int32_t* pts,
                     int cap);
 
+/* OpenMP threads of the oracle's parallel loops. */
+void orc_set_threads(int n);
+int orc_max_threads(void);
+
 /* particle_container::add for up to cap - n points (pts: npts (row, col)
  * pairs), in place; returns the new particle count (particle_container.hpp:370-387). */
 int orc_particles_add(orc_particle* parts, int n, int cap, int32_t* sparse, int nr, int nc, const int32_t* pts,
