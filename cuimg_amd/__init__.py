@@ -117,6 +117,7 @@ def lib():
         L.cuimg_hip_tracker_destroy.argtypes = [vp]
         L.cuimg_hip_tracker_run.argtypes = [vp, P(Img)]
         L.cuimg_hip_tracker_run_host.argtypes = [vp, vp, ctypes.c_size_t]
+        L.cuimg_hip_tracker_run_host_rgb.argtypes = [vp, vp, ctypes.c_size_t, c_int]
         L.cuimg_hip_tracker_clear.argtypes = [vp]
         L.cuimg_hip_tracker_nparticles.argtypes = [vp, c_int, P(ctypes.c_uint32)]
         L.cuimg_hip_tracker_get_particles.argtypes = [vp, c_int, vp, ctypes.c_uint32, P(ctypes.c_uint32)]
@@ -393,8 +394,15 @@ class Tracker:
         if isinstance(frame, (DeviceImage, View)):
             self.ctx.check(self.L.cuimg_hip_tracker_run(self.h, ctypes.byref(frame.d)), "tracker_run")
         else:
-            frame = np.ascontiguousarray(frame, np.uint8)
-            self.ctx.check(self.L.cuimg_hip_tracker_run_host(self.h, frame.ctypes.data, frame.shape[1]),
+            frame = np.asarray(frame, np.uint8)
+            if frame.ndim == 3:  # interleaved channels: channel 0 is the gray frame (get_x)
+                frame = np.ascontiguousarray(frame)
+                self.ctx.check(self.L.cuimg_hip_tracker_run_host_rgb(self.h, frame.ctypes.data, frame.strides[0],
+                                                                     frame.shape[2]), "tracker_run_host_rgb")
+                return
+            if frame.strides[1] != 1:
+                frame = np.ascontiguousarray(frame)
+            self.ctx.check(self.L.cuimg_hip_tracker_run_host(self.h, frame.ctypes.data, frame.strides[0]),
                            "tracker_run_host")
 
     def clear(self):

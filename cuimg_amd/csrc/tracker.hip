@@ -99,6 +99,20 @@ constexpr int kRing = 8;
 
 }  // namespace cuimg
 
+// Host-frame ingestion of one stream: frames go through two pinned staging
+// buffers to two device frames on the batch's copy stream, so the upload of
+// frame t+1 runs while the step of frame t does.  channels > 1: the packed
+// interleaved frame is copied whole and channel 0 extracted on the device.
+struct host_ingest {
+  int channels = 0;
+  uint8_t* pinned[2] = {nullptr, nullptr};
+  void* raw[2] = {nullptr, nullptr};
+  cuimg_img gray[2]{};
+  hipEvent_t copied[2] = {nullptr, nullptr};    // the upload from pinned[k] is done
+  hipEvent_t consumed[2] = {nullptr, nullptr};  // the step that read gray[k] is done
+  int k = 0;
+};
+
 struct cuimg_hip_batch {
   cuimg_hip_ctx* ctx = nullptr;
   int nstreams = 0, njobs = 0;
@@ -128,7 +142,8 @@ struct cuimg_hip_batch {
   int blur_th = 0;    // blur tile height, 0 = heuristic (tuning)
   bool use_graph = true;
   hipGraphExec_t exec = nullptr;
-  std::vector<cuimg_img> frame_buf;          // per stream, for run_host
+  std::vector<host_ingest> ingest;           // per stream, for run_host
+  hipStream_t copy = nullptr;                // host-frame uploads (default priority)
   std::vector<void*> allocs;
   std::vector<cuimg_hip_tracker*> views;
 };
@@ -314,7 +329,48 @@ __global__ __launch_bounds__(1024) void step_mres_kernel(const level_job* jobs) 
   }
 }
 
+// Channel 0 of a packed interleaved u8 frame (the reference sample's get_x).
+__global__ __launch_bounds__(256) void extract_x_kernel(const uint8_t* raw, int channels, uint8_t* out, int pitch,
+                                                        int nr, int nc) {
+  const int c = blockIdx.x * 256 + threadIdx.x, r = blockIdx.y;
+  if (c < nc) out[(size_t)r * pitch + c] = raw[((size_t)r * nc + c) * channels];
+}
+
 // ------------------------------------------------------------------ host
+static void ingest_free(cuimg_hip_ctx* ctx, host_ingest& H) {
+  for (int i = 0; i < 2; i++) {
+    if (H.copied[i]) (void)hipEventSynchronize(H.copied[i]);
+    if (H.consumed[i]) (void)hipEventSynchronize(H.consumed[i]);
+    if (H.pinned[i]) (void)hipHostFree(H.pinned[i]);
+    if (H.raw[i]) (void)hipFree(H.raw[i]);
+    image_free(ctx, &H.gray[i]);
+    if (H.copied[i]) (void)hipEventDestroy(H.copied[i]);
+    if (H.consumed[i]) (void)hipEventDestroy(H.consumed[i]);
+  }
+  H = host_ingest{};
+}
+
+static int ingest_alloc(cuimg_hip_ctx* ctx, host_ingest& H, int nr, int nc, int channels) {
+  ingest_free(ctx, H);
+  const size_t bytes = (size_t)nr * nc * channels;
+  for (int i = 0; i < 2; i++) {
+    if (hipHostMalloc(reinterpret_cast<void**>(&H.pinned[i]), bytes) != hipSuccess ||
+        (channels > 1 && hipMalloc(&H.raw[i], bytes) != hipSuccess) ||
+        hipEventCreateWithFlags(&H.copied[i], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&H.consumed[i], hipEventDisableTiming) != hipSuccess) {
+      ingest_free(ctx, H);
+      return set_error(ctx, CUIMG_E_NOMEM, "run_host: staging allocation failed");
+    }
+    const int rc = image_alloc(ctx, nr, nc, 0, CUIMG_U8, &H.gray[i]);
+    if (rc) {
+      ingest_free(ctx, H);
+      return rc;
+    }
+  }
+  H.channels = channels;
+  return 0;
+}
+
 static int dev_alloc(cuimg_hip_batch* b, size_t bytes, void** out) {
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
@@ -345,7 +401,8 @@ static void destroy_batch(cuimg_hip_batch* b) {
     image_free(b->ctx, &R.sal);
     image_free(b->ctx, &R.sparse);
   }
-  for (auto& f : b->frame_buf) image_free(b->ctx, &f);
+  for (auto& H : b->ingest) ingest_free(b->ctx, H);
+  if (b->copy) (void)hipStreamDestroy(b->copy);
   for (void* p : b->allocs) (void)hipFree(p);
   if (b->hin) (void)hipHostFree(b->hin);
   for (auto& e : b->ring_ev)
@@ -661,7 +718,7 @@ static int create_batch(cuimg_hip_ctx* ctx, int nstreams, const int* nrows, cons
   b->lv.resize(b->njobs);
   b->hj.resize(b->njobs);
   b->hs.resize(nstreams);
-  b->frame_buf.resize(nstreams);
+  b->ingest.resize(nstreams);
   int rc = 0;
   if ((rc = dev_alloc_n(b, b->njobs, &b->cnt))) return fail(rc);
   if (hipMemsetAsync(b->cnt, 0, sizeof(level_counters) * b->njobs, ctx->stream) != hipSuccess)
@@ -685,7 +742,8 @@ static int create_batch(cuimg_hip_ctx* ctx, int nstreams, const int* nrows, cons
       return fail(set_error(ctx, CUIMG_E_NOMEM, "batch: event creation failed"));
   if (hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming) != hipSuccess ||
-      hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking) != hipSuccess)
+      hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->copy, hipStreamNonBlocking) != hipSuccess)
     return fail(set_error(ctx, CUIMG_E_NOMEM, "batch: stream or event creation failed"));
   if ((rc = rebuild(b))) return fail(rc);
   *out = b;
@@ -972,15 +1030,48 @@ int cuimg_hip_tracker_run(cuimg_hip_tracker* t, const cuimg_img* frame) {
   return batch_step(t->b, frame);
 }
 
-int cuimg_hip_tracker_run_host(cuimg_hip_tracker* t, const uint8_t* frame, size_t pitch) {
-  if (!t || !frame) return CUIMG_E_INVALID;
+int cuimg_hip_tracker_run_host_rgb(cuimg_hip_tracker* t, const uint8_t* frame, size_t pitch, int channels) {
+  if (!t || !frame || (channels != 1 && channels != 3 && channels != 4)) return CUIMG_E_INVALID;
   cuimg_hip_batch* b = t->b;
-  cuimg_img& fb = b->frame_buf[t->stream];
+  cuimg_hip_ctx* ctx = b->ctx;
+  if (b->nstreams != 1)
+    return set_error(ctx, CUIMG_E_INVALID, "tracker_run_host: a stream of a batch steps with batch_step");
   const level_res& R0 = b->lv[b->first[t->stream]];
-  if (!fb.begin) CUIMG_TRY(image_alloc(b->ctx, R0.nr, R0.nc, 0, CUIMG_U8, &fb));
-  CUIMG_HIP(b->ctx, hipMemcpy2DAsync(fb.begin, fb.pitch, frame, pitch, fb.ncols, fb.nrows, hipMemcpyHostToDevice,
-                                     b->ctx->stream));
-  return cuimg_hip_tracker_run(t, &fb);
+  const int nr = R0.nr, nc = R0.nc;
+  const size_t row = (size_t)nc * channels;
+  if (pitch < row) return set_error(ctx, CUIMG_E_INVALID, "tracker_run_host: pitch %zu below %zu", pitch, row);
+  host_ingest& H = b->ingest[t->stream];
+  if (H.channels != channels) CUIMG_TRY(ingest_alloc(ctx, H, nr, nc, channels));
+  const int k = H.k;
+  H.k ^= 1;
+  CUIMG_HIP(ctx, hipEventSynchronize(H.copied[k]));  // the upload from this staging buffer two calls ago
+  if (pitch == row) {
+    std::memcpy(H.pinned[k], frame, row * nr);
+  } else {
+    for (int r = 0; r < nr; r++) std::memcpy(H.pinned[k] + row * r, frame + pitch * r, row);
+  }
+  // the device frame is free once the step that read it (two calls ago) is done
+  CUIMG_HIP(ctx, hipStreamWaitEvent(b->copy, H.consumed[k], 0));
+  if (channels == 1)
+    CUIMG_HIP(ctx, hipMemcpy2DAsync(H.gray[k].begin, H.gray[k].pitch, H.pinned[k], row, row, nr, hipMemcpyHostToDevice,
+                                    b->copy));
+  else
+    CUIMG_HIP(ctx, hipMemcpyAsync(H.raw[k], H.pinned[k], row * nr, hipMemcpyHostToDevice, b->copy));
+  CUIMG_HIP(ctx, hipEventRecord(H.copied[k], b->copy));
+  CUIMG_HIP(ctx, hipStreamWaitEvent(ctx->stream, H.copied[k], 0));
+  if (channels > 1)
+    CUIMG_TRY(launch(ctx, "extract_x", [&] {
+      extract_x_kernel<<<dim3(ceil_div(nc, 256), nr), 256, 0, ctx->stream>>>(
+          static_cast<const uint8_t*>(H.raw[k]), channels, static_cast<uint8_t*>(H.gray[k].begin), H.gray[k].pitch,
+          nr, nc);
+    }));
+  const int rc = cuimg_hip_tracker_run(t, &H.gray[k]);
+  CUIMG_HIP(ctx, hipEventRecord(H.consumed[k], ctx->stream));
+  return rc;
+}
+
+int cuimg_hip_tracker_run_host(cuimg_hip_tracker* t, const uint8_t* frame, size_t pitch) {
+  return cuimg_hip_tracker_run_host_rgb(t, frame, pitch, 1);
 }
 
 int cuimg_hip_tracker_set_tuning(cuimg_hip_tracker* t, int lk_ppw, int blur_th) {

@@ -69,12 +69,16 @@ namespace cuimg
       if (owner_ && owner_->h.t) cuimg_hip_tracker_destroy(owner_->h.t);
     }
 
+    // A host frame goes through the library's pinned double buffer: the call
+    // returns once the frame is staged, and its upload overlaps the previous
+    // frame step.
     tracker& run(const I& in)
     {
+      if (s_ != 0) throw std::runtime_error("cuimg: run() is called on the finest scale");
       ensure();
-      if (!st_->staging) st_->staging.reset(new device_image2d<gl8u>(domain_, 0, *st_->h.ctx));
-      copy(in, *st_->staging);
-      return run(*st_->staging);
+      st_->h.ctx->check(cuimg_hip_tracker_run_host(st_->h.t, reinterpret_cast<const uint8_t*>(in.begin()), in.pitch()),
+                        "tracker run");
+      return *this;
     }
 
     tracker& run(const device_image2d<gl8u>& in)
@@ -111,7 +115,6 @@ namespace cuimg
       std::vector<obox2d> domains;
       std::vector<tracker*> scales;
       std::vector<std::unique_ptr<tracker>> owned;
-      std::unique_ptr<device_image2d<gl8u>> staging;
     };
 
     tracker(state* st, int s, const obox2d& d)

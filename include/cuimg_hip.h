@@ -214,8 +214,16 @@ int cuimg_hip_tracker_destroy(cuimg_hip_tracker* t);
  * scales.  `frame` is a U8 device image of the tracker domain (any border).
  * Asynchronous on the context stream. */
 int cuimg_hip_tracker_run(cuimg_hip_tracker* t, const cuimg_img* frame);
-/* Same from a host buffer (includes the host-to-device copy). */
+/* Same from a host gray frame (pitch in bytes).  The frame is copied into one
+ * of two pinned staging buffers before the call returns (the caller may reuse
+ * it at once) and uploaded on a copy stream while the previous step still
+ * runs; the step waits for its upload. */
 int cuimg_hip_tracker_run_host(cuimg_hip_tracker* t, const uint8_t* frame, size_t pitch);
+/* Same from a host frame of `channels` interleaved u8 channels (1, 3 or 4):
+ * the gray frame is channel 0, as the reference sample's get_x(frame)
+ * (samples/tracking/main.cc:116-117), extracted on the device after the
+ * upload of the packed frame. */
+int cuimg_hip_tracker_run_host_rgb(cuimg_hip_tracker* t, const uint8_t* frame, size_t pitch, int channels);
 /* Per-scale settings; take effect from the next frame.  Reading fills `out`. */
 int cuimg_hip_tracker_set_scale_params(cuimg_hip_tracker* t, int scale, const cuimg_scale_params* in);
 int cuimg_hip_tracker_get_scale_params(cuimg_hip_tracker* t, int scale, cuimg_scale_params* out);
