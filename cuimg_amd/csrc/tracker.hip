@@ -786,6 +786,9 @@ static int enqueue_step(cuimg_hip_batch* b, bool capturing) {
   if (fork) CUIMG_HIP(ctx, hipEventRecord(b->join_ev, b->aux));
   if (b->total[K_BLUR]) CUIMG_TRY(launch(ctx, "gaussian_sep", [&] { launch_blur(b); }));
   CUIMG_TRY(launch(ctx, "fills", [&] { step_fill_kernel<<<b->total[K_FILL], 256, 0, st>>>(J, pf[K_FILL], nj); }));
+  // join before the matcher: FAST takes about as long as blur + fill, and a
+  // FAST still running would hold CU slots the matcher's workgroups wait for
+  if (fork) CUIMG_HIP(ctx, hipStreamWaitEvent(st, b->join_ev, 0));
   CUIMG_TRY(launch(ctx, "lk_track", [&] {
     lk_variant(b->P0.lk_ws, b->P0.lk_top_n, b->P0.profile == 0, [&](auto ti, auto npl, auto sel) {
       using TI = typename decltype(ti)::type;
@@ -793,7 +796,6 @@ static int enqueue_step(cuimg_hip_batch* b, bool capturing) {
                                                                                                 b->lk_ppw);
     });
   }));
-  if (fork) CUIMG_HIP(ctx, hipStreamWaitEvent(st, b->join_ev, 0));
   CUIMG_TRY(launch(ctx, "particle_mask",
                    [&] { step_pmask_kernel<<<b->total[K_PMASK], 256, 0, st>>>(J, pf[K_PMASK], nj); }));
   CUIMG_TRY(launch(ctx, "detect_count", [&] { step_detc_kernel<<<b->total[K_DETC], 256, 0, st>>>(J, pf[K_DETC], nj); }));
