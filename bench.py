@@ -50,7 +50,7 @@ def lk_algorithmic_bytes(level_shapes, nparts):
 
 
 KERNELS = ("pyramid_build", "gaussian_sep", "fills", "lk_track", "particle_mask", "fast_saliency",
-           "detect_count", "compact_scan", "scatter_insert", "merge_scan", "merge_round1", "merge_resolve")
+           "detect_count", "scatter_insert", "merge_scan", "merge_round1", "merge_resolve")
 
 
 def upload_video(cu, ctx, nframes):

@@ -23,9 +23,10 @@
 // Raster order on the GPU: a box row covers `box` image rows, so the raster
 // order of new points is (box row, row inside the box, box column).  The
 // workgroup that selects a box row also counts its points per (box row, row
-// offset) "sub-row", one scan over sub-rows gives the offsets, and one wave per
-// sub-row writes its points with a ballot prefix.  Compaction is a count /
-// scan / scatter over 1024-particle blocks.  The serial merge is resolved
+// offset) "sub-row", a prefix over sub-rows gives the offsets, and one wave per
+// sub-row writes its points with a ballot prefix.  Compaction is a count and a
+// scatter over 1024-particle blocks (with a scan launch between them in the
+// standalone entry points; the frame step sums the prefixes in the scatter).  The serial merge is resolved
 // exactly in parallel rounds and a serial replay of the few actors they leave
 // open.
 #pragma once
@@ -70,6 +71,13 @@ __device__ inline uint32_t block_excl_scan(uint32_t x, uint32_t* tmp, uint32_t* 
   if (total) *total = tmp[31];
   __syncthreads();
   return r;
+}
+
+// Sum of x[lo, hi) over the wave; every lane gets it.
+__device__ inline uint32_t wave_range_sum(const uint32_t* x, int lo, int hi) {
+  uint32_t s = 0;
+  for (int i = lo + (int)(threadIdx.x & 63); i < hi; i += 64) s += x[i];
+  return (uint32_t)__shfl(wave_incl_scan(s), 63, 64);
 }
 
 // ------------------------------------------------------------------ fills
@@ -298,9 +306,10 @@ __device__ inline void compact_scan_body(const detect_args& a) {
   }
 }
 
-__device__ inline void compact_scatter_body(const detect_args& a, int bid) {
+// Compaction scatter of 1024-particle block bid whose first kept particle goes
+// to index off0.
+__device__ inline void compact_scatter_at(const detect_args& a, int bid, uint32_t off0, uint32_t n_old) {
   __shared__ uint32_t tmp[64];
-  const uint32_t n_old = a.cnt->n_old;
   const uint32_t base = bid * kCompactBlock + threadIdx.x * 4;
   uint32_t c = 0;
   uint8_t f[4];
@@ -309,7 +318,7 @@ __device__ inline void compact_scatter_body(const detect_args& a, int bid) {
     f[k] = i < n_old ? a.flags[i] : 0;
     c += f[k] & 1;
   }
-  uint32_t off = a.blockcnt[bid] + block_excl_scan(c, tmp, nullptr);
+  uint32_t off = off0 + block_excl_scan(c, tmp, nullptr);
   for (int k = 0; k < 4; k++) {
     const uint32_t i = base + k;
     if (i >= n_old) break;
@@ -325,16 +334,17 @@ __device__ inline void compact_scatter_body(const detect_args& a, int bid) {
   }
 }
 
-// One wave per sub-row (4 per 256-thread block) visits its selected points in
-// column order: emit(k, r, c) with k the point's rank among all new points
-// (subcnt holds the scanned sub-row offsets).
+// After compact_scan_body: blockcnt holds exclusive offsets.
+__device__ inline void compact_scatter_body(const detect_args& a, int bid) {
+  compact_scatter_at(a, bid, a.blockcnt[bid], a.cnt->n_old);
+}
+
+// The wave of sub-row `sub` visits its selected points in column order:
+// emit(k, r, c) with k the point's rank among all new points, starting at k0.
 template <class Emit>
-__device__ inline void for_new_points(const detect_args& a, int bid, Emit emit) {
-  const int sub = bid * 4 + (threadIdx.x >> 6);
+__device__ inline void for_new_points_from(const detect_args& a, int sub, uint32_t k, Emit emit) {
   const int lane = threadIdx.x & 63;
-  if (sub >= a.nbr * a.box) return;
   const int bi = sub / a.box, dr = sub % a.box;
-  uint32_t k = a.subcnt[sub];
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int b0 = 0; b0 < a.nbc; b0 += 64) {
     const int bj = b0 + lane;
@@ -346,11 +356,48 @@ __device__ inline void for_new_points(const detect_args& a, int bid, Emit emit) 
   }
 }
 
+// One wave per sub-row (4 per 256-thread block); after compact_scan_body
+// subcnt holds the scanned sub-row offsets.
+template <class Emit>
+__device__ inline void for_new_points(const detect_args& a, int bid, Emit emit) {
+  const int sub = bid * 4 + (threadIdx.x >> 6);
+  if (sub >= a.nbr * a.box) return;
+  for_new_points_from(a, sub, a.subcnt[sub], emit);
+}
+
+// Frame-step compaction without a scan launch: blockcnt and subcnt hold raw
+// counts, each block or wave sums the prefix it needs, and block 0 writes the
+// container counters (nothing else in this launch reads them; the count launch
+// saved the previous size in n_old).
+__device__ inline void compact_scatter_step(const detect_args& a, int bid) {
+  const uint32_t off0 = wave_range_sum(a.blockcnt, 0, bid);
+  compact_scatter_at(a, bid, off0, a.cnt->n_old);
+  if (bid == 0) {
+    const uint32_t n_live = wave_range_sum(a.blockcnt, 0, a.nblocks);
+    const uint32_t n_new = wave_range_sum(a.subcnt, 0, a.nbr * a.box);
+    if (threadIdx.x == 0) {
+      level_counters& C = *a.cnt;
+      uint32_t total = n_live + n_new;
+      C.n_live = n_live;
+      C.n_new = n_new;
+      if (total > a.cap) {
+        C.overflow = 1;
+        total = a.cap;
+      }
+      C.n = total;
+    }
+  }
+}
+
 // Appends the new points after the live particles of the compacted vector.
-__device__ inline void insert_new_body(const detect_args& a, int bid) {
-  const level_counters& C = *a.cnt;
-  const uint32_t n_live = C.n_live, n_old = C.n_old, n = C.n;
-  for_new_points(a, bid, [&](uint32_t kk, int r, int c) {
+__device__ inline void insert_new_step(const detect_args& a, int bid) {
+  const int sub = bid * 4 + (threadIdx.x >> 6);
+  if (sub >= a.nbr * a.box) return;
+  const uint32_t n_live = wave_range_sum(a.blockcnt, 0, a.nblocks);
+  const uint32_t n_new = wave_range_sum(a.subcnt, 0, a.nbr * a.box);
+  const uint32_t k0 = wave_range_sum(a.subcnt, 0, sub);
+  const uint32_t n = min(n_live + n_new, a.cap), n_old = a.cnt->n_old;
+  for_new_points_from(a, sub, k0, [&](uint32_t kk, int r, int c) {
     const uint32_t j = n_live + kk;
     if (j < n) {
       cuimg_particle_f p;

@@ -274,16 +274,9 @@ __global__ __launch_bounds__(256) void step_detc_kernel(const level_job* jobs, c
   const step_flags f = flags_of(J);
   if (!f.detect) return;
   const detect_args d = uni(J.det[f.cur]);
+  if (lb == 0 && threadIdx.x == 0) J.cnt->n_old = J.cnt->n;  // the size before this compaction
   if (lb < d.nbr) nms_row_body(d, J.prm.fast_th, lb);
   else compact_count_body(d, lb - d.nbr);
-}
-
-__global__ __launch_bounds__(1024) void step_cscan_kernel(const level_job* jobs) {
-  const level_job& J = jobs[blockIdx.x];
-  const step_flags f = flags_of(J);
-  if (!f.detect) return;
-  const detect_args d = uni(J.det[f.cur]);
-  compact_scan_body(d);
 }
 
 // Compaction scatter and insertion of the new points (disjoint outputs).
@@ -292,8 +285,8 @@ __global__ __launch_bounds__(256) void step_scati_kernel(const level_job* jobs, 
   const step_flags f = flags_of(J);
   if (!f.detect) return;
   const detect_args d = uni(J.det[f.cur]);
-  if (lb < d.nblocks) compact_scatter_body(d, lb);
-  else insert_new_body(d, lb - d.nblocks);
+  if (lb < d.nblocks) compact_scatter_step(d, lb);
+  else insert_new_step(d, lb - d.nblocks);
 }
 
 __global__ __launch_bounds__(256) void step_mscan_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
@@ -799,7 +792,6 @@ static int enqueue_step(cuimg_hip_batch* b, bool capturing) {
   CUIMG_TRY(launch(ctx, "particle_mask",
                    [&] { step_pmask_kernel<<<b->total[K_PMASK], 256, 0, st>>>(J, pf[K_PMASK], nj); }));
   CUIMG_TRY(launch(ctx, "detect_count", [&] { step_detc_kernel<<<b->total[K_DETC], 256, 0, st>>>(J, pf[K_DETC], nj); }));
-  CUIMG_TRY(launch(ctx, "compact_scan", [&] { step_cscan_kernel<<<nj, 1024, 0, st>>>(J); }));
   CUIMG_TRY(launch(ctx, "scatter_insert",
                    [&] { step_scati_kernel<<<b->total[K_SCATI], 256, 0, st>>>(J, pf[K_SCATI], nj); }));
   CUIMG_TRY(launch(ctx, "merge_scan", [&] { step_mscan_kernel<<<b->total[K_MSCAN], 256, 0, st>>>(J, pf[K_MSCAN], nj); }));
