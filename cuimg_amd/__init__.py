@@ -372,7 +372,9 @@ class Tracker:
 
     def close(self):
         if getattr(self, "h", None):
-            if self._owner:
+            # after its context is closed the tracker is only forgotten: its
+            # destroy call would use the closed context
+            if self._owner and self.ctx.h:
                 self.L.cuimg_hip_tracker_destroy(self.h)
             self.h = None
 
@@ -509,7 +511,8 @@ class Batch:
 
     def close(self):
         if getattr(self, "h", None):
-            self.L.cuimg_hip_batch_destroy(self.h)
+            if self.ctx.h:  # see Tracker.close
+                self.L.cuimg_hip_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):
