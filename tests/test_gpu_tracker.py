@@ -212,3 +212,15 @@ def test_cpp_sample_facade_matches_c_abi():
     r = subprocess.run([exe, "3", "20", "8", "240", "320"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert r.stdout.strip().endswith("OK")
+
+
+def test_lockstep_1080p_full_size(ctx, orc):
+    """The headline workload's size and threshold (1920x1080, 4 scales, FAST th
+    8, ~100k particles) for a few frames in lockstep with the oracle: the same
+    bars as at the small sizes."""
+    video = cu.synth_video(0xC0FFEE02, 1080, 1920, 4, 1)
+    st = run_lockstep(ctx, orc, video, oracle.profile_params("B", 4, 8.0), cu.profile_params("B", 4, 8.0), 4)
+    print(st)
+    assert st["tracked"] > 150000
+    assert st["within"] >= 0.999 * st["tracked"]
+    assert st["decision_mismatch"] <= 0.001 * st["decisions"]
