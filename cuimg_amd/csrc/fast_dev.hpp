@@ -13,7 +13,8 @@
 //
 // Here the run test is branch-free: with the 16-bit masks B (bright) and D
 // (dark), a circular run of length >= n exists iff the AND of the mask with
-// its n-1 successive rotations is non-zero.  A 256-thread workgroup stages a
+// its n-1 successive rotations is non-zero, computed in log2(n) + 1 shift/AND
+// steps on the mask repeated over 32 bits.  A 256-thread workgroup stages a
 // (16+6) x (64+6) byte tile in LDS (one global read per input byte) and each
 // thread scores 4 pixels of one column.
 #pragma once
@@ -28,13 +29,19 @@ static __constant__ int8_t kRingC[16] = {0, 1, 2, 3, 3, 3, 2, 1, 0, -1, -2, -3, 
 
 constexpr int kFTW = 64, kFTH = 16, kFH = 3;
 
+// A circular run of n (1..16) set bits in the 16-bit mask m.  On m repeated
+// over 32 bits, bit j of r marks a run of `len` set bits starting at j; the
+// run doubles per step, and a last step of n - len <= len completes it.  Bits
+// 0..15 of the result cover every start of the circular mask.
 __device__ __forceinline__ bool has_run(unsigned m, int n) {
-  unsigned r = m;
-  for (int k = 1; k < n; k++) {
-    unsigned rot = ((m >> k) | (m << (16 - k))) & 0xFFFFu;
-    r &= rot;
+  unsigned r = m | (m << 16);
+  int len = 1;
+  while (2 * len <= n) {
+    r &= r >> len;
+    len *= 2;
   }
-  return r != 0;
+  if (n > len) r &= r >> (n - len);
+  return (r & 0xFFFFu) != 0;
 }
 
 __device__ __forceinline__ uint8_t fast_score_lds(const uint8_t* t, int stride, int n, int th) {
