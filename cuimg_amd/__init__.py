@@ -145,6 +145,7 @@ def lib():
         L.cuimg_hip_batch_create.argtypes = [vp, c_int, P(c_int), P(c_int), P(TrackerParams), P(vp)]
         L.cuimg_hip_batch_destroy.argtypes = [vp]
         L.cuimg_hip_batch_step.argtypes = [vp, P(Img)]
+        L.cuimg_hip_batch_step_host.argtypes = [vp, P(vp), P(ctypes.c_size_t), c_int]
         L.cuimg_hip_batch_set_graph.argtypes = [vp, c_int]
         L.cuimg_hip_batch_tracker.argtypes = [vp, c_int, P(vp)]
         L.cuimg_hip_batch_set_tuning.argtypes = [vp, c_int, c_int]
@@ -522,8 +523,23 @@ class Batch:
             pass
 
     def step(self, frames):
-        arr = (Img * self.n)(*[f.d for f in frames])
-        self.ctx.check(self.L.cuimg_hip_batch_step(self.h, arr), "batch_step")
+        """One frame step of every stream: device images, or host arrays
+        (2-D gray, or 3-D interleaved channels with channel 0 the gray frame;
+        every stream the same channel count)."""
+        if all(isinstance(f, (DeviceImage, View)) for f in frames):
+            arr = (Img * self.n)(*[f.d for f in frames])
+            self.ctx.check(self.L.cuimg_hip_batch_step(self.h, arr), "batch_step")
+            return
+        hs = []
+        for f in frames:
+            f = np.asarray(f, np.uint8)
+            if f.ndim == 3 or f.strides[-1] != 1:
+                f = np.ascontiguousarray(f)
+            hs.append(f)
+        ch = hs[0].shape[2] if hs[0].ndim == 3 else 1
+        ptrs = (ctypes.c_void_p * self.n)(*[f.ctypes.data for f in hs])
+        pitches = (ctypes.c_size_t * self.n)(*[f.strides[0] for f in hs])
+        self.ctx.check(self.L.cuimg_hip_batch_step_host(self.h, ptrs, pitches, ch), "batch_step_host")
 
     def set_graph(self, on):
         self.ctx.check(self.L.cuimg_hip_batch_set_graph(self.h, 1 if on else 0), "batch_set_graph")

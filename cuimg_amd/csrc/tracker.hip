@@ -364,6 +364,54 @@ static int ingest_alloc(cuimg_hip_ctx* ctx, host_ingest& H, int nr, int nc, int 
   return 0;
 }
 
+// Stages a host frame of stream i: copy into the next pinned buffer (the call
+// returns after this copy), upload on the copy stream once the step that last
+// read the device frame is done, make the context stream wait for the upload,
+// extract channel 0 of an interleaved frame.  *dev is the device frame.
+static int stage_host_frame(cuimg_hip_batch* b, int i, const uint8_t* frame, size_t pitch, int channels,
+                            const cuimg_img** dev) {
+  cuimg_hip_ctx* ctx = b->ctx;
+  const level_res& R0 = b->lv[b->first[i]];
+  const int nr = R0.nr, nc = R0.nc;
+  const size_t row = (size_t)nc * channels;
+  if (!frame || (channels != 1 && channels != 3 && channels != 4))
+    return set_error(ctx, CUIMG_E_INVALID, "host frame %d: null or %d channels", i, channels);
+  if (pitch < row) return set_error(ctx, CUIMG_E_INVALID, "host frame %d: pitch %zu below %zu", i, pitch, row);
+  host_ingest& H = b->ingest[i];
+  if (H.channels != channels) CUIMG_TRY(ingest_alloc(ctx, H, nr, nc, channels));
+  const int k = H.k;
+  CUIMG_HIP(ctx, hipEventSynchronize(H.copied[k]));  // the upload from this staging buffer two steps ago
+  if (pitch == row) {
+    std::memcpy(H.pinned[k], frame, row * nr);
+  } else {
+    for (int r = 0; r < nr; r++) std::memcpy(H.pinned[k] + row * r, frame + pitch * r, row);
+  }
+  CUIMG_HIP(ctx, hipStreamWaitEvent(b->copy, H.consumed[k], 0));
+  if (channels == 1)
+    CUIMG_HIP(ctx, hipMemcpy2DAsync(H.gray[k].begin, H.gray[k].pitch, H.pinned[k], row, row, nr, hipMemcpyHostToDevice,
+                                    b->copy));
+  else
+    CUIMG_HIP(ctx, hipMemcpyAsync(H.raw[k], H.pinned[k], row * nr, hipMemcpyHostToDevice, b->copy));
+  CUIMG_HIP(ctx, hipEventRecord(H.copied[k], b->copy));
+  CUIMG_HIP(ctx, hipStreamWaitEvent(ctx->stream, H.copied[k], 0));
+  if (channels > 1)
+    CUIMG_TRY(launch(ctx, "extract_x", [&] {
+      extract_x_kernel<<<dim3(ceil_div(nc, 256), nr), 256, 0, ctx->stream>>>(
+          static_cast<const uint8_t*>(H.raw[k]), channels, static_cast<uint8_t*>(H.gray[k].begin), H.gray[k].pitch,
+          nr, nc);
+    }));
+  *dev = &H.gray[k];
+  return 0;
+}
+
+// After the step that reads stream i's staged frame is enqueued.
+static int host_frame_consumed(cuimg_hip_batch* b, int i) {
+  host_ingest& H = b->ingest[i];
+  CUIMG_HIP(b->ctx, hipEventRecord(H.consumed[H.k], b->ctx->stream));
+  H.k ^= 1;
+  return 0;
+}
+
 static int dev_alloc(cuimg_hip_batch* b, size_t bytes, void** out) {
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
@@ -955,6 +1003,20 @@ int cuimg_hip_batch_destroy(cuimg_hip_batch* b) {
 
 int cuimg_hip_batch_step(cuimg_hip_batch* b, const cuimg_img* frames) { return batch_step(b, frames); }
 
+int cuimg_hip_batch_step_host(cuimg_hip_batch* b, const uint8_t* const* frames, const size_t* pitches,
+                              int channels) {
+  if (!b || !frames || !pitches) return CUIMG_E_INVALID;
+  std::vector<cuimg_img> dev(b->nstreams);
+  for (int i = 0; i < b->nstreams; i++) {
+    const cuimg_img* d = nullptr;
+    CUIMG_TRY(stage_host_frame(b, i, frames[i], pitches[i], channels, &d));
+    dev[i] = *d;
+  }
+  const int rc = batch_step(b, dev.data());
+  for (int i = 0; i < b->nstreams; i++) CUIMG_TRY(host_frame_consumed(b, i));
+  return rc;
+}
+
 int cuimg_hip_batch_set_graph(cuimg_hip_batch* b, int enable) {
   if (!b) return CUIMG_E_INVALID;
   b->use_graph = enable != 0;
@@ -1027,40 +1089,12 @@ int cuimg_hip_tracker_run(cuimg_hip_tracker* t, const cuimg_img* frame) {
 int cuimg_hip_tracker_run_host_rgb(cuimg_hip_tracker* t, const uint8_t* frame, size_t pitch, int channels) {
   if (!t || !frame || (channels != 1 && channels != 3 && channels != 4)) return CUIMG_E_INVALID;
   cuimg_hip_batch* b = t->b;
-  cuimg_hip_ctx* ctx = b->ctx;
   if (b->nstreams != 1)
-    return set_error(ctx, CUIMG_E_INVALID, "tracker_run_host: a stream of a batch steps with batch_step");
-  const level_res& R0 = b->lv[b->first[t->stream]];
-  const int nr = R0.nr, nc = R0.nc;
-  const size_t row = (size_t)nc * channels;
-  if (pitch < row) return set_error(ctx, CUIMG_E_INVALID, "tracker_run_host: pitch %zu below %zu", pitch, row);
-  host_ingest& H = b->ingest[t->stream];
-  if (H.channels != channels) CUIMG_TRY(ingest_alloc(ctx, H, nr, nc, channels));
-  const int k = H.k;
-  H.k ^= 1;
-  CUIMG_HIP(ctx, hipEventSynchronize(H.copied[k]));  // the upload from this staging buffer two calls ago
-  if (pitch == row) {
-    std::memcpy(H.pinned[k], frame, row * nr);
-  } else {
-    for (int r = 0; r < nr; r++) std::memcpy(H.pinned[k] + row * r, frame + pitch * r, row);
-  }
-  // the device frame is free once the step that read it (two calls ago) is done
-  CUIMG_HIP(ctx, hipStreamWaitEvent(b->copy, H.consumed[k], 0));
-  if (channels == 1)
-    CUIMG_HIP(ctx, hipMemcpy2DAsync(H.gray[k].begin, H.gray[k].pitch, H.pinned[k], row, row, nr, hipMemcpyHostToDevice,
-                                    b->copy));
-  else
-    CUIMG_HIP(ctx, hipMemcpyAsync(H.raw[k], H.pinned[k], row * nr, hipMemcpyHostToDevice, b->copy));
-  CUIMG_HIP(ctx, hipEventRecord(H.copied[k], b->copy));
-  CUIMG_HIP(ctx, hipStreamWaitEvent(ctx->stream, H.copied[k], 0));
-  if (channels > 1)
-    CUIMG_TRY(launch(ctx, "extract_x", [&] {
-      extract_x_kernel<<<dim3(ceil_div(nc, 256), nr), 256, 0, ctx->stream>>>(
-          static_cast<const uint8_t*>(H.raw[k]), channels, static_cast<uint8_t*>(H.gray[k].begin), H.gray[k].pitch,
-          nr, nc);
-    }));
-  const int rc = cuimg_hip_tracker_run(t, &H.gray[k]);
-  CUIMG_HIP(ctx, hipEventRecord(H.consumed[k], ctx->stream));
+    return set_error(b->ctx, CUIMG_E_INVALID, "tracker_run_host: a stream of a batch steps with batch_step");
+  const cuimg_img* dev = nullptr;
+  CUIMG_TRY(stage_host_frame(b, t->stream, frame, pitch, channels, &dev));
+  const int rc = cuimg_hip_tracker_run(t, dev);
+  CUIMG_TRY(host_frame_consumed(b, t->stream));
   return rc;
 }
 

@@ -330,6 +330,13 @@ int cuimg_hip_batch_destroy(cuimg_hip_batch* b);
 /* One frame step of every stream: frames[i] is a U8 device image of stream i's
  * domain.  Asynchronous on the context stream. */
 int cuimg_hip_batch_step(cuimg_hip_batch* b, const cuimg_img* frames);
+/* The same from host frames: frames[i] (pitches[i] bytes per row) holds stream
+ * i's frame with `channels` interleaved u8 channels (1, 3 or 4; channel 0 is
+ * the gray frame).  Each is staged like cuimg_hip_tracker_run_host: copied into
+ * a pinned buffer before the call returns, uploaded while the previous step
+ * runs.  No counterpart in the reference (which has no multi-stream form). */
+int cuimg_hip_batch_step_host(cuimg_hip_batch* b, const uint8_t* const* frames, const size_t* pitches,
+                              int channels);
 int cuimg_hip_batch_set_graph(cuimg_hip_batch* b, int enable);
 /* Tracker view of stream i for the tracker accessors (owned by the batch;
  * tracker_run on it is refused, tracker_destroy does nothing). */

@@ -138,3 +138,34 @@ def test_lk_track_standalone_equals_tracker(ctx):
         after = t.particles(s)  # no detection ran: same vector, moved in place
         for k in parts.dtype.names:
             assert np.array_equal(parts[k], after[k]), (s, k)
+
+
+def test_batch_step_from_host_frames(ctx):
+    """batch_step from host frames (gray, and 3-channel with channel 0 the
+    gray frame) equals batch_step from device frames, bit for bit."""
+    shapes = [(96, 128), (120, 160), (72, 96)]
+    params = cu.profile_params("B", 2)
+    video = [cu.synth_video(0xC0FFEE50 + i, r, c, 5, 1) for i, (r, c) in enumerate(shapes)]
+    rng = np.random.default_rng(3)
+    bd, bh, bc = (cu.Batch(ctx, shapes, params) for _ in range(3))
+    for f in range(5):
+        print(f"batch host frame {f}", flush=True)
+        dev = [ctx.upload(video[i][f]) for i in range(len(shapes))]
+        bd.step(dev)
+        bh.step([video[i][f] for i in range(len(shapes))])
+        rgb = []
+        for i, (r, c) in enumerate(shapes):
+            x = rng.integers(0, 256, (r, c, 3), dtype=np.uint8)
+            x[:, :, 0] = video[i][f]
+            rgb.append(x)
+        bc.step(rgb)
+        for d in dev:
+            d.free()
+    for i in range(len(shapes)):
+        for s in range(2):
+            want = bd.tracker(i).particles(s)
+            for b in (bh, bc):
+                got = b.tracker(i).particles(s)
+                assert got.tobytes() == want.tobytes(), (i, s)
+    for b in (bd, bh, bc):
+        b.close()
