@@ -472,6 +472,24 @@ def measured_traffic(path=TRAFFIC_PROFILE):
     return None
 
 
+# VALU wave instructions of the matcher launch, from a rocprofv3 SQ_INSTS_VALU
+# run of this benchmark's workload; the summary names the command.
+VALU_PROFILE = os.path.join(ROOT, "profiles", "lk_track_1080p_th8_valu.txt")
+# one wave64 VALU instruction per SIMD every 4 cycles: 256 CUs x 4 SIMDs x 2.4 GHz / 4
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
+
+
+def measured_valu(path=VALU_PROFILE):
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.strip().startswith("valu_instr_per_launch"):
+                    return int(line.split("=")[1].strip())
+    except OSError:
+        pass
+    return None
+
+
 def lk_roofline(tr):
     """Roofline entry of the matcher: algorithmic bytes per launch (one launch
     per frame step covers every scale) over its average measured duration."""
@@ -481,11 +499,19 @@ def lk_roofline(tr):
     avg_parts = [u / prof["frames"] for u in prof["per_scale"]]
     bytes_per_launch = lk_algorithmic_bytes(tr["shapes"], avg_parts)
     achieved = bytes_per_launch / (k["avg_us"] * 1e-6) / 1e9
+    valu = measured_valu()
+    valu_entry = None
+    if valu:
+        rate = valu / (k["avg_us"] * 1e-6)
+        valu_entry = {"instr_per_launch": valu, "achieved_ginstr_s": round(rate / 1e9, 1),
+                      "peak_ginstr_s": round(VALU_ISSUE_PEAK / 1e9, 1), "frac": round(rate / VALU_ISSUE_PEAK, 4),
+                      "model": "one wave64 VALU instruction per SIMD every 4 cycles, 256 CUs x 4 SIMDs at 2.4 GHz; "
+                               "instructions per launch from rocprofv3 SQ_INSTS_VALU"}
     return {"kernel": "lk_track", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(),
-            "bytes_per_launch": int(bytes_per_launch), "avg_us": k["avg_us"],
-            "note": "the matcher is bound by instruction latency, not HBM: bytes = both FP32 frames of every "
-                    "scale once + 40 B per particle matched; traffic from rocprofv3 FETCH_SIZE + WRITE_SIZE"}
+            "bytes_per_launch": int(bytes_per_launch), "avg_us": k["avg_us"], "valu": valu_entry,
+            "note": "the matcher is bound by VALU instruction issue, not HBM (see valu): bytes = both FP32 frames "
+                    "of every scale once + 40 B per particle matched; traffic from rocprofv3 FETCH_SIZE + WRITE_SIZE"}
 
 
 def main():

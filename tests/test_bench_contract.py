@@ -30,3 +30,17 @@ def test_traffic_from_counter_summary(tmp_path):
     p.write_text("header\ntraffic_bytes_per_launch = 1234\n")
     assert bench.measured_traffic(str(p)) == 1234
     assert bench.measured_traffic(str(tmp_path / "missing.txt")) is None
+
+
+def test_valu_from_counter_summary(tmp_path):
+    v = bench.measured_valu()
+    assert v is not None and v > 0
+    p = tmp_path / "summary.txt"
+    p.write_text("header\nvalu_instr_per_launch = 4321\n")
+    assert bench.measured_valu(str(p)) == 4321
+    assert bench.measured_valu(str(tmp_path / "missing.txt")) is None
+    tr = {"kernels": {"lk_track": {"avg_us": 100.0}}, "prof": {"frames": 1, "per_scale": [10, 0, 0, 0]},
+          "shapes": [(1080 >> s, 1920 >> s) for s in range(4)]}
+    r = bench.lk_roofline(tr)
+    assert r["bound"] == "hbm" and 0 < r["frac"] < 1
+    assert abs(r["valu"]["frac"] - v / 100e-6 / bench.VALU_ISSUE_PEAK) < 1e-3
