@@ -224,3 +224,17 @@ def test_lockstep_1080p_full_size(ctx, orc):
     assert st["tracked"] > 150000
     assert st["within"] >= 0.999 * st["tracked"]
     assert st["decision_mismatch"] <= 0.001 * st["decisions"]
+
+
+@pytest.mark.parametrize("shape,levels", [((251, 333), 3), ((131, 517), 4)])
+def test_lockstep_ragged_sizes(ctx, orc, shape, levels):
+    """Odd level sizes (floor halving), partial tiles and boxes at every level."""
+    nr, nc = shape
+    video = cu.synth_video(0xC0FFEE07, nr, nc, 6, 1)
+    st = run_lockstep(ctx, orc, video, oracle.profile_params("B", levels, 12.0),
+                      cu.profile_params("B", levels, 12.0), 6)
+    print(st)
+    assert st["tracked"] > 500
+    assert st["within"] >= 0.999 * st["tracked"]
+    assert st["decision_mismatch"] <= 0.001 * st["decisions"]
+    assert st["frames_exact"] >= 4
