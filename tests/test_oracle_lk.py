@@ -10,6 +10,8 @@ bookkeeping (selection with ties, the error's index and counter) bit-exactly.
 import numpy as np
 import pytest
 
+import oracle
+
 f32 = np.float32
 
 
@@ -196,3 +198,40 @@ def test_matches_independent_restatement(orc, profile):
         tr2, dist2 = np_match(A, B, G, p, pred, **kw)
         assert np.array_equal(tr, tr2), (trial, tr, tr2)
         assert np.float32(dist) == dist2, (trial, dist, dist2)
+
+
+def _texture(seed, nr, nc):
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    t = gaussian_filter(rng.normal(0, 1, (nr, nc)), 2.0)
+    t = (t - t.min()) / (t.max() - t.min()) * 255.0
+    return np.round(t).astype(np.uint8)
+
+
+def test_oracle_recursion_recovers_large_translation(orc):
+    """recursive_prediction (predictions.h:60-74, get_flow_at
+    pyrlk_tracker.hpp:1025-1063): a shift of (+12, -18) px is far beyond one
+    7x7 level; the 4-level coarse-to-fine recursion (prediction 2 x the coarser
+    flow) recovers it for the particles detected on the first frame, while the
+    same frames on one level do not."""
+    canvas = _texture(11, 400, 500)
+    dy, dx = 12, -18
+    f0 = np.ascontiguousarray(canvas[60:300, 60:380])
+    f1 = np.ascontiguousarray(canvas[60 - dy:300 - dy, 60 - dx:380 - dx])  # content moves by (dy, dx)
+    t = orc.tracker(240, 320, oracle.profile_params("B", 4, 8.0))
+    t.run(f0)
+    t.run(f1)
+    p = t.particles(0)
+    moved = p[p["age"] >= 2]
+    assert len(moved) > 100
+    sr, sc = moved["speed_r"].astype(np.float64), moved["speed_c"].astype(np.float64)
+    assert abs(np.median(sr) - dy) < 0.05 and abs(np.median(sc) - dx) < 0.05
+    close = (np.abs(sr - dy) < 0.5) & (np.abs(sc - dx) < 0.5)
+    assert close.mean() > 0.5  # the rest see content that entered the frame
+    # the same frames on one level: the shift is out of the matcher's reach
+    t1 = orc.tracker(240, 320, oracle.profile_params("B", 1, 8.0))
+    t1.run(f0)
+    t1.run(f1)
+    q = t1.particles(0)
+    q = q[q["age"] >= 2]
+    assert abs(np.median(q["speed_r"]) - dy) > 5 and abs(np.median(q["speed_c"]) - dx) > 5
