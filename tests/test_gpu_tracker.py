@@ -238,3 +238,12 @@ def test_lockstep_ragged_sizes(ctx, orc, shape, levels):
     assert st["within"] >= 0.999 * st["tracked"]
     assert st["decision_mismatch"] <= 0.001 * st["decisions"]
     assert st["frames_exact"] >= 4
+
+
+def test_static_video_known_answer(ctx):
+    """The oracle's static-video invariants (tests/test_oracle_lk.py) on the GPU
+    tracker."""
+    import test_oracle_lk as t
+    frame = t._texture(5, 160, 224)
+    g = cu.Tracker(ctx, 160, 224, cu.profile_params("B", 3, 20.0))
+    t._static_invariants(g, 3, 5, frame)
