@@ -14,8 +14,15 @@ Parity status of the restated functions (what pins each one):
   * FAST saliency: known-answer corners and an independent Python restatement;
   * LK matcher: bilinear exactness, zero motion, recovered translations,
     degenerate-G failure, rejection branches;
-  * particle container / merge: scripted sequences and an independent Python
-    restatement of the serial semantics;
+  * detector mask and box selection: independent Python restatements
+    (Chebyshev-distance mask, numpy first-occurrence argmax per box);
+  * particle container add / compact / merge: independent Python restatements
+    of the serial semantics and the reference's own assert after compaction;
+  * coarse-to-fine recursion: a known large translation recovered on 4 levels
+    and not on one;
+  * frame step and container moves: the static-video known answer; beyond it
+    (the interplay of matching, detection and merging over moving frames)
+    parity unpinned, checked only through the GPU agreeing with it in lockstep;
   * the photometric loss gradient: central finite differences.
 """
 import ctypes
