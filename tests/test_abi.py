@@ -60,3 +60,24 @@ def test_no_device_is_reported_not_crashing():
     except cu.CuimgError:
         return
     raise AssertionError("Context() succeeded without a device")
+
+
+def test_entry_points_reject_null_handles():
+    """Every stateful entry point checks its handle before any device call:
+    NULL contexts, trackers and batches give an error code, not a crash."""
+    L = cu.lib()
+    img = cu.Img()
+    n = ctypes.c_uint32()
+    calls = [
+        lambda: L.cuimg_hip_particle_mask(None, None, 0, ctypes.byref(img)),
+        lambda: L.cuimg_hip_fast_select(None, ctypes.byref(img), ctypes.byref(img), 3, 20.0, None, 0, ctypes.byref(n)),
+        lambda: L.cuimg_hip_particles_add(None, None, 0, 0, None, 0, ctypes.byref(img), ctypes.byref(n)),
+        lambda: L.cuimg_hip_particles_compact(None, None, 0, None, None, ctypes.byref(img), ctypes.byref(n)),
+        lambda: L.cuimg_hip_particles_merge(None, None, 0, ctypes.byref(img)),
+        lambda: L.cuimg_hip_tracker_run_host(None, None, 0),
+        lambda: L.cuimg_hip_tracker_run_host_rgb(None, None, 0, 3),
+        lambda: L.cuimg_hip_batch_step_host(None, None, None, 1),
+        lambda: L.cuimg_hip_pyr_down(None, None, None, 0),
+    ]
+    for call in calls:
+        assert call() != 0
