@@ -363,6 +363,7 @@ class Tracker:
         self.L = ctx.L
         self.params = params
         self.nscales = params.nscales
+        self.shape = (nrows, ncols)
         if _view is not None:
             self.h, self._owner = _view, False
             return
@@ -373,8 +374,8 @@ class Tracker:
 
     def close(self):
         if getattr(self, "h", None):
-            # after its context is closed the tracker is only forgotten: its
-            # destroy call would use the closed context
+            # closing the context destroyed the tracker already: it is only
+            # forgotten then
             if self._owner and self.ctx.h:
                 self.L.cuimg_hip_tracker_destroy(self.h)
             self.h = None
@@ -397,14 +398,11 @@ class Tracker:
         if isinstance(frame, (DeviceImage, View)):
             self.ctx.check(self.L.cuimg_hip_tracker_run(self.h, ctypes.byref(frame.d)), "tracker_run")
         else:
-            frame = np.asarray(frame, np.uint8)
+            frame = host_frame(frame, self.shape)
             if frame.ndim == 3:  # interleaved channels: channel 0 is the gray frame (get_x)
-                frame = np.ascontiguousarray(frame)
                 self.ctx.check(self.L.cuimg_hip_tracker_run_host_rgb(self.h, frame.ctypes.data, frame.strides[0],
                                                                      frame.shape[2]), "tracker_run_host_rgb")
                 return
-            if frame.strides[1] != 1:
-                frame = np.ascontiguousarray(frame)
             self.ctx.check(self.L.cuimg_hip_tracker_run_host(self.h, frame.ctypes.data, frame.strides[0]),
                            "tracker_run_host")
 
@@ -490,6 +488,27 @@ class Tracker:
         return v.value
 
 
+def host_frame(frame, shape):
+    """A host frame as the C ABI reads it: u8 rows of (nrows, ncols) pixels
+    (2-D) or of interleaved channels (3-D, 1/3/4 channels), each row contiguous
+    and rows at a positive pitch.  Anything else (a short or long frame, a
+    flipped or strided view) is copied or rejected here, never over-read."""
+    frame = np.asarray(frame)
+    if frame.dtype != np.uint8:
+        raise ValueError(f"host frame must be uint8, not {frame.dtype}")
+    if frame.ndim not in (2, 3) or tuple(frame.shape[:2]) != tuple(shape):
+        raise ValueError(f"host frame of shape {frame.shape} for a {shape[0]}x{shape[1]} tracker")
+    if frame.ndim == 3 and frame.shape[2] not in (1, 3, 4):
+        raise ValueError(f"host frame with {frame.shape[2]} channels (1, 3 or 4)")
+    if frame.ndim == 3 and frame.shape[2] == 1:
+        frame = frame[:, :, 0]
+    inner = frame.shape[2] if frame.ndim == 3 else 1
+    row_contiguous = frame.strides[1] == inner and (frame.ndim == 2 or frame.strides[2] == 1)
+    if frame.strides[0] < frame.shape[1] * inner or not row_contiguous:
+        frame = np.ascontiguousarray(frame)
+    return frame
+
+
 class Batch:
     """Independent trackers stepped together (cuimg_hip_batch): shapes is a list
     of (nrows, ncols), params a TrackerParams or one per stream."""
@@ -530,13 +549,12 @@ class Batch:
             arr = (Img * self.n)(*[f.d for f in frames])
             self.ctx.check(self.L.cuimg_hip_batch_step(self.h, arr), "batch_step")
             return
-        hs = []
-        for f in frames:
-            f = np.asarray(f, np.uint8)
-            if f.ndim == 3 or f.strides[-1] != 1:
-                f = np.ascontiguousarray(f)
-            hs.append(f)
+        if len(frames) != self.n:
+            raise ValueError(f"{len(frames)} frames for {self.n} streams")
+        hs = [host_frame(f, shape) for f, shape in zip(frames, self.shapes)]
         ch = hs[0].shape[2] if hs[0].ndim == 3 else 1
+        if any((f.shape[2] if f.ndim == 3 else 1) != ch for f in hs):
+            raise ValueError("every stream's host frame needs the same channel count")
         ptrs = (ctypes.c_void_p * self.n)(*[f.ctypes.data for f in hs])
         pitches = (ctypes.c_size_t * self.n)(*[f.strides[0] for f in hs])
         self.ctx.check(self.L.cuimg_hip_batch_step_host(self.h, ptrs, pitches, ch), "batch_step_host")

@@ -266,7 +266,12 @@ __global__ void zero_kernel(float* p, size_t n) {
 int lk_loss_adjoint(cuimg_hip_ctx* ctx, const cuimg_img* I0, const cuimg_img* I1, const float* pos2,
                     const float* d2, uint32_t n, int sigma, int K, int ws, double* loss_total, float* loss_p,
                     float* dL_dd2, const cuimg_img* dL_dI1, float* scratch, size_t scratch_floats) {
-  if (!I0 || !I1 || I0->dtype != CUIMG_F32 || I1->dtype != CUIMG_F32 || I0->nrows != I1->nrows ||
+  if (!ctx) return CUIMG_E_INVALID;
+  if (!loss_total || !loss_p || !dL_dd2 || !scratch || (n && (!pos2 || !d2)))
+    return set_error(ctx, CUIMG_E_INVALID, "lk_loss_adjoint: null output, scratch or particle array");
+  if (!I0 || !I1 || !I0->begin || !I1->begin || (dL_dI1 && !dL_dI1->begin))
+    return set_error(ctx, CUIMG_E_INVALID, "lk_loss_adjoint: null image");
+  if (I0->dtype != CUIMG_F32 || I1->dtype != CUIMG_F32 || I0->nrows != I1->nrows ||
       I0->ncols != I1->ncols)
     return set_error(ctx, CUIMG_E_INVALID, "lk_loss_adjoint: I0, I1 must be F32 of one domain");
   if (ws < 1 || ws > kAdjMaxWs || !(ws & 1) || K < 1 || K > kAdjMaxK)

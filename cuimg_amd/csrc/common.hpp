@@ -32,6 +32,9 @@ struct cuimg_hip_ctx {
   std::map<std::string, std::pair<int64_t, double>> stats;
   // span timer (cuimg_hip_timer_start / _stop)
   hipEvent_t span[2] = {nullptr, nullptr};
+  // trackers and batches made from this context and not yet destroyed, with
+  // their destroy functions: cuimg_hip_ctx_destroy destroys them first
+  std::vector<std::pair<void*, void (*)(void*)>> children;
 };
 
 namespace cuimg {
@@ -80,6 +83,15 @@ inline int launch(cuimg_hip_ctx* ctx, const char* name, F&& f) {
 }
 
 inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+inline void add_child(cuimg_hip_ctx* ctx, void* p, void (*destroy)(void*)) { ctx->children.emplace_back(p, destroy); }
+inline void drop_child(cuimg_hip_ctx* ctx, void* p) {
+  for (size_t i = 0; i < ctx->children.size(); i++)
+    if (ctx->children[i].first == p) {
+      ctx->children.erase(ctx->children.begin() + (long)i);
+      return;
+    }
+}
 
 // Entry points shared between translation units.
 int image_alloc(cuimg_hip_ctx* ctx, int nrows, int ncols, int border, int dtype, cuimg_img* out);

@@ -175,6 +175,13 @@ int cuimg_hip_ctx_create(int device, cuimg_hip_ctx** out) {
 
 int cuimg_hip_ctx_destroy(cuimg_hip_ctx* ctx) {
   if (!ctx) return 0;
+  // trackers and batches still alive go first (newest first): they free their
+  // own buffers, streams and events through this context
+  while (!ctx->children.empty()) {
+    const auto child = ctx->children.back();
+    ctx->children.pop_back();
+    child.second(child.first);
+  }
   (void)hipStreamSynchronize(ctx->stream);
   for (auto& kv : ctx->image_base) (void)hipFree(kv.second);
   for (auto& t : ctx->pending) {

@@ -432,6 +432,7 @@ static int dev_alloc_n(cuimg_hip_batch* b, size_t n, T** out) {
 
 static void destroy_batch(cuimg_hip_batch* b) {
   if (!b) return;
+  drop_child(b->ctx, b);
   (void)hipStreamSynchronize(b->ctx->stream);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   for (auto& R : b->lv) {
@@ -473,6 +474,9 @@ static int check_params(cuimg_hip_ctx* ctx, int nrows, int ncols, const cuimg_tr
     return set_error(ctx, CUIMG_E_UNSUPPORTED, "tracker: detector box %d not in 1..%d", P->box_size, kMaxBox);
   if (P->detector_frequency < 1 || P->filtering_frequency < 1 || P->fast_th < 0.f || P->fast_n < 1)
     return set_error(ctx, CUIMG_E_INVALID, "tracker: bad detector / filtering settings");
+  if (P->lk_top_n < 1 || P->lk_max_iters < 0)
+    return set_error(ctx, CUIMG_E_INVALID, "tracker: LK top_n %d (>= 1) / max_iters %d (>= 0)", P->lk_top_n,
+                     P->lk_max_iters);
   if (P->profile != 0 && P->profile != 1)
     return set_error(ctx, CUIMG_E_INVALID, "tracker: profile %d not in {0,1}", P->profile);
   if (P->profile == 1 && (P->blur_K < 1 || P->blur_K > kMaxK || P->blur_sigma < 1))
@@ -581,7 +585,8 @@ static void build_job(cuimg_hip_batch* b, int i, int s) {
       a.pd_c1 = 3;
       a.pd_r2 = R.nr - 4;
       a.pd_c2 = R.nc - 4;
-      a.P = lk_params_dev{P.lk_ws, P.lk_max_iters, P.lk_top_n, P.lk_eps, 1e-4f, R.sp.k, P.profile == 0 ? 1 : 0};
+      a.P = lk_params_dev{P.lk_ws, P.lk_max_iters, std::min(P.lk_top_n, P.lk_ws * P.lk_ws), P.lk_eps, 1e-4f, R.sp.k,
+                          P.profile == 0 ? 1 : 0};
     }
   for (int cur = 0; cur < 2; cur++) {
     detect_args& d = J.det[cur];
@@ -993,7 +998,9 @@ extern "C" {
 // ------------------------------------------------------------------ batch
 int cuimg_hip_batch_create(cuimg_hip_ctx* ctx, int nstreams, const int* nrows, const int* ncols,
                            const cuimg_tracker_params* params, cuimg_hip_batch** out) {
-  return create_batch(ctx, nstreams, nrows, ncols, params, out);
+  CUIMG_TRY(create_batch(ctx, nstreams, nrows, ncols, params, out));
+  add_child(ctx, *out, [](void* p) { destroy_batch(static_cast<cuimg_hip_batch*>(p)); });
+  return 0;
 }
 
 int cuimg_hip_batch_destroy(cuimg_hip_batch* b) {
@@ -1066,6 +1073,11 @@ int cuimg_hip_tracker_create(cuimg_hip_ctx* ctx, int nrows, int ncols, const cui
   t->b = b;
   t->stream = 0;
   t->owner = true;
+  add_child(ctx, t, [](void* p) {
+    auto* tr = static_cast<cuimg_hip_tracker*>(p);
+    destroy_batch(tr->b);
+    delete tr;
+  });
   *out = t;
   return 0;
 }
@@ -1073,6 +1085,7 @@ int cuimg_hip_tracker_create(cuimg_hip_ctx* ctx, int nrows, int ncols, const cui
 int cuimg_hip_tracker_destroy(cuimg_hip_tracker* t) {
   if (!t) return 0;
   if (t->owner) {
+    drop_child(t->b->ctx, t);
     destroy_batch(t->b);
     delete t;
   }

@@ -64,6 +64,9 @@ typedef struct cuimg_hip_ctx cuimg_hip_ctx;
 
 /* ---------------------------------------------------------------- context */
 int cuimg_hip_ctx_create(int device, cuimg_hip_ctx** out);
+/* Also destroys every tracker and batch made from the context that is still
+ * alive (their handles are invalid afterwards); images allocated through the
+ * context are freed. */
 int cuimg_hip_ctx_destroy(cuimg_hip_ctx* ctx);
 const char* cuimg_hip_last_error(const cuimg_hip_ctx* ctx);
 int cuimg_hip_sync(cuimg_hip_ctx* ctx);

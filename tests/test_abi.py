@@ -78,6 +78,8 @@ def test_entry_points_reject_null_handles():
         lambda: L.cuimg_hip_tracker_run_host_rgb(None, None, 0, 3),
         lambda: L.cuimg_hip_batch_step_host(None, None, None, 1),
         lambda: L.cuimg_hip_pyr_down(None, None, None, 0),
+        lambda: L.cuimg_hip_lk_loss_adjoint(None, ctypes.byref(img), ctypes.byref(img), None, None, 0, 1, 3, 7, None,
+                                            None, None, None, None, 0),
     ]
     for call in calls:
         assert call() != 0
