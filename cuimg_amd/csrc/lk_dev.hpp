@@ -32,21 +32,34 @@
 // re-solves its synthetic ancestors at scales L-1 .. s+1, as the reference's
 // recursion does, each solve feeding the next one's prediction.
 //
-// Numerics: per-sample arithmetic (bilinear weights, samples, Sobel) is written
-// without FMA contraction in the oracle's operation order, so template and
-// image samples are bit-identical to the CPU oracle; only the window sums are
-// reduced in tree order instead of raster order.
+// Numerics: the arithmetic is fixed operation by operation (contraction of a
+// product into an add is off in this file, every FMA is written out):
+//  * bilinear weights a0 = q - trunc(q), 1 - a0, ..., w00 = (1-a0)(1-a1), ...
+//    each rounded; a sample is w00 v00, then fma(w10, v10, .), fma(w01, v01, .),
+//    fma(w11, v11, .);
+//  * Sobel sums, the 2x2 solve, the tests and the updates: each operation
+//    rounded, in the reference's order;
+//  * window sums: lane l of a group accumulates its pixels l, l + GL, ... in
+//    ascending order (products with fma), then the group's lanes are added
+//    pairwise by the DPP reductions below.
+// The oracle's "GPU arithmetic" mode performs exactly these operations, so the
+// two agree bit for bit.
 #pragma once
 
 #include "tracker_state.hpp"
 
 #include <type_traits>
 
+#pragma clang fp contract(off)
+
 namespace cuimg {
 
-__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
-__device__ __forceinline__ float add_rn(float a, float b) { return __fadd_rn(a, b); }
-__device__ __forceinline__ float sub_rn(float a, float b) { return __fsub_rn(a, b); }
+// Single rounded operations (written here, under contraction off: the HIP
+// header's __fmul_rn / __fadd_rn are plain operators compiled with
+// contraction on, so a product of theirs can still be fused into an add).
+__device__ __forceinline__ float mul_rn(float a, float b) { return a * b; }
+__device__ __forceinline__ float add_rn(float a, float b) { return a + b; }
+__device__ __forceinline__ float sub_rn(float a, float b) { return a - b; }
 
 template <int CTRL>
 __device__ __forceinline__ float mov_dpp(float x) {
@@ -131,12 +144,13 @@ __device__ __forceinline__ bw weights(float qr, float qc) {
   return w;
 }
 
-// li's left-to-right sum (((w00 v00 + w10 v10) + w01 v01) + w11 v11).
+// li's left-to-right sum w00 v00 + w10 v10 + w01 v01 + w11 v11, the last
+// three terms fused.
 __device__ __forceinline__ float blend(const bw& w, float v00, float v10, float v01, float v11) {
   float s = mul_rn(w.w00, v00);
-  s = add_rn(s, mul_rn(w.w10, v10));
-  s = add_rn(s, mul_rn(w.w01, v01));
-  s = add_rn(s, mul_rn(w.w11, v11));
+  s = fmaf(w.w10, v10, s);
+  s = fmaf(w.w01, v01, s);
+  s = fmaf(w.w11, v11, s);
   return s;
 }
 
@@ -401,9 +415,9 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
 #pragma unroll
   for (int k = 0; k < NS; k++) {
     if (sel[k]) {
-      g00 += gs[k].x * gs[k].x;
-      g01 += gs[k].x * gs[k].y;
-      g11 += gs[k].y * gs[k].y;
+      g00 = fmaf(gs[k].x, gs[k].x, g00);
+      g01 = fmaf(gs[k].x, gs[k].y, g01);
+      g11 = fmaf(gs[k].y, gs[k].y, g11);
     }
     cnt += __popcll(group_ballot<GL>(sel[k]));
   }
@@ -450,8 +464,8 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
     for (int k = 0; k < NS; k++)
       if (sel[k]) {
         const float dt = sub_rn(as[k], sampleB(k));
-        b0 += gs[k].x * dt;
-        b1 += gs[k].y * dt;
+        b0 = fmaf(gs[k].x, dt, b0);
+        b1 = fmaf(gs[k].y, dt, b1);
       }
     b0 = group_sum<GL>(b0);
     b1 = group_sum<GL>(b1);
@@ -471,7 +485,7 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
   for (int k = 0; k < NS; k++)
     if (W.in[k]) {
       const int idx = (W.r[k] + hws) * hws + (W.c[k] + hws);
-      e += fabsf(sub_rn(S.tpl[idx], sampleB(k)));
+      e = add_rn(e, fabsf(sub_rn(S.tpl[idx], sampleB(k))));
     }
   const float err = group_sum<GL>(e);
   match_out o;
@@ -630,3 +644,5 @@ inline void lk_variant(int ws, int top_n, int image_u8, F&& f) {
 }
 
 }  // namespace cuimg
+
+#pragma clang fp contract(fast)

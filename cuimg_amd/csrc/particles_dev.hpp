@@ -469,8 +469,10 @@ __device__ __forceinline__ void st64(unsigned long long* p, unsigned long long v
 }
 __device__ __forceinline__ uint32_t round_tag(uint32_t r) { return (255u - r) << 24; }
 
+// norml2 (cuimg/builtin_math.h:123-130): both squares rounded, then added.
 __device__ __forceinline__ float norml2_rn(float a, float b) {
-  return sqrtf(__fadd_rn(__fmul_rn(a, a), __fmul_rn(b, b)));
+#pragma clang fp contract(off)
+  return sqrtf(a * a + b * b);
 }
 
 __host__ __device__ inline uint32_t pow2_at_least(uint32_t n) {

@@ -5,24 +5,37 @@ tracking hot path (each function cites the reference file:line it follows).
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it,
 and only as the checker or as the timed CPU baseline; the product never does.
 
+Two arithmetic readings where the reference's rounding is not reproducible
+(it adds window terms in loop order with whatever contraction its compiler
+chose): the default rounds every product before adding it, in the reference's
+loop order; the "GPU arithmetic" mode (tracker sum_lanes 16 / 64, gaussian
+fused=True) performs the GPU's operations: fused multiply-adds for the blur
+taps and the last three bilinear terms, window sums in lane-group order.
+
 Parity status of the restated functions (what pins each one):
   * Gaussian coefficients: the reference's own generator, built from its source
     under oracle/_ref, plus a committed fixture of its output;
   * separable Gaussian: impulse / constant / ramp closed forms, brute-force 2D
-    sums, an FP64 twin;
+    sums, an FP64 twin; the fused mode by a Python restatement with exactly
+    rounded FMAs;
   * box pyramid: integer closed form;
   * FAST saliency: known-answer corners and an independent Python restatement;
   * LK matcher: bilinear exactness, zero motion, recovered translations,
-    degenerate-G failure, rejection branches;
+    degenerate-G failure, rejection branches, and bit-exact agreement with an
+    independent numpy restatement in both arithmetic readings;
   * detector mask and box selection: independent Python restatements
     (Chebyshev-distance mask, numpy first-occurrence argmax per box);
   * particle container add / compact / merge: independent Python restatements
     of the serial semantics and the reference's own assert after compaction;
   * coarse-to-fine recursion: a known large translation recovered on 4 levels
     and not on one;
-  * frame step and container moves: the static-video known answer; beyond it
-    (the interplay of matching, detection and merging over moving frames)
-    parity unpinned, checked only through the GPU agreeing with it in lockstep;
+  * frame step and container moves: the static-video known answer; on an
+    exact integer-translation video with detector_frequency 3 and
+    filtering_frequency 2, the closed forms of the detection / compaction /
+    merge cadence, ages, positions (birth pixel + age x shift) and the sparse
+    image's last writer; crafted merges whose outcome follows by hand from the
+    reference's merge loop and c8 order; committed end-to-end fixtures of
+    the 320x240 video (regression pin);
   * the photometric loss gradient: central finite differences.
 """
 import ctypes
@@ -53,7 +66,13 @@ class TrackerParams(ctypes.Structure):
                 ("box_size", ctypes.c_int32), ("detector_frequency", ctypes.c_int32),
                 ("filtering_frequency", ctypes.c_int32), ("k", ctypes.c_int32),
                 ("with_merge", ctypes.c_int32), ("blur_sigma", ctypes.c_int32),
-                ("blur_K", ctypes.c_int32)]
+                ("blur_K", ctypes.c_int32), ("sum_lanes", ctypes.c_int32)]
+
+
+def gpu_sum_lanes(ws):
+    """Lanes per particle of the GPU matcher for a window side: the oracle's
+    window sums in that group order reproduce the GPU's rounding."""
+    return 16 if ws <= 7 else 64
 
 
 def profile_params(profile, nscales, fast_th=20.0, **kw):
@@ -78,6 +97,7 @@ def profile_params(profile, nscales, fast_th=20.0, **kw):
     p.with_merge = 1
     p.blur_sigma = 1
     p.blur_K = 3
+    p.sum_lanes = 0
     for k, v in kw.items():
         setattr(p, k, v)
     return p
@@ -100,8 +120,8 @@ class Oracle:
         L.orc_gauss_coef_meta.restype = c_float
         L.orc_gauss_coef_meta.argtypes = [c_int, c_int]
         L.orc_gauss_coefs.argtypes = [c_int, c_int, c_int, P(c_float)]
-        L.orc_gaussian_sep_f32.argtypes = [P(c_float), c_int, c_int, c_int, P(c_float), c_int, P(c_float)]
-        L.orc_gaussian_sep_u8.argtypes = [P(ctypes.c_uint8), c_int, c_int, P(c_float), c_int, P(c_float)]
+        L.orc_gaussian_sep_f32.argtypes = [P(c_float), c_int, c_int, c_int, P(c_float), c_int, c_int, P(c_float)]
+        L.orc_gaussian_sep_u8.argtypes = [P(ctypes.c_uint8), c_int, c_int, P(c_float), c_int, c_int, P(c_float)]
         L.orc_gaussian_sep_f64.argtypes = [P(c_double), c_int, c_int, c_int, P(c_double), c_int, P(c_double)]
         L.orc_pyr_down_u8.argtypes = [P(ctypes.c_uint8), c_int, c_int, P(ctypes.c_uint8)]
         L.orc_pyr_down_f32.argtypes = [P(c_float), c_int, c_int, P(c_float)]
@@ -111,7 +131,7 @@ class Oracle:
                                         P(ctypes.c_uint8)]
         L.orc_sobel_gradient.argtypes = [P(c_float), c_int, c_int, c_int, P(c_float)]
         L.orc_lk_match.argtypes = [P(c_float), P(c_float), P(c_float), c_int, c_int, c_int, c_int,
-                                   c_float, c_int, c_int, c_float, c_float, c_float, c_float,
+                                   c_float, c_int, c_int, c_int, c_float, c_float, c_float, c_float,
                                    P(c_float), P(c_float)]
         L.orc_tracker_create.restype = vp
         L.orc_tracker_create.argtypes = [c_int, c_int, P(TrackerParams)]
@@ -161,19 +181,21 @@ class Oracle:
         self.L.orc_gauss_coefs(int(sigma), int(K), 1 if meta else 0, _ptr(out, ctypes.c_float))
         return out
 
-    def gaussian(self, img, coef, K):
+    def gaussian(self, img, coef, K, fused=False):
+        """fused=True: taps added with fused multiply-adds (the GPU's arithmetic)."""
         coef = np.ascontiguousarray(coef, np.float32)
+        f = 1 if fused else 0
         if img.dtype == np.uint8:
             img = np.ascontiguousarray(img)
             out = np.zeros(img.shape, np.float32)
             self.L.orc_gaussian_sep_u8(_ptr(img, ctypes.c_uint8), img.shape[0], img.shape[1],
-                                       _ptr(coef, ctypes.c_float), K, _ptr(out, ctypes.c_float))
+                                       _ptr(coef, ctypes.c_float), K, f, _ptr(out, ctypes.c_float))
             return out
         img = np.ascontiguousarray(img, np.float32)
         ch = 1 if img.ndim == 2 else img.shape[2]
         out = np.zeros(img.shape, np.float32)
         self.L.orc_gaussian_sep_f32(_ptr(img, ctypes.c_float), img.shape[0], img.shape[1], ch,
-                                    _ptr(coef, ctypes.c_float), K, _ptr(out, ctypes.c_float))
+                                    _ptr(coef, ctypes.c_float), K, f, _ptr(out, ctypes.c_float))
         return out
 
     def gaussian_f64(self, img, coef, K):
@@ -226,14 +248,14 @@ class Oracle:
                                   1 if integer_source else 0, _ptr(out, ctypes.c_float))
         return out
 
-    def lk_match(self, A, B, G, p, pred, ws=7, max_iters=5, eps=0.1, top_n=49, template_u8=0):
+    def lk_match(self, A, B, G, p, pred, ws=7, max_iters=5, eps=0.1, top_n=49, template_u8=0, sum_lanes=0):
         A = np.ascontiguousarray(A, np.float32)
         B = np.ascontiguousarray(B, np.float32)
         G = np.ascontiguousarray(G, np.float32)
         tr = np.zeros(2, np.float32)
         dist = np.zeros(1, np.float32)
         self.L.orc_lk_match(_ptr(A, ctypes.c_float), _ptr(B, ctypes.c_float), _ptr(G, ctypes.c_float),
-                            A.shape[0], A.shape[1], ws, max_iters, eps, top_n, template_u8,
+                            A.shape[0], A.shape[1], ws, max_iters, eps, top_n, template_u8, sum_lanes,
                             float(p[0]), float(p[1]), float(pred[0]), float(pred[1]),
                             _ptr(tr, ctypes.c_float), _ptr(dist, ctypes.c_float))
         return tr, float(dist[0])

@@ -29,19 +29,19 @@ void orc_gauss_coefs(int sigma, int K, int meta_rounding, float* out) {
   gauss_coefs(sigma, K, meta_rounding != 0, out);
 }
 
-void orc_gaussian_sep_f32(const float* in, int nr, int nc, int ch, const float* coef, int K,
+void orc_gaussian_sep_f32(const float* in, int nr, int nc, int ch, const float* coef, int K, int fused,
                           float* out) {
   for (int k = 0; k < ch; k++) {
     image<float> a(nr, nc), o;
     for (size_t i = 0; i < a.px.size(); i++) a.px[i] = in[i * ch + k];
-    gaussian_sep_f32<float>(a, coef, K, o);
+    gaussian_sep_f32<float>(a, coef, K, o, fused != 0);
     for (size_t i = 0; i < o.px.size(); i++) out[i * ch + k] = o.px[i];
   }
 }
 
-void orc_gaussian_sep_u8(const uint8_t* in, int nr, int nc, const float* coef, int K, float* out) {
+void orc_gaussian_sep_u8(const uint8_t* in, int nr, int nc, const float* coef, int K, int fused, float* out) {
   image<float> o;
-  gaussian_sep_f32<uint8_t>(wrap(in, nr, nc), coef, K, o);
+  gaussian_sep_f32<uint8_t>(wrap(in, nr, nc), coef, K, o, fused != 0);
   std::memcpy(out, o.px.data(), sizeof(float) * o.px.size());
 }
 
@@ -85,7 +85,7 @@ void orc_sobel_gradient(const float* prev, int nr, int nc, int integer_source, f
 }
 
 void orc_lk_match(const float* A, const float* B, const float* G, int nr, int nc, int ws,
-                  int max_iters, float eps, int top_n, int template_u8, float p_r, float p_c,
+                  int max_iters, float eps, int top_n, int template_u8, int sum_lanes, float p_r, float p_c,
                   float pred_r, float pred_c, float* out_tr2, float* out_dist) {
   image<float> a = wrap(A, nr, nc), b = wrap(B, nr, nc);
   image<f2> g = wrap(reinterpret_cast<const f2*>(G), nr, nc);
@@ -95,6 +95,7 @@ void orc_lk_match(const float* A, const float* B, const float* G, int nr, int nc
   P.eps = eps;
   P.top_n = top_n;
   P.template_u8 = template_u8;
+  P.group_lanes = sum_lanes;
   lk_level L{&a, &b, &g};
   match_result m = lk_match(P, f2{p_r, p_c}, f2{pred_r, pred_c}, L);
   out_tr2[0] = m.tr.r;
@@ -111,6 +112,7 @@ static tracker_params to_params(const orc_tracker_params* p) {
   t.lk.eps = p->lk_eps;
   t.lk.top_n = p->lk_top_n;
   t.lk.template_u8 = p->profile == 0 ? 1 : 0;
+  t.lk.group_lanes = p->sum_lanes;
   t.fast_n = p->fast_n;
   t.fast_th = p->fast_th;
   t.box_size = p->box_size;

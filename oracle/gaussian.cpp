@@ -2,6 +2,7 @@
 // TEST INFRASTRUCTURE ONLY (see oracle_core.hpp).
 #include "oracle_core.hpp"
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 
@@ -43,15 +44,19 @@ void gauss_coefs(int sigma, int K, bool meta_rounding, float* out) {
 // x = -K..K, clamp-to-edge reads (texture clamp addressing), FP32 intermediate.
 // The recursive template  v_R*c_R + iter<R+1>  associates right to left:
 //   c_{-K}v_{-K} + (c_{-K+1}v_{-K+1} + (... + c_K v_K)).
+// fused: every term after the first (tap +K) is added with one fused
+// multiply-add, s = fma(c_j, v_j, s) (the GPU's arithmetic); otherwise each
+// product is rounded before it is added.
 template <typename TI>
-void gaussian_sep_f32(const image<TI>& in, const float* coef, int K, image<float>& out) {
+void gaussian_sep_f32(const image<TI>& in, const float* coef, int K, image<float>& out, bool fused) {
   const int nr = in.nr, nc = in.nc;
   image<float> tmp(nr, nc);
 #pragma omp parallel for schedule(static, 2)
   for (int r = 0; r < nr; r++)
     for (int c = 0; c < nc; c++) {
       float s = (float)in.clamped(r, c + K) * coef[2 * K];
-      for (int j = K - 1; j >= -K; j--) s = (float)in.clamped(r, c + j) * coef[j + K] + s;
+      for (int j = K - 1; j >= -K; j--)
+        s = fused ? std::fmaf(coef[j + K], (float)in.clamped(r, c + j), s) : (float)in.clamped(r, c + j) * coef[j + K] + s;
       tmp(r, c) = s;
     }
   out = image<float>(nr, nc);
@@ -59,12 +64,13 @@ void gaussian_sep_f32(const image<TI>& in, const float* coef, int K, image<float
   for (int r = 0; r < nr; r++)
     for (int c = 0; c < nc; c++) {
       float s = tmp.clamped(r + K, c) * coef[2 * K];
-      for (int j = K - 1; j >= -K; j--) s = tmp.clamped(r + j, c) * coef[j + K] + s;
+      for (int j = K - 1; j >= -K; j--)
+        s = fused ? std::fmaf(coef[j + K], tmp.clamped(r + j, c), s) : tmp.clamped(r + j, c) * coef[j + K] + s;
       out(r, c) = s;
     }
 }
-template void gaussian_sep_f32<float>(const image<float>&, const float*, int, image<float>&);
-template void gaussian_sep_f32<uint8_t>(const image<uint8_t>&, const float*, int, image<float>&);
+template void gaussian_sep_f32<float>(const image<float>&, const float*, int, image<float>&, bool);
+template void gaussian_sep_f32<uint8_t>(const image<uint8_t>&, const float*, int, image<float>&, bool);
 
 // FP64 twin of the same operation order, used to bound FP32 error.
 void gaussian_sep_f64(const image<double>& in, const double* coef, int K, image<double>& out) {

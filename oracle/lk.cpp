@@ -1,41 +1,48 @@
 // Oracle: pyramid Lucas-Kanade matcher.  TEST INFRASTRUCTURE ONLY (see oracle_core.hpp).
 #include "oracle_core.hpp"
 
+#include <algorithm>
+#include <cmath>
+
 namespace orc {
 
 // Bilinear sample `li` (tracking2/pyrlk_tracker.hpp:130-146): base pixel is the
 // coordinate truncated toward zero, weights are the (possibly negative)
-// remainders, terms summed left to right.
-static inline float li(const image<float>& img, f2 p) {
+// remainders, terms summed left to right.  fused: the last three terms are
+// added with fused multiply-adds (the GPU's arithmetic); otherwise every
+// product is rounded first.
+static inline float blend4(float w00, float w10, float w01, float w11, float v00, float v10, float v01, float v11,
+                           bool fused) {
+  float s = w00 * v00;
+  if (fused) {
+    s = std::fmaf(w10, v10, s);
+    s = std::fmaf(w01, v01, s);
+    return std::fmaf(w11, v11, s);
+  }
+  s = s + w10 * v10;
+  s = s + w01 * v01;
+  return s + w11 * v11;
+}
+
+static inline float li(const image<float>& img, f2 p, bool fused) {
   int xr = tz(p.r), xc = tz(p.c);
   float a0 = p.r - (float)xr;
   float a1 = p.c - (float)xc;
   float v00 = img.clamped(xr, xc), v10 = img.clamped(xr + 1, xc);
   float v01 = img.clamped(xr, xc + 1), v11 = img.clamped(xr + 1, xc + 1);
-  float s = (1 - a0) * (1 - a1) * v00;
-  s = s + a0 * (1 - a1) * v10;
-  s = s + (1 - a0) * a1 * v01;
-  s = s + a0 * a1 * v11;
-  return s;
+  float w00 = (1 - a0) * (1 - a1), w10 = a0 * (1 - a1), w01 = (1 - a0) * a1, w11 = a0 * a1;
+  return blend4(w00, w10, w01, w11, v00, v10, v01, v11, fused);
 }
 
-static inline f2 li2(const image<f2>& img, f2 p) {
+static inline f2 li2(const image<f2>& img, f2 p, bool fused) {
   int xr = tz(p.r), xc = tz(p.c);
   float a0 = p.r - (float)xr;
   float a1 = p.c - (float)xc;
   f2 v00 = img.clamped(xr, xc), v10 = img.clamped(xr + 1, xc);
   f2 v01 = img.clamped(xr, xc + 1), v11 = img.clamped(xr + 1, xc + 1);
   float w00 = (1 - a0) * (1 - a1), w10 = a0 * (1 - a1), w01 = (1 - a0) * a1, w11 = a0 * a1;
-  f2 s;
-  s.r = w00 * v00.r;
-  s.r = s.r + w10 * v10.r;
-  s.r = s.r + w01 * v01.r;
-  s.r = s.r + w11 * v11.r;
-  s.c = w00 * v00.c;
-  s.c = s.c + w10 * v10.c;
-  s.c = s.c + w01 * v01.c;
-  s.c = s.c + w11 * v11.c;
-  return s;
+  return f2{blend4(w00, w10, w01, w11, v00.r, v10.r, v01.r, v11.r, fused),
+            blend4(w00, w10, w01, w11, v00.c, v10.c, v01.c, v11.c, fused)};
 }
 
 // Sobel/8 gradient of the previous frame (tracking2/pyrlk_tracker.hpp:909-938):
@@ -69,6 +76,42 @@ void sobel_gradient(const image<float>& P, bool integer_source, image<f2>& g) {
     }
 }
 
+// Window sums in a defined order.  The reference adds the terms in its loop
+// order (raster, :231-250, :287-299, :320-333) with whatever contraction its
+// compiler chose, so its rounding is not reproducible; the oracle offers two
+// definitions:
+//  * lanes == 0: raster order, each product rounded, then added;
+//  * lanes == 16 or 64: pixel i goes to lane i % lanes; each lane accumulates
+//    its pixels in ascending order, a product term as fma(a, b, acc); lane
+//    sums are then combined pairwise within rows of 16 lanes
+//    ((x0+x1)+(x2+x3)) + ((x4+x5)+(x6+x7)) and so on, and for 64 lanes the four
+//    row sums as (r3 + r2) + (r1 + r0).
+struct window_sum {
+  int lanes;
+  float acc[64];
+  float serial;
+  explicit window_sum(int l) : lanes(l), serial(0.f) { std::fill(acc, acc + 64, 0.f); }
+  void fma(int i, float a, float b) {
+    if (lanes) acc[i % lanes] = std::fmaf(a, b, acc[i % lanes]);
+    else serial = serial + a * b;
+  }
+  void add(int i, float x) {
+    if (lanes) acc[i % lanes] = acc[i % lanes] + x;
+    else serial = serial + x;
+  }
+  static float row16(const float* x) {
+    const float p0 = x[0] + x[1], p1 = x[2] + x[3], p2 = x[4] + x[5], p3 = x[6] + x[7];
+    const float p4 = x[8] + x[9], p5 = x[10] + x[11], p6 = x[12] + x[13], p7 = x[14] + x[15];
+    return ((p0 + p1) + (p2 + p3)) + ((p4 + p5) + (p6 + p7));
+  }
+  float total() const {
+    if (!lanes) return serial;
+    if (lanes == 16) return row16(acc);
+    const float r0 = row16(acc), r1 = row16(acc + 16), r2 = row16(acc + 32), r3 = row16(acc + 48);
+    return (r3 + r2) + (r1 + r0);
+  }
+};
+
 // iterative_lk_match_5x5::match (tracking2/pyrlk_tracker.hpp:165-335),
 // parameterised by window, iteration count, selection size and template type.
 // Defined behaviour where the reference is undefined:
@@ -88,6 +131,8 @@ match_result lk_match(const lk_params& P, f2 p, f2 pred_tr, const lk_level& L) {
   const image<float>& A = *L.A;
   const image<float>& B = *L.B;
   const image<f2>& Ag = *L.G;
+  // the GPU's arithmetic (fused bilinear blends) goes with its window-sum order
+  const bool fused = P.group_lanes != 0;
 
   f2 prediction{p.r + pred_tr.r, p.c + pred_tr.c};
   f2 v = prediction;
@@ -101,8 +146,8 @@ match_result lk_match(const lk_params& P, f2 p, f2 pred_tr, const lk_level& L) {
       f2 q{p.r + (float)r, p.c + (float)c};
       if (Ag.has(tz(q.r), tz(q.c))) {
         valid[i] = 1;
-        gs[i] = li2(Ag, q);
-        float a = li(A, q);
+        gs[i] = li2(Ag, q, fused);
+        float a = li(A, q, fused);
         // float -> uint8 through int (the x86 conversion; values outside
         // [0,255] only arise from the negative weights of truncated negative
         // coordinates and then wrap modulo 256).
@@ -122,17 +167,18 @@ match_result lk_match(const lk_params& P, f2 p, f2 pred_tr, const lk_level& L) {
     for (int k = 0; k < (int)idx.size() && k < P.top_n; k++) mask[idx[k]] = 1;
   }
 
-  float G00 = 0.f, G01 = 0.f, G10 = 0.f, G11 = 0.f;
+  window_sum s00(P.group_lanes), s01(P.group_lanes), s11(P.group_lanes);
   int cpt = 0;
   for (int i = 0; i < n; i++)
     if (mask[i]) {
       float gx = gs[i].r, gy = gs[i].c;
-      G00 += gx * gx;
-      G01 += gx * gy;
-      G10 += gx * gy;
-      G11 += gy * gy;
+      s00.fma(i, gx, gx);
+      s01.fma(i, gx, gy);
+      s11.fma(i, gy, gy);
       cpt++;
     }
+  // G is symmetric: G10 = G01 term by term in either order
+  const float G00 = s00.total(), G01 = s01.total(), G10 = G01, G11 = s11.total();
 
   {
     float h = 0.5f * (G00 + G11);
@@ -149,17 +195,18 @@ match_result lk_match(const lk_params& P, f2 p, f2 pred_tr, const lk_level& L) {
 
   const box dom = inner(B.nr, B.nc, 3);
   for (int k = 0; k < P.max_iters && std::sqrt(nk0 * nk0 + nk1 * nk1) >= P.eps; k++) {
-    float b0 = 0.f, b1 = 0.f;
+    window_sum sb0(P.group_lanes), sb1(P.group_lanes);
     cpt = 0;
     for (int i = 0, r = -hws; r <= hws; r++)
       for (int c = -hws; c <= hws; c++, i++)
         if (mask[i]) {
           f2 q{v.r + (float)r, v.c + (float)c};
-          float dt = as[i] - li(B, q);
-          b0 += gs[i].r * dt;
-          b1 += gs[i].c * dt;
+          float dt = as[i] - li(B, q, fused);
+          sb0.fma(i, gs[i].r, dt);
+          sb1.fma(i, gs[i].c, dt);
           cpt++;
         }
+    const float b0 = sb0.total(), b1 = sb1.total();
     nk0 = I00 * b0 + I01 * b1;
     nk1 = I10 * b0 + I11 * b1;
     v.r += nk0;
@@ -168,15 +215,15 @@ match_result lk_match(const lk_params& P, f2 p, f2 pred_tr, const lk_level& L) {
     if (!dom.has(tz(v.r), tz(v.c))) return fail;
   }
 
-  float err = 0.f;
-  for (int r = -hws; r <= hws; r++)
-    for (int c = -hws; c <= hws; c++) {
+  window_sum se(P.group_lanes);
+  for (int w = 0, r = -hws; r <= hws; r++)
+    for (int c = -hws; c <= hws; c++, w++) {
       f2 q{v.r + (float)r, v.c + (float)c};
       int i = (r + hws) * hws + (c + hws);
-      err += std::fabs(as[i] - li(B, q));
+      se.add(w, std::fabs(as[i] - li(B, q, fused)));
       cpt++;
     }
-  return match_result{{v.r - p.r, v.c - p.c}, err / cpt};
+  return match_result{{v.r - p.r, v.c - p.c}, se.total() / cpt};
 }
 
 }  // namespace orc

@@ -33,6 +33,7 @@ typedef struct {
   int32_t with_merge;
   int32_t blur_sigma;
   int32_t blur_K;
+  int32_t sum_lanes;      /* order of the matcher's window sums: 0 raster, 16 / 64 lane groups */
 } orc_tracker_params;
 
 float orc_gauss_coef_runtime(float sigma, int x);
@@ -40,9 +41,10 @@ float orc_gauss_coef_meta(int sigma, int x);
 void orc_gauss_coefs(int sigma, int K, int meta_rounding, float* out);
 
 /* Separable Gaussian, GPU meta-kernel semantics; ch interleaved channels. */
-void orc_gaussian_sep_f32(const float* in, int nr, int nc, int ch, const float* coef, int K,
+/* fused: taps added with fused multiply-adds (the GPU's arithmetic). */
+void orc_gaussian_sep_f32(const float* in, int nr, int nc, int ch, const float* coef, int K, int fused,
                           float* out);
-void orc_gaussian_sep_u8(const uint8_t* in, int nr, int nc, const float* coef, int K, float* out);
+void orc_gaussian_sep_u8(const uint8_t* in, int nr, int nc, const float* coef, int K, int fused, float* out);
 void orc_gaussian_sep_f64(const double* in, int nr, int nc, int ch, const double* coef, int K,
                           double* out);
 
@@ -57,8 +59,8 @@ void orc_sobel_gradient(const float* prev, int nr, int nc, int integer_source, f
 
 /* One LK match on a single level (A, B float images, G = 2-channel gradient). */
 void orc_lk_match(const float* A, const float* B, const float* G, int nr, int nc, int ws,
-                  int max_iters, float eps, int top_n, int template_u8, float p_r, float p_c,
-                  float pred_r, float pred_c, float* out_tr2, float* out_dist);
+                  int max_iters, float eps, int top_n, int template_u8, int sum_lanes, float p_r,
+                  float p_c, float pred_r, float pred_c, float* out_tr2, float* out_dist);
 
 void* orc_tracker_create(int nr, int nc, const orc_tracker_params* p);
 void orc_tracker_destroy(void* t);

@@ -80,7 +80,7 @@ float gauss_coef_meta(int sigma, int x);
 void gauss_coefs(int sigma, int K, bool meta_rounding, float* out);
 
 template <typename TI>
-void gaussian_sep_f32(const image<TI>& in, const float* coef, int K, image<float>& out);
+void gaussian_sep_f32(const image<TI>& in, const float* coef, int K, image<float>& out, bool fused = false);
 void gaussian_sep_f64(const image<double>& in, const double* coef, int K, image<double>& out);
 
 // ---------------------------------------------------------------- pyramid
@@ -100,6 +100,11 @@ struct lk_params {
   int top_n = 49;       // strongest-gradient pixels used; >= ws*ws means all
   int template_u8 = 0;  // 1: template samples truncated to uint8 (u8 images)
   float min_eig = 1e-4f;
+  // Order of the window sums (G, b, error).  0: raster order, one FP32 add
+  // per pixel.  16 or 64: the order of a group of that many lanes, each lane
+  // l summing pixels l, l + lanes, ... with fused multiply-adds, the lane
+  // sums then added pairwise (see lk.cpp).
+  int group_lanes = 0;
 };
 
 // One pyramid level as the matcher sees it: A = previous frame, B = current

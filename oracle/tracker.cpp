@@ -61,8 +61,10 @@ void tracker::prepare_matcher_images(level_state& L) {
     L.B = to_float(L.cur_u8);
     sobel_gradient(L.A, true, L.G);
   } else {
-    gaussian_sep_f32<uint8_t>(L.prev_u8, blur_coef.data(), P.blur_K, L.A);
-    gaussian_sep_f32<uint8_t>(L.cur_u8, blur_coef.data(), P.blur_K, L.B);
+    // the GPU's arithmetic (fused taps) goes with its window-sum order
+    const bool fused = P.lk.group_lanes != 0;
+    gaussian_sep_f32<uint8_t>(L.prev_u8, blur_coef.data(), P.blur_K, L.A, fused);
+    gaussian_sep_f32<uint8_t>(L.cur_u8, blur_coef.data(), P.blur_K, L.B, fused);
     sobel_gradient(L.A, false, L.G);
   }
 }

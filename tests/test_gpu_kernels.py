@@ -1,7 +1,9 @@
 """GPU parity of the image kernels against the CPU oracle (through the C ABI).
 
-Gaussian: within 1e-5 relative of the oracle (FP32, the oracle sums without
-FMA contraction, the GPU with FMA; both accumulate tap +K first).
+Gaussian: within 1e-5 relative of the oracle's reading of the reference (each
+product rounded, taps accumulated from +K down), and bit-exact against the
+oracle's fused mode (the same taps added with fused multiply-adds, the GPU's
+arithmetic).
 Pyramid, border fill and FAST saliency: bit-exact.
 """
 import numpy as np
@@ -73,6 +75,8 @@ def test_gaussian_matches_oracle(ctx, orc, kind, shape, sigma):
     got = dst.download()
     ref = orc.gaussian(img, coef, K)
     assert rel_err(got, ref) < 1e-5
+    # the same operations as the oracle's fused mode: bit-exact
+    assert np.array_equal(got, orc.gaussian(img, coef, K, fused=True))
 
 
 def test_gaussian_multi_levels(ctx, orc):

@@ -1,0 +1,38 @@
+"""The oracle's frame step against the committed end-to-end fixture
+(tests/golden/frame_step_320x240.json, written by
+tests/golden/make_frame_step_fixture.py): a regression pin of every frame's
+dense vector, sparse image, saliency and matches, for both matcher profiles
+and both arithmetic readings, and of the synthetic video itself."""
+import json
+import os
+
+import pytest
+
+import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden", "frame_step_320x240.json")
+
+
+@pytest.fixture(scope="module")
+def fixture():
+    with open(GOLDEN) as f:
+        return json.load(f)
+
+
+def test_video_is_the_fixture_video(fixture):
+    import golden.make_frame_step_fixture as m
+    assert m.digest(m.video()) == fixture["video"]["digest"]
+
+
+@pytest.mark.parametrize("run", ["B/raster", "B/gpu", "R/raster", "R/gpu"])
+def test_oracle_frame_steps_match_fixture(orc, fixture, run):
+    import golden.make_frame_step_fixture as m
+    profile, arith = run.split("/")
+    ws = 7 if profile == "B" else 11
+    lanes = oracle.gpu_sum_lanes(ws) if arith == "gpu" else 0
+    t = orc.tracker(m.NR, m.NC, oracle.profile_params(profile, m.NSCALES, m.TH, sum_lanes=lanes))
+    v = m.video()
+    for f, want in enumerate(fixture["runs"][run]):
+        t.run(v[f])
+        assert m.state(t, m.NSCALES) == want, (run, f)

@@ -147,3 +147,35 @@ def test_rgba_channels_independent(orc):
     out = orc.gaussian(img, c, 3)
     for k in range(4):
         assert np.array_equal(out[:, :, k], orc.gaussian(np.ascontiguousarray(img[:, :, k]), c, 3))
+
+
+def test_fused_mode_matches_independent_restatement(orc):
+    """The oracle's fused mode (the GPU's arithmetic: tap +K first, every
+    further tap added with a fused multiply-add, row pass then column pass,
+    clamped reads) against a Python restatement with exactly rounded FMAs."""
+    from test_oracle_lk import f32_fma
+    rng = np.random.default_rng(21)
+    for shape, K, kind in (((9, 11), 3, "f32"), ((6, 5), 2, "u8"), ((4, 13), 4, "f32")):
+        img = (rng.integers(0, 256, shape, dtype=np.uint8) if kind == "u8"
+               else rng.uniform(0, 255, shape).astype(np.float32))
+        coef = orc.coefs(1, K)
+        nr, nc = shape
+        x = img.astype(np.float32)
+        tmp = np.zeros(shape, np.float32)
+        for r in range(nr):
+            for c in range(nc):
+                s = np.float32(x[r, min(c + K, nc - 1)] * coef[2 * K])
+                for j in range(K - 1, -K - 1, -1):
+                    s = f32_fma(coef[j + K], x[r, min(max(c + j, 0), nc - 1)], s)
+                tmp[r, c] = s
+        want = np.zeros(shape, np.float32)
+        for r in range(nr):
+            for c in range(nc):
+                s = np.float32(tmp[min(r + K, nr - 1), c] * coef[2 * K])
+                for j in range(K - 1, -K - 1, -1):
+                    s = f32_fma(coef[j + K], tmp[min(max(r + j, 0), nr - 1), c], s)
+                want[r, c] = s
+        got = orc.gaussian(img, coef, K, fused=True)
+        assert np.array_equal(got, want), (shape, K, kind)
+        # and it is a different rounding from the unfused reading
+        assert np.abs(orc.gaussian(img, coef, K) - got).max() < 1e-3

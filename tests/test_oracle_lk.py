@@ -98,22 +98,69 @@ def test_leaving_domain_fails(orc):
 
 
 # ----------------------------------------------------------- numpy restatement
-def np_li(img, qr, qc):
+def np_li(img, qr, qc, fused=False):
     nr, nc = img.shape
     xr, xc = int(qr), int(qc)
     a0, a1 = f32(f32(qr) - f32(xr)), f32(f32(qc) - f32(xc))
     g = lambda r, c: img[min(max(r, 0), nr - 1), min(max(c, 0), nc - 1)]
     b0, b1 = f32(f32(1) - a0), f32(f32(1) - a1)
     s = f32(f32(b0 * b1) * g(xr, xc))
-    s = f32(s + f32(f32(a0 * b1) * g(xr + 1, xc)))
-    s = f32(s + f32(f32(b0 * a1) * g(xr, xc + 1)))
-    s = f32(s + f32(f32(a0 * a1) * g(xr + 1, xc + 1)))
+    terms = ((f32(a0 * b1), g(xr + 1, xc)), (f32(b0 * a1), g(xr, xc + 1)), (f32(a0 * a1), g(xr + 1, xc + 1)))
+    for w, v in terms:
+        s = f32_fma(w, v, s) if fused else f32(s + f32(w * v))
     return s
 
 
-def np_match(A, B, G, p, pred, ws, max_iters, eps, top_n, template_u8):
+def f32_fma(a, b, c):
+    """fma in float32: a*b + c exactly (fractions), rounded once to nearest
+    even."""
+    from fractions import Fraction
+    v = Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c))
+    x = np.float32(float(v))
+    best = None
+    for y in (np.nextafter(x, np.float32(-np.inf)), x, np.nextafter(x, np.float32(np.inf))):
+        d = abs(Fraction(float(y)) - v)
+        even = (int(np.float32(y).view(np.uint32)) & 1) == 0
+        if best is None or d < best[0] or (d == best[0] and even):
+            best = (d, y)
+    return np.float32(best[1])
+
+
+class Sum:
+    """A window sum in raster order (lanes 0) or in the lane-group order: lane
+    i % lanes accumulates, lanes are added pairwise in rows of 16, rows as
+    (r3 + r2) + (r1 + r0)."""
+
+    def __init__(self, lanes):
+        self.lanes = lanes
+        self.acc = [f32(0)] * max(lanes, 1)
+
+    def fma(self, i, a, b):
+        if self.lanes:
+            self.acc[i % self.lanes] = f32_fma(a, b, self.acc[i % self.lanes])
+        else:
+            self.acc[0] = f32(self.acc[0] + f32(a * b))
+
+    def add(self, i, x):
+        k = i % self.lanes if self.lanes else 0
+        self.acc[k] = f32(self.acc[k] + x)
+
+    def total(self):
+        if not self.lanes:
+            return self.acc[0]
+
+        def tree(x):
+            while len(x) > 1:
+                x = [f32(x[2 * j] + x[2 * j + 1]) for j in range(len(x) // 2)]
+            return x[0]
+        rows = [tree(self.acc[16 * r:16 * r + 16]) for r in range(self.lanes // 16)]
+        return rows[0] if len(rows) == 1 else f32(f32(rows[3] + rows[2]) + f32(rows[1] + rows[0]))
+
+
+def np_match(A, B, G, p, pred, ws, max_iters, eps, top_n, template_u8, lanes=0):
     nr, nc = A.shape
     hws, n = ws // 2, ws * ws
+    fz = lanes != 0  # the GPU's arithmetic: fused bilinear blends with the lane-group sums
     p = (f32(p[0]), f32(p[1]))
     prediction = (f32(p[0] + f32(pred[0])), f32(p[1] + f32(pred[1])))
     v = list(prediction)
@@ -122,8 +169,8 @@ def np_match(A, B, G, p, pred, ws, max_iters, eps, top_n, template_u8):
         for c in range(-hws, hws + 1):
             qr, qc = f32(p[0] + f32(r)), f32(p[1] + f32(c))
             if 0 <= int(qr) < nr and 0 <= int(qc) < nc:
-                g = (np_li(G[:, :, 0], qr, qc), np_li(G[:, :, 1], qr, qc))
-                a = np_li(A, qr, qc)
+                g = (np_li(G[:, :, 0], qr, qc, fz), np_li(G[:, :, 1], qr, qc, fz))
+                a = np_li(A, qr, qc, fz)
                 if template_u8:
                     a = f32(np.uint8(int(a) & 0xFF))
                 gs.append(g), as_.append(a), valid.append(True)
@@ -132,13 +179,14 @@ def np_match(A, B, G, p, pred, ws, max_iters, eps, top_n, template_u8):
     mag = [np.sqrt(f32(f32(g[0] * g[0]) + f32(g[1] * g[1]))) for g in gs]
     order = sorted([i for i in range(n) if valid[i]], key=lambda i: (-mag[i], i))
     sel = set(order[:top_n])
-    G00 = G01 = G11 = f32(0)
+    s00, s01, s11 = Sum(lanes), Sum(lanes), Sum(lanes)
     for i in range(n):
         if i in sel:
             gx, gy = gs[i]
-            G00 = f32(G00 + f32(gx * gx))
-            G01 = f32(G01 + f32(gx * gy))
-            G11 = f32(G11 + f32(gy * gy))
+            s00.fma(i, gx, gx)
+            s01.fma(i, gx, gy)
+            s11.fma(i, gy, gy)
+    G00, G01, G11 = s00.total(), s01.total(), s11.total()
     h, q = f32(f32(0.5) * f32(G00 + G11)), f32(f32(0.5) * f32(G00 - G11))
     rr = np.sqrt(f32(f32(q * q) + f32(G01 * G01)))
     if min(abs(f32(h + rr)), abs(f32(h - rr))) < f32(1e-4):
@@ -149,17 +197,18 @@ def np_match(A, B, G, p, pred, ws, max_iters, eps, top_n, template_u8):
     nk = (f32(1), f32(1))
     k, cpt = 0, len(sel)
     while k < max_iters and np.sqrt(f32(f32(nk[0] * nk[0]) + f32(nk[1] * nk[1]))) >= f32(eps):
-        b0 = b1 = f32(0)
+        sb0, sb1 = Sum(lanes), Sum(lanes)
         cpt = 0
         i = 0
         for r in range(-hws, hws + 1):
             for c in range(-hws, hws + 1):
                 if i in sel:
-                    dt = f32(as_[i] - np_li(B, f32(v[0] + f32(r)), f32(v[1] + f32(c))))
-                    b0 = f32(b0 + f32(gs[i][0] * dt))
-                    b1 = f32(b1 + f32(gs[i][1] * dt))
+                    dt = f32(as_[i] - np_li(B, f32(v[0] + f32(r)), f32(v[1] + f32(c)), fz))
+                    sb0.fma(i, gs[i][0], dt)
+                    sb1.fma(i, gs[i][1], dt)
                     cpt += 1
                 i += 1
+        b0, b1 = sb0.total(), sb1.total()
         nk = (f32(f32(I00 * b0) + f32(I01 * b1)), f32(f32(I01 * b0) + f32(I11 * b1)))
         v = [f32(v[0] + nk[0]), f32(v[1] + nk[1])]
         d0, d1 = f32(prediction[0] - v[0]), f32(prediction[1] - v[1])
@@ -168,20 +217,26 @@ def np_match(A, B, G, p, pred, ws, max_iters, eps, top_n, template_u8):
         if not (3 <= int(v[0]) <= nr - 4 and 3 <= int(v[1]) <= nc - 4):
             return np.array([-1, -1], np.float32), f32(1000)
         k += 1
-    err = f32(0)
+    se = Sum(lanes)
+    w = 0
     for r in range(-hws, hws + 1):
         for c in range(-hws, hws + 1):
             idx = (r + hws) * hws + (c + hws)
-            err = f32(err + abs(f32(as_[idx] - np_li(B, f32(v[0] + f32(r)), f32(v[1] + f32(c))))))
+            se.add(w, abs(f32(as_[idx] - np_li(B, f32(v[0] + f32(r)), f32(v[1] + f32(c)), fz))))
             cpt += 1
-    return np.array([f32(v[0] - p[0]), f32(v[1] - p[1])], np.float32), f32(err / f32(cpt))
+            w += 1
+    return np.array([f32(v[0] - p[0]), f32(v[1] - p[1])], np.float32), f32(se.total() / f32(cpt))
 
 
+@pytest.mark.parametrize("lanes", [0, "gpu"])
 @pytest.mark.parametrize("profile", ["R", "B"])
-def test_matches_independent_restatement(orc, profile):
+def test_matches_independent_restatement(orc, profile, lanes):
+    """Bit-exact agreement with the numpy restatement, with window sums in
+    raster order and in the GPU's lane-group order."""
     rng = np.random.default_rng(10 if profile == "R" else 11)
     kw = (dict(ws=11, max_iters=30, eps=0.1, top_n=25, template_u8=1) if profile == "R"
           else dict(ws=7, max_iters=5, eps=0.1, top_n=49, template_u8=0))
+    kw["lanes"] = oracle.gpu_sum_lanes(kw["ws"]) if lanes == "gpu" else 0
     for trial in range(12):
         if profile == "R":
             A = rng.integers(0, 256, (36, 40)).astype(np.float32)
@@ -194,7 +249,8 @@ def test_matches_independent_restatement(orc, profile):
             B = np.roll(A, (1, 0), axis=(0, 1)) + rng.normal(0, 1, A.shape).astype(np.float32)
         p = (float(rng.uniform(2, 33)), float(rng.uniform(2, 37)))
         pred = (float(rng.uniform(-1.5, 1.5)), float(rng.uniform(-1.5, 1.5)))
-        tr, dist = orc.lk_match(A, B, G, p, pred, **kw)
+        okw = {k: v for k, v in kw.items() if k != "lanes"}
+        tr, dist = orc.lk_match(A, B, G, p, pred, sum_lanes=kw["lanes"], **okw)
         tr2, dist2 = np_match(A, B, G, p, pred, **kw)
         assert np.array_equal(tr, tr2), (trial, tr, tr2)
         assert np.float32(dist) == dist2, (trial, dist, dist2)
