@@ -142,6 +142,7 @@ def lib():
         L.cuimg_hip_tracker_set_tuning.argtypes = [vp, c_int, c_int]
         L.cuimg_hip_lk_track.argtypes = [vp, P(Img), P(Img), c_int, c_int, vp, ctypes.c_uint32, P(LkParams),
                                          P(Img), vp, vp, vp]
+        L.cuimg_hip_lk_track_noalloc.argtypes = L.cuimg_hip_lk_track.argtypes
         L.cuimg_hip_batch_create.argtypes = [vp, c_int, P(c_int), P(c_int), P(TrackerParams), P(vp)]
         L.cuimg_hip_batch_destroy.argtypes = [vp]
         L.cuimg_hip_batch_step.argtypes = [vp, P(Img)]

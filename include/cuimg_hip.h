@@ -318,6 +318,18 @@ typedef struct {
 int cuimg_hip_lk_track(cuimg_hip_ctx* ctx, const cuimg_img* prev, const cuimg_img* cur, int nlevels,
                        int scale, cuimg_particle_f* parts, uint32_t n, const cuimg_lk_params* params,
                        const cuimg_img* sparse, int32_t* accept, float* tr2, float* dist);
+/* cuimg_hip_lk_track without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results.  Its scratch (the matcher's counters and, when
+ * accept / tr2 / dist are NULL, the decision buffers) is a fixed device array
+ * of 256 B + 16 B x 262144 particles, one per GPU, held until the process
+ * ends: with NULL decision buffers n is at most 262144 (CUIMG_E_CAPACITY
+ * beyond).  Every call on a GPU uses that GPU's array: calls that overlap on
+ * one GPU (other host threads or contexts) race. */
+int cuimg_hip_lk_track_noalloc(cuimg_hip_ctx* ctx, const cuimg_img* prev, const cuimg_img* cur, int nlevels,
+                               int scale, cuimg_particle_f* parts, uint32_t n, const cuimg_lk_params* params,
+                               const cuimg_img* sparse, int32_t* accept, float* tr2, float* dist);
 
 /* ----------------------------------------------------------------- batch */
 /* Many independent trackers (video streams, each its own domain and nscales;
