@@ -113,11 +113,27 @@ template <typename T>
 __device__ __forceinline__ const T* px(const void* begin, int pitch, int r, int c) {
   return reinterpret_cast<const T*>(static_cast<const char*>(begin) + (ptrdiff_t)r * pitch) + c;
 }
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return max(lo, min(v, hi)); }  // lo <= hi
 
 // Output size above which image kernels write with non-temporal stores: the 8
 // XCDs' L2s (4 MiB each) could not hand such an image to the next kernel anyway.
 constexpr size_t kStreamOutBytes = 32u << 20;
+
+// Raw buffer view of an image: a load is one instruction addressed by a 32-bit
+// VGPR byte offset, a wave-uniform SGPR byte offset and an immediate, with no
+// 64-bit address arithmetic.  No range check (num_records = 2^32 - 1): callers
+// keep offsets inside the image.  0x00020000 is the gfx9 family's raw 32-bit
+// word format.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, -1, 0x00020000);
+}
+__device__ __forceinline__ float ld_f32(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, soff, 0));
+}
+__device__ __forceinline__ float2 ld_f32x2(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff) {
+  const auto w = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, soff, 0);
+  return make_float2(__uint_as_float(w[0]), __uint_as_float(w[1]));
+}
 
 // Non-temporal stores (the line is not kept in L2).  Written in asm: as
 // __builtin_nontemporal_store the compiler merges `nt ? nt_store : store` into

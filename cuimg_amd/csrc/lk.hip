@@ -100,7 +100,7 @@ static int lk_track_impl(cuimg_hip_ctx* ctx, const cuimg_img* prev, const cuimg_
   a.pd_c1 = 3;
   a.pd_r2 = S.nrows - 4;
   a.pd_c2 = S.ncols - 4;
-  a.P = lk_params_dev{p->ws, p->max_iters, std::min(p->top_n, p->ws * p->ws), p->eps, 1e-4f, p->k, p->template_u8};
+  a.P = make_lk_params(p->ws, p->max_iters, std::min(p->top_n, p->ws * p->ws), p->eps, p->k, p->template_u8);
   // counters and, when not given, decision buffers
   void* tmp = scratch_dev;
   if (!tmp) {

@@ -244,6 +244,30 @@ __device__ __forceinline__ void load_patch(float* dst, int q, const void* img, i
   }
 }
 
+// load_patch for FP32 images through a raw buffer view: row offsets are one
+// 24-bit multiply-add, no 64-bit address arithmetic.
+template <int PS, int GL, int MAXQ>
+__device__ __forceinline__ void load_patch_buf(float* dst, int q, __amdgpu_buffer_rsrc_t r, int pitch, int nr,
+                                               int nc, int r0, int c0, int lane) {
+  static_assert(GL % PS == 0, "whole rows per pass");
+  constexpr int RPP = GL / PS;
+  constexpr int MP = (MAXQ + RPP - 1) / RPP;
+  const int col = lane % PS, row = lane / PS;
+  const bool on = col < q;
+  const uint32_t cb = (uint32_t)clampi(c0 + col, 0, nc - 1) * 4u;
+  float v[MP];
+#pragma unroll
+  for (int m = 0; m < MP; m++) {
+    const int i = row + m * RPP;
+    if (on && i < q) v[m] = ld_f32(r, __umul24((uint32_t)clampi(r0 + i, 0, nr - 1), (uint32_t)pitch) + cb, 0);
+  }
+#pragma unroll
+  for (int m = 0; m < MP; m++) {
+    const int i = row + m * RPP;
+    if (on && i < q) dst[i * PS + col] = v[m];
+  }
+}
+
 // Orders the wave's LDS writes before its later LDS reads (one wave, in-order
 // LDS queue: only the compiler has to be kept from moving accesses).
 __device__ __forceinline__ void wave_sync_lds() {
@@ -258,6 +282,7 @@ template <int NPL>
 struct lane_win {
   static constexpr int NS = lk_cfg<NPL>::NS;
   int r[NS], c[NS];
+  int idx[NS];  // raster index (r + hws) * ws + (c + hws) of the window pixel
   float fr[NS], fc[NS];
   bool in[NS];
   int g[lk_lds<NPL, false>::GPASS];   // whole gradient patch
@@ -272,11 +297,22 @@ __device__ __forceinline__ lane_win<NPL> make_lane_win(int ws, int lane) {
   const float iws = 1.f / (float)ws, iqg = 1.f / (float)QG;
 #pragma unroll
   for (int k = 0; k < NS; k++) {
-    const int i = lane + GL * k;
-    const int q = divq(i, iws);
-    W.in[k] = i < ws * ws;
-    W.r[k] = q - hws;
-    W.c[k] = i - q * ws - hws;
+    int row, col;
+    if constexpr (GL == 16) {
+      // a lane holds one window column at every other row: lane 8 h + c has
+      // the pixels (h + 2k, c), windows up to 7x7 (8 rows, 8 columns)
+      row = (lane >> 3) + 2 * k;
+      col = lane & 7;
+      W.in[k] = row < ws && col < ws;
+    } else {
+      const int i = lane + GL * k;
+      row = divq(i, iws);
+      col = i - row * ws;
+      W.in[k] = i < ws * ws;
+    }
+    W.r[k] = row - hws;
+    W.c[k] = col - hws;
+    W.idx[k] = row * ws + col;
     W.fr[k] = (float)W.r[k];
     W.fc[k] = (float)W.c[k];
   }
@@ -372,7 +408,7 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
   bool sel[NS];
 #pragma unroll
   for (int k = 0; k < NS; k++) {
-    const int i = lane + GL * k;
+    const int i = W.idx[k];
     gs[k] = make_float2(0.f, 0.f);
     as[k] = 0.f;
     sel[k] = false;
@@ -397,7 +433,7 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
     // rank among valid pixels by (|g| desc, raster index asc); keep the top_n
 #pragma unroll
     for (int k = 0; k < NS; k++) {
-      const int i = lane + GL * k;
+      const int i = W.idx[k];
       if (sel[k]) {
         const float m = S.mag[i];
         int rank = 0;
@@ -457,7 +493,8 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
 
   float nk0 = 1.f, nk1 = 1.f;
   const int rlo = 3, rhi = L.nr - 4, clo = 3, chi = L.nc - 4;
-  for (int it = 0; it < P.max_iters && sqrtf(add_rn(mul_rn(nk0, nk0), mul_rn(nk1, nk1))) >= P.eps; it++) {
+  // sqrtf(|delta|^2) >= eps and |pred - v| > 4 ws as the equivalent squared tests
+  for (int it = 0; it < P.max_iters && add_rn(mul_rn(nk0, nk0), mul_rn(nk1, nk1)) >= P.step_sq; it++) {
     ensure_covered();
     float b0 = 0.f, b1 = 0.f;
 #pragma unroll
@@ -473,7 +510,8 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
     nk1 = add_rn(mul_rn(I10, b0), mul_rn(I11, b1));
     v.x = add_rn(v.x, nk0);
     v.y = add_rn(v.y, nk1);
-    if (norml2(sub_rn(prediction.x, v.x), sub_rn(prediction.y, v.y)) > (float)(ws * 4)) return fail;
+    const float dr = sub_rn(prediction.x, v.x), dc = sub_rn(prediction.y, v.y);
+    if (add_rn(mul_rn(dr, dr), mul_rn(dc, dc)) > P.far_sq) return fail;
     const int vr = tz(v.x), vc = tz(v.y);
     if (!(vr >= rlo && vr <= rhi && vc >= clo && vc <= chi)) return fail;
   }
@@ -494,26 +532,284 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
   return o;
 }
 
+
+
+// Sobel/8 gradient of an FP32 level (pyrlk_cpu::match_particles,
+// pyrlk_tracker.hpp:909-938, taken of the blurred frame): [0] = d/drow,
+// [1] = d/dcol, the terms added left to right as written there, zero on the
+// 1-pixel outer ring.  Rows [r0, r1) of the image; thread t of the block takes
+// columns t, t + blockDim.x, ...
+__device__ inline void sobel_rows_body(const float* in, int in_pitch, float2* out, int out_pitch, int nr, int nc,
+                                       int r0, int r1) {
+  for (int r = r0; r < r1; r++) {
+    const float* up = px<float>(in, in_pitch, max(r - 1, 0), 0);
+    const float* mid = px<float>(in, in_pitch, r, 0);
+    const float* dn = px<float>(in, in_pitch, min(r + 1, nr - 1), 0);
+    float2* o = px<float2>(out, out_pitch, r, 0);
+    const bool ring = r == 0 || r == nr - 1;
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+      float2 g = make_float2(0.f, 0.f);
+      if (!ring && c > 0 && c < nc - 1) {
+        const float pmm = up[c - 1], pm0 = up[c], pmp = up[c + 1];
+        const float p0m = mid[c - 1], p0p = mid[c + 1];
+        const float ppm = dn[c - 1], pp0 = dn[c], ppp = dn[c + 1];
+        float gr = add_rn(ppm, mul_rn(2.f, pp0));
+        gr = add_rn(gr, ppp);
+        gr = sub_rn(gr, pmm);
+        gr = sub_rn(gr, mul_rn(2.f, pm0));
+        gr = sub_rn(gr, pmp);
+        float gc = add_rn(pmp, mul_rn(2.f, p0p));
+        gc = add_rn(gc, ppp);
+        gc = sub_rn(gc, pmm);
+        gc = sub_rn(gc, mul_rn(2.f, p0m));
+        gc = sub_rn(gc, ppm);
+        g = make_float2(mul_rn(gr, 0.125f), mul_rn(gc, 0.125f));
+      }
+      o[c] = g;
+    }
+  }
+}
+
+// ------------------------------------------------------------ fast path
+// Windows up to 7x7 on FP32 frames whose gradient image is given (a frame
+// step computes it once per level and frame): the same operations as
+// wave_match, organised around the common case of an "exact" window.
+//
+// A window of half width h around coordinate x is exact when x - h >= 0 and
+// x + h stays below the next power of two above x: then every x + r, |r| <= h,
+// is computed exactly, truncates to tz(x) + r and has the fraction of x, so
+// all samples of the window share the bilinear weights of x itself (the same
+// bits wave_match computes per sample).  Samples then need no weights and no
+// per-sample address: lane 8 h' + c reads column c of rows h' + 2k of the
+// (ws+1)^2 grid of corners.  Other windows (near a power of two, near the
+// image edge, negative coordinates) take the per-sample path.
+
+__device__ __forceinline__ bool exact_axis(float x, int h) {
+  const float lo = sub_rn(x, (float)h), hi = add_rn(x, (float)h);
+  const float top = __uint_as_float(((__float_as_uint(x) >> 23) + 1u) << 23);
+  return lo >= 0.f && hi < top;
+}
+
+struct lk_lds16 {
+  static constexpr int WSMAX = 7;
+  static constexpr int QB = WSMAX + 1 + 2 * kMB;  // current-frame patch side (14)
+  static constexpr int PW = 16;                   // its row stride
+  static constexpr int TPL = (WSMAX - 1) * (WSMAX / 2) + WSMAX;
+  float B[QB * PW];
+  float tpl[TPL];
+};
+
+
+// iterative_lk_match_5x5::match for one 16-lane group (7x7 windows, FP32
+// frames, every window pixel used, gradient image in L.G).  The grid lanes
+// outside the window (column 7, row 7) compute samples too, from addresses
+// inside the patch or image, and their results are discarded: no lane masks
+// on the exact paths.
+__device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const lk_params_dev& P,
+                                                  const lane_win<1>& W, float2 p, float2 pred_tr, lk_lds16& S) {
+  constexpr int GL = 16, NS = lk_cfg<1>::NS, PW = lk_lds16::PW;
+  constexpr int ws = 7, hws = ws / 2, n = ws * ws, QB = lk_lds16::QB;
+  const match_out fail{make_float2(-1.f, -1.f), 1000.f};
+  const int lane = threadIdx.x & (GL - 1);
+  const int hr = lane >> 3, hc = lane & 7;  // this lane's grid row parity and column
+  const float2 prediction = make_float2(add_rn(p.x, pred_tr.x), add_rn(p.y, pred_tr.y));
+  float2 v = prediction;
+  const __amdgpu_buffer_rsrc_t rA = raw_rsrc(L.A), rG = raw_rsrc(L.G);
+
+  // ---- current-frame patch around the prediction (LDS, clamped reads)
+  int oBr = tz(v.x) - hws - kMB, oBc = tz(v.y) - hws - kMB;
+  const __amdgpu_buffer_rsrc_t rB = raw_rsrc(L.B);
+  wave_sync_lds();
+  load_patch_buf<PW, GL, lk_lds16::QB>(S.B, QB, rB, L.pitchB, L.nr, L.nc, oBr, oBc, lane);
+
+  // ---- template: bilinear samples of A and of its gradient
+  float2 gs[NS];
+  float as[NS];
+  bool sel[NS];
+  const int pr = tz(p.x), pc = tz(p.y);
+  // one row and column of margin beyond the window's corners: the grid's row 7
+  // and column 7 lanes read there
+  const bool exact_t = exact_axis(p.x, hws) && exact_axis(p.y, hws) && pr + hws + 2 <= L.nr - 1 &&
+                       pc + hws + 2 <= L.nc - 1;
+  if (exact_t) {
+    const bw w = weights(p.x, p.y);
+    const uint32_t r0 = (uint32_t)(pr - hws + hr);
+    const uint32_t oa0 = __umul24(r0, (uint32_t)L.pitchA) + (uint32_t)(pc - hws + hc) * 4u;
+    const uint32_t og0 = __umul24(r0, (uint32_t)L.pitchG) + (uint32_t)(pc - hws + hc) * 8u;
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+      sel[k] = W.in[k];
+      // rows r0 + 2k and r0 + 2k + 1 (the latter through the uniform pitch offset)
+      const uint32_t oa = oa0 + (uint32_t)(2 * k) * (uint32_t)L.pitchA;
+      const uint32_t og = og0 + (uint32_t)(2 * k) * (uint32_t)L.pitchG;
+      const float a00 = ld_f32(rA, oa, 0), a01 = ld_f32(rA, oa + 4, 0), a10 = ld_f32(rA, oa, L.pitchA),
+                  a11 = ld_f32(rA, oa + 4, L.pitchA);
+      const float2 g00 = ld_f32x2(rG, og, 0), g01 = ld_f32x2(rG, og + 8, 0), g10 = ld_f32x2(rG, og, L.pitchG),
+                   g11 = ld_f32x2(rG, og + 8, L.pitchG);
+      as[k] = W.in[k] ? blend(w, a00, a10, a01, a11) : 0.f;
+      gs[k] = W.in[k] ? make_float2(blend(w, g00.x, g10.x, g01.x, g11.x), blend(w, g00.y, g10.y, g01.y, g11.y))
+                      : make_float2(0.f, 0.f);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+      gs[k] = make_float2(0.f, 0.f);
+      as[k] = 0.f;
+      sel[k] = false;
+      if (W.in[k]) {
+        const bw w = weights(add_rn(p.x, W.fr[k]), add_rn(p.y, W.fc[k]));
+        if (w.xr >= 0 && w.xr < L.nr && w.xc >= 0 && w.xc < L.nc) {
+          sel[k] = true;
+          const uint32_t r0 = (uint32_t)w.xr, r1 = (uint32_t)min(w.xr + 1, L.nr - 1);
+          const uint32_t c0 = (uint32_t)w.xc, c1 = (uint32_t)min(w.xc + 1, L.nc - 1);
+          const uint32_t ra0 = __umul24(r0, (uint32_t)L.pitchA), ra1 = __umul24(r1, (uint32_t)L.pitchA);
+          const uint32_t rg0 = __umul24(r0, (uint32_t)L.pitchG), rg1 = __umul24(r1, (uint32_t)L.pitchG);
+          const float a00 = ld_f32(rA, ra0 + 4 * c0, 0), a01 = ld_f32(rA, ra0 + 4 * c1, 0),
+                      a10 = ld_f32(rA, ra1 + 4 * c0, 0), a11 = ld_f32(rA, ra1 + 4 * c1, 0);
+          const float2 g00 = ld_f32x2(rG, rg0 + 8 * c0, 0), g01 = ld_f32x2(rG, rg0 + 8 * c1, 0),
+                       g10 = ld_f32x2(rG, rg1 + 8 * c0, 0), g11 = ld_f32x2(rG, rg1 + 8 * c1, 0);
+          as[k] = blend(w, a00, a10, a01, a11);
+          gs[k] = make_float2(blend(w, g00.x, g10.x, g01.x, g11.x), blend(w, g00.y, g10.y, g01.y, g11.y));
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NS; k++)
+    if (W.in[k] && W.idx[k] < lk_lds16::TPL) S.tpl[W.idx[k]] = as[k];
+
+  float g00 = 0.f, g01 = 0.f, g11 = 0.f;
+  int cnt = 0;
+#pragma unroll
+  for (int k = 0; k < NS; k++) {
+    if (sel[k]) {
+      g00 = fmaf(gs[k].x, gs[k].x, g00);
+      g01 = fmaf(gs[k].x, gs[k].y, g01);
+      g11 = fmaf(gs[k].y, gs[k].y, g11);
+    }
+    cnt += __popcll(group_ballot<GL>(sel[k]));
+  }
+  const float G00 = group_sum<GL>(g00), G01 = group_sum<GL>(g01), G11 = group_sum<GL>(g11);
+  const float nsel = (float)cnt;
+  const float G10 = G01;
+  {
+    const float h = mul_rn(0.5f, add_rn(G00, G11));
+    const float q = mul_rn(0.5f, sub_rn(G00, G11));
+    const float rr = sqrtf(add_rn(mul_rn(q, q), mul_rn(G01, G10)));
+    const float min_ev = fminf(fabsf(add_rn(h, rr)), fabsf(sub_rn(h, rr)));
+    if (min_ev < P.min_eig) return fail;
+  }
+  const float det = sub_rn(mul_rn(G00, G11), mul_rn(G10, G01));
+  const float invdet = 1.f / det;
+  const float I00 = mul_rn(G11, invdet), I01 = mul_rn(-G01, invdet);
+  const float I10 = mul_rn(-G10, invdet), I11 = mul_rn(G00, invdet);
+
+  // the window at v has all its corners inside the patch
+  auto ensure_covered = [&]() {
+    const int vr = tz(v.x), vc = tz(v.y);
+    if (vr - hws - 1 - oBr >= 0 && vr + hws + 2 - oBr <= QB - 1 && vc - hws - 1 - oBc >= 0 &&
+        vc + hws + 2 - oBc <= QB - 1)
+      return;
+    oBr = vr - hws - kMB;
+    oBc = vc - hws - kMB;
+    wave_sync_lds();
+    load_patch_buf<PW, GL, lk_lds16::QB>(S.B, QB, rB, L.pitchB, L.nr, L.nc, oBr, oBc, lane);
+    wave_sync_lds();
+  };
+  // current-frame samples at v + (r, c) for the lane's pixels
+  auto sample_window = [&](float (&out)[NS]) {
+    if (exact_axis(v.x, hws) && exact_axis(v.y, hws)) {
+      const bw w = weights(v.x, v.y);
+      const float* b0 = &S.B[(tz(v.x) - hws + hr - oBr) * PW + (tz(v.y) - hws + hc - oBc)];
+#pragma unroll
+      for (int k = 0; k < NS; k++) {
+        const float* b = b0 + 2 * k * PW;
+        out[k] = blend(w, b[0], b[PW], b[1], b[PW + 1]);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < NS; k++) {
+        out[k] = 0.f;
+        if (W.in[k]) {
+          const bw w = weights(add_rn(v.x, W.fr[k]), add_rn(v.y, W.fc[k]));
+          const float* b = &S.B[(w.xr - oBr) * PW + (w.xc - oBc)];
+          out[k] = blend(w, b[0], b[PW], b[1], b[PW + 1]);
+        }
+      }
+    }
+  };
+
+  wave_sync_lds();  // the first patch and the template prefix are in LDS
+  float nk0 = 1.f, nk1 = 1.f;
+  const int rlo = 3, rhi = L.nr - 4, clo = 3, chi = L.nc - 4;
+  // sqrtf(|delta|^2) >= eps and |pred - v| > 4 ws as the equivalent squared tests
+  for (int it = 0; it < P.max_iters && add_rn(mul_rn(nk0, nk0), mul_rn(nk1, nk1)) >= P.step_sq; it++) {
+    ensure_covered();
+    float smp[NS];
+    sample_window(smp);
+    float b0 = 0.f, b1 = 0.f;
+#pragma unroll
+    for (int k = 0; k < NS; k++)
+      if (sel[k]) {
+        const float dt = sub_rn(as[k], smp[k]);
+        b0 = fmaf(gs[k].x, dt, b0);
+        b1 = fmaf(gs[k].y, dt, b1);
+      }
+    b0 = group_sum<GL>(b0);
+    b1 = group_sum<GL>(b1);
+    nk0 = add_rn(mul_rn(I00, b0), mul_rn(I01, b1));
+    nk1 = add_rn(mul_rn(I10, b0), mul_rn(I11, b1));
+    v.x = add_rn(v.x, nk0);
+    v.y = add_rn(v.y, nk1);
+    const float dr = sub_rn(prediction.x, v.x), dc = sub_rn(prediction.y, v.y);
+    if (add_rn(mul_rn(dr, dr), mul_rn(dc, dc)) > P.far_sq) return fail;
+    const int vr = tz(v.x), vc = tz(v.y);
+    if (!(vr >= rlo && vr <= rhi && vc >= clo && vc <= chi)) return fail;
+  }
+
+  // error with the reference's index (r+hws)*hws + (c+hws) and counter
+  ensure_covered();
+  float smp[NS];
+  sample_window(smp);
+  float e = 0.f;
+#pragma unroll
+  for (int k = 0; k < NS; k++)
+    if (W.in[k]) {
+      const int idx = (W.r[k] + hws) * hws + (W.c[k] + hws);
+      e = add_rn(e, fabsf(sub_rn(S.tpl[idx], smp[k])));
+    }
+  const float err = group_sum<GL>(e);
+  match_out o;
+  o.tr = make_float2(sub_rn(v.x, p.x), sub_rn(v.y, p.y));
+  o.dist = err / (nsel + (float)n);
+  return o;
+}
+
 // One particle per group of lanes.  The particle at scale s and its synthetic age-1
 // ancestors at p/2, p/4, ... are solved in one loop from the coarsest scale
 // down (recursive_prediction / get_flow_at): an ancestor whose prediction
 // leaves domain - border(3) contributes the identity, so does a match ending
 // outside its level's domain, and a failed match (-1,-1) is kept; without a
 // coarser scale the prediction is the speed (zero at age 1).
-template <typename TI, int NPL, bool SEL>
+// FASTG: the level views carry gradient images and the window class is 7x7 on
+// FP32 frames with every pixel used (wave_match16).
+template <typename TI, int NPL, bool SEL, bool FASTG = false>
 // Workgroup `bid` matches particles [bid * W * ppw, (bid + 1) * W * ppw), W =
 // kLkWaves; the
 // grid covers the capacity, so the hardware's workgroup dispatch balances the
 // particles' very different costs and workgroups past the count exit at once.
 __device__ inline void lk_body(const lk_args& a, int bid, int nbid, int ppw) {
+  static_assert(!FASTG || (NPL == 1 && !SEL && std::is_same<TI, float>::value), "fast path: 7x7 FP32, all pixels");
+  // FASTG launches are for ws == 7 (the host picks the variant)
   constexpr int GL = lk_cfg<NPL>::GL, GPW = lk_cfg<NPL>::GPW;
-  __shared__ lk_lds<NPL, SEL> lds_all[kLkWaves * GPW];
+  using LDS = std::conditional_t<FASTG, lk_lds16, lk_lds<NPL, SEL>>;
+  __shared__ LDS lds_all[kLkWaves * GPW];
   __shared__ lk_level_view lvs[kMaxLevels];
   __shared__ uint32_t blk_live, blk_acc;
   const int wave = threadIdx.x >> 6;
   const int grp = (threadIdx.x & 63) / GL;
   const int lane = threadIdx.x & (GL - 1);
-  lk_lds<NPL, SEL>& S = lds_all[wave * GPW + grp];
+  LDS& S = lds_all[wave * GPW + grp];
   // Every argument is read once, before this kernel stores anything, so the
   // reads are scalar and the values stay in registers (or LDS for the views)
   // whether `a` is a kernel argument or an entry of a job table.
@@ -573,7 +869,9 @@ __device__ inline void lk_body(const lk_args& a, int bid, int nbid, int ppw) {
           flow = make_float2(0.f, 0.f);
           continue;
         }
-        const match_out m = wave_match<TI, NPL, SEL>(lv, P, W, p, ptr, S);
+        match_out m;
+        if constexpr (FASTG) m = wave_match16(lv, P, W, p, ptr, S);
+        else m = wave_match<TI, NPL, SEL>(lv, P, W, p, ptr, S);
         const int mr = tz(add_rn(p.x, m.tr.x)), mc = tz(add_rn(p.y, m.tr.y));
         if (j > s) {
           flow = (mr >= 0 && mr < lv.nr && mc >= 0 && mc < lv.nc) ? m.tr : make_float2(0.f, 0.f);

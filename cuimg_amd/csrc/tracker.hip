@@ -43,6 +43,7 @@ struct level_res {
   int nr = 0, nc = 0;
   cuimg_img u8[2]{};
   cuimg_img blur[2]{};
+  cuimg_img grad[2]{};  // Sobel/8 of blur[k] (blurred-frame profile)
   cuimg_img sal{}, sparse{};
   uint8_t* boxcode = nullptr;
   uint32_t* subcnt = nullptr;
@@ -64,6 +65,15 @@ struct job_params {
   float fast_th;
 };
 
+// Sobel/8 gradient of a blurred level: rows of `in` into `out`.
+struct grad_job {
+  const float* in;
+  float2* out;
+  int in_pitch, out_pitch;
+  int nr, nc;
+};
+constexpr int kGradRows = 4;  // rows per gradient workgroup
+
 struct level_job {
   int stream, s;
   int nr, nc;
@@ -81,6 +91,7 @@ struct level_job {
   detect_args det[2];   // [cur]
   merge_args mg[2];     // [current vector after the detector]
   gauss_job blur[2];    // [slot]
+  grad_job grad[2];     // [slot]
 };
 
 struct stream_job {
@@ -94,7 +105,7 @@ struct frame_in {
   int vec_ok;
 };
 
-enum { K_PYR, K_BLUR, K_FILL, K_LK, K_PMASK, K_FAST, K_DETC, K_SCATI, K_MSCAN, K_MR1, K_NKINDS };
+enum { K_PYR, K_BLUR, K_GRAD, K_FILL, K_LK, K_PMASK, K_FAST, K_DETC, K_SCATI, K_MSCAN, K_MR1, K_NKINDS };
 constexpr int kRing = 8;
 
 }  // namespace cuimg
@@ -230,6 +241,15 @@ __global__ __launch_bounds__(64) void step_blur_kernel(const level_job* jobs, co
   gauss_tile<uint8_t, float, 4, K>(G, lb % G.tiles_x, lb / G.tiles_x, cp);
 }
 
+// Blurred-frame profile: the Sobel/8 gradient of every blurred level of the
+// new frame (the matcher reads it as the previous frame's gradient next step).
+__global__ __launch_bounds__(256) void step_grad_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
+  CUIMG_JOB(prefix, njobs);
+  const grad_job g = uni(J.grad[J.cnt->frame & 1u]);
+  const int r0 = lb * kGradRows;
+  sobel_rows_body(g.in, g.in_pitch, g.out, g.out_pitch, g.nr, g.nc, r0, min(r0 + kGradRows, g.nr));
+}
+
 // before_matching (sparse <- -1, particle_container.hpp:187-193) and the
 // detector mask reset (mask <- 255).
 __global__ __launch_bounds__(256) void step_fill_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
@@ -240,13 +260,15 @@ __global__ __launch_bounds__(256) void step_fill_kernel(const level_job* jobs, c
   fill_u32_body(J.sparse, J.sparse_pitch, -b, J.nr + b, -b, J.nc + b, kEmpty, lb, nb);
 }
 
-template <typename TI, int NPL, bool SEL>
+// FAST: 7x7 windows on FP32 frames with every pixel used, which come with
+// their gradient images (wave_match16).
+template <typename TI, int NPL, bool SEL, bool FAST>
 __global__ __launch_bounds__(kLkThreads) __attribute__((amdgpu_waves_per_eu(4))) void step_lk_kernel(const level_job* jobs, const uint32_t* prefix, int njobs,
                                                       int ppw) {
   CUIMG_JOB(prefix, njobs);
   const step_flags f = flags_of(J);
   if (!f.match) return;
-  lk_body<TI, NPL, SEL>(J.lk[f.slot * 2 + f.cur], lb, nb, ppw);
+  lk_body<TI, NPL, SEL, FAST>(J.lk[f.slot * 2 + f.cur], lb, nb, ppw);
 }
 
 __global__ __launch_bounds__(256) void step_pmask_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
@@ -439,6 +461,7 @@ static void destroy_batch(cuimg_hip_batch* b) {
     for (int k = 0; k < 2; k++) {
       image_free(b->ctx, &R.u8[k]);
       image_free(b->ctx, &R.blur[k]);
+      image_free(b->ctx, &R.grad[k]);
     }
     image_free(b->ctx, &R.sal);
     image_free(b->ctx, &R.sparse);
@@ -501,7 +524,10 @@ static int alloc_level(cuimg_hip_batch* b, const cuimg_tracker_params* P, int r,
                             P->with_merge};
   for (int k = 0; k < 2; k++) {
     CUIMG_TRY(image_alloc(ctx, r, c, 3, CUIMG_U8, &R.u8[k]));
-    if (P->profile == 1) CUIMG_TRY(image_alloc(ctx, r, c, 0, CUIMG_F32, &R.blur[k]));
+    if (P->profile == 1) {
+      CUIMG_TRY(image_alloc(ctx, r, c, 0, CUIMG_F32, &R.blur[k]));
+      CUIMG_TRY(image_alloc(ctx, r, c, 0, CUIMG_F32X2, &R.grad[k]));
+    }
   }
   CUIMG_TRY(image_alloc(ctx, r, c, 1, CUIMG_U8, &R.sal));
   CUIMG_TRY(image_alloc(ctx, r, c, 7, CUIMG_U32, &R.sparse));
@@ -532,7 +558,9 @@ static lk_level_view matcher_view(const cuimg_hip_batch* b, int j, int prev, int
   const level_res& R = b->lv[j];
   const cuimg_img& A = b->P0.profile == 0 ? R.u8[prev] : R.blur[prev];
   const cuimg_img& B = b->P0.profile == 0 ? R.u8[cur] : R.blur[cur];
-  return lk_level_view{A.begin, B.begin, A.pitch, B.pitch, R.nr, R.nc};
+  const cuimg_img& G = R.grad[prev];  // empty for the u8 profile
+  return lk_level_view{A.begin, B.begin, A.pitch, B.pitch, R.nr, R.nc, static_cast<const float2*>(G.begin), G.pitch,
+                       0};
 }
 
 static int blur_th(const cuimg_hip_batch* b) {
@@ -585,8 +613,8 @@ static void build_job(cuimg_hip_batch* b, int i, int s) {
       a.pd_c1 = 3;
       a.pd_r2 = R.nr - 4;
       a.pd_c2 = R.nc - 4;
-      a.P = lk_params_dev{P.lk_ws, P.lk_max_iters, std::min(P.lk_top_n, P.lk_ws * P.lk_ws), P.lk_eps, 1e-4f, R.sp.k,
-                          P.profile == 0 ? 1 : 0};
+      a.P = make_lk_params(P.lk_ws, P.lk_max_iters, std::min(P.lk_top_n, P.lk_ws * P.lk_ws), P.lk_eps, R.sp.k,
+                           P.profile == 0 ? 1 : 0);
     }
   for (int cur = 0; cur < 2; cur++) {
     detect_args& d = J.det[cur];
@@ -618,6 +646,13 @@ static void build_job(cuimg_hip_batch* b, int i, int s) {
     m.sparse = J.sparse;
     m.sparse_pitch = J.sparse_pitch;
     m.cnt = J.cnt;
+  }
+  for (int slot = 0; slot < 2; slot++) {
+    grad_job& gj = J.grad[slot];
+    gj = grad_job{};
+    if (P.profile != 1) continue;
+    gj = grad_job{static_cast<const float*>(R.blur[slot].begin), static_cast<float2*>(R.grad[slot].begin),
+                  R.blur[slot].pitch, R.grad[slot].pitch, R.nr, R.nc};
   }
   const int th = blur_th(b);
   for (int slot = 0; slot < 2; slot++) {
@@ -670,6 +705,7 @@ static int build_maps(cuimg_hip_batch* b) {
     const level_job& J = b->hj[j];
     const long cap = R.cap;
     cnt[K_BLUR].push_back(b->P0.profile == 1 ? (uint32_t)(ceil_div(R.nc, 256) * ceil_div(R.nr, th)) : 0u);
+    cnt[K_GRAD].push_back(b->P0.profile == 1 ? (uint32_t)ceil_div(R.nr, kGradRows) : 0u);
     cnt[K_FILL].push_back((uint32_t)J.fill_sparse_blocks);
     cnt[K_PMASK].push_back((uint32_t)std::max<long>(1, std::min<long>(512, (cap + 255) / 256)));
     cnt[K_FAST].push_back((uint32_t)(J.fast_tiles_x * ceil_div(R.nr, kFTH)));
@@ -831,15 +867,22 @@ static int enqueue_step(cuimg_hip_batch* b, bool capturing) {
   CUIMG_TRY(launch(ctx, "fast_saliency", [&] { step_fast_kernel<<<b->total[K_FAST], 256, 0, fst>>>(J, pf[K_FAST], nj); }));
   if (fork) CUIMG_HIP(ctx, hipEventRecord(b->join_ev, b->aux));
   if (b->total[K_BLUR]) CUIMG_TRY(launch(ctx, "gaussian_sep", [&] { launch_blur(b); }));
+  if (b->total[K_GRAD])
+    CUIMG_TRY(launch(ctx, "gradient", [&] { step_grad_kernel<<<b->total[K_GRAD], 256, 0, st>>>(J, pf[K_GRAD], nj); }));
   CUIMG_TRY(launch(ctx, "fills", [&] { step_fill_kernel<<<b->total[K_FILL], 256, 0, st>>>(J, pf[K_FILL], nj); }));
   // join before the matcher: FAST takes about as long as blur + fill, and a
   // FAST still running would hold CU slots the matcher's workgroups wait for
   if (fork) CUIMG_HIP(ctx, hipStreamWaitEvent(st, b->join_ev, 0));
+  const bool fast = b->P0.profile == 1 && b->P0.lk_ws == 7 && b->P0.lk_top_n >= 49;
   CUIMG_TRY(launch(ctx, "lk_track", [&] {
+    if (fast) {
+      step_lk_kernel<float, 1, false, true><<<b->total[K_LK], kLkThreads, 0, st>>>(J, pf[K_LK], nj, b->lk_ppw);
+      return;
+    }
     lk_variant(b->P0.lk_ws, b->P0.lk_top_n, b->P0.profile == 0, [&](auto ti, auto npl, auto sel) {
       using TI = typename decltype(ti)::type;
-      step_lk_kernel<TI, decltype(npl)::value, decltype(sel)::value><<<b->total[K_LK], kLkThreads, 0, st>>>(J, pf[K_LK], nj,
-                                                                                                b->lk_ppw);
+      step_lk_kernel<TI, decltype(npl)::value, decltype(sel)::value, false>
+          <<<b->total[K_LK], kLkThreads, 0, st>>>(J, pf[K_LK], nj, b->lk_ppw);
     });
   }));
   CUIMG_TRY(launch(ctx, "particle_mask",
@@ -1277,7 +1320,10 @@ int cuimg_hip_tracker_matcher_images(cuimg_hip_tracker* t, int scale, cuimg_img*
   const bool u8 = t->b->P0.profile == 0;
   if (A) *A = u8 ? R.u8[prev] : R.blur[prev];
   if (B) *B = u8 ? R.u8[cur] : R.blur[cur];
-  if (G) std::memset(G, 0, sizeof(*G));  // computed on the fly, not stored
+  if (G) {
+    if (u8) std::memset(G, 0, sizeof(*G));  // the u8 profile computes it on the fly
+    else *G = R.grad[prev];
+  }
   return 0;
 }
 

@@ -4,6 +4,8 @@
 
 #include "common.hpp"
 
+#include <cmath>
+
 namespace cuimg {
 
 constexpr int kMaxLevels = 8;
@@ -28,12 +30,16 @@ struct level_counters {
   unsigned long long accepted_total;  // of which accepted (moved)
 };
 
-// One pyramid level as the matcher sees it: A = previous frame, B = current.
+// One pyramid level as the matcher sees it: A = previous frame, B = current,
+// G = Sobel/8 gradient of A (FP32 frames in a frame step; NULL: the matcher
+// computes it from A itself).
 struct lk_level_view {
   const void* A;
   const void* B;
   int pitchA, pitchB;
   int nr, nc;
+  const float2* G;
+  int pitchG, pad;
 };
 
 struct lk_params_dev {
@@ -41,7 +47,32 @@ struct lk_params_dev {
   float eps, min_eig;
   int k;
   int template_u8;
+  float step_sq;  // sqrt_ge_threshold(eps): |delta| >= eps  <=>  |delta|^2 >= step_sq
+  float far_sq;   // sqrt_le_bound(4 ws): |pred - v| > 4 ws  <=>  |pred - v|^2 > far_sq
 };
+
+// Squared thresholds equivalent to the matcher's norm tests (host code).  sqrtf
+// is correctly rounded and monotone, so for s >= 0 (or NaN):
+//   sqrtf(s) >= e  <=>  s >= sqrt_ge_threshold(e), the least float whose root is >= e;
+//   sqrtf(s) >  L  <=>  s >  sqrt_le_bound(L), the greatest float whose root is <= L.
+inline float sqrt_ge_threshold(float e) {
+  if (!(e > 0.f)) return e == e ? 0.f : e;  // e <= 0: every s >= 0 passes; NaN: none
+  float y = e * e;
+  while (y > 0.f && std::sqrt(y) >= e) y = std::nextafter(y, -HUGE_VALF);
+  while (std::sqrt(y) < e) y = std::nextafter(y, HUGE_VALF);
+  return y;
+}
+inline float sqrt_le_bound(float L) {
+  if (!(L >= 0.f)) return L == L ? -1.f : L;  // L < 0: every s >= 0 passes; NaN: none
+  float y = L * L;
+  while (y > 0.f && std::sqrt(y) > L) y = std::nextafter(y, -HUGE_VALF);
+  while (std::sqrt(std::nextafter(y, HUGE_VALF)) <= L) y = std::nextafter(y, HUGE_VALF);
+  return y;
+}
+inline lk_params_dev make_lk_params(int ws, int max_iters, int top_n, float eps, int k, int template_u8) {
+  return lk_params_dev{ws, max_iters, top_n, eps, 1e-4f, k, template_u8, sqrt_ge_threshold(eps),
+                       sqrt_le_bound((float)(ws * 4))};
+}
 
 struct lk_args {
   lk_level_view lv[kMaxLevels];

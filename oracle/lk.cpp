@@ -81,22 +81,25 @@ void sobel_gradient(const image<float>& P, bool integer_source, image<f2>& g) {
 // compiler chose, so its rounding is not reproducible; the oracle offers two
 // definitions:
 //  * lanes == 0: raster order, each product rounded, then added;
-//  * lanes == 16 or 64: pixel i goes to lane i % lanes; each lane accumulates
-//    its pixels in ascending order, a product term as fma(a, b, acc); lane
-//    sums are then combined pairwise within rows of 16 lanes
-//    ((x0+x1)+(x2+x3)) + ((x4+x5)+(x6+x7)) and so on, and for 64 lanes the four
-//    row sums as (r3 + r2) + (r1 + r0).
+//  * lanes == 16 or 64: window pixel i (row r = i / ws, column c = i % ws)
+//    goes to lane 8 (r % 2) + c for 16 lanes (windows up to 7x7: a lane holds
+//    one column at every other row) and to lane i % 64 for 64 lanes; each lane
+//    accumulates its pixels in ascending order, a product term as
+//    fma(a, b, acc); lane sums are then combined pairwise within rows of 16
+//    lanes ((x0+x1)+(x2+x3)) + ((x4+x5)+(x6+x7)) and so on, and for 64 lanes
+//    the four row sums as (r3 + r2) + (r1 + r0).
 struct window_sum {
-  int lanes;
+  int lanes, ws;
   float acc[64];
   float serial;
-  explicit window_sum(int l) : lanes(l), serial(0.f) { std::fill(acc, acc + 64, 0.f); }
+  window_sum(int l, int w) : lanes(l), ws(w), serial(0.f) { std::fill(acc, acc + 64, 0.f); }
+  int lane(int i) const { return lanes == 16 ? ((i / ws) & 1) * 8 + i % ws : i % lanes; }
   void fma(int i, float a, float b) {
-    if (lanes) acc[i % lanes] = std::fmaf(a, b, acc[i % lanes]);
+    if (lanes) acc[lane(i)] = std::fmaf(a, b, acc[lane(i)]);
     else serial = serial + a * b;
   }
   void add(int i, float x) {
-    if (lanes) acc[i % lanes] = acc[i % lanes] + x;
+    if (lanes) acc[lane(i)] = acc[lane(i)] + x;
     else serial = serial + x;
   }
   static float row16(const float* x) {
@@ -167,7 +170,7 @@ match_result lk_match(const lk_params& P, f2 p, f2 pred_tr, const lk_level& L) {
     for (int k = 0; k < (int)idx.size() && k < P.top_n; k++) mask[idx[k]] = 1;
   }
 
-  window_sum s00(P.group_lanes), s01(P.group_lanes), s11(P.group_lanes);
+  window_sum s00(P.group_lanes, ws), s01(P.group_lanes, ws), s11(P.group_lanes, ws);
   int cpt = 0;
   for (int i = 0; i < n; i++)
     if (mask[i]) {
@@ -195,7 +198,7 @@ match_result lk_match(const lk_params& P, f2 p, f2 pred_tr, const lk_level& L) {
 
   const box dom = inner(B.nr, B.nc, 3);
   for (int k = 0; k < P.max_iters && std::sqrt(nk0 * nk0 + nk1 * nk1) >= P.eps; k++) {
-    window_sum sb0(P.group_lanes), sb1(P.group_lanes);
+    window_sum sb0(P.group_lanes, ws), sb1(P.group_lanes, ws);
     cpt = 0;
     for (int i = 0, r = -hws; r <= hws; r++)
       for (int c = -hws; c <= hws; c++, i++)
@@ -215,7 +218,7 @@ match_result lk_match(const lk_params& P, f2 p, f2 pred_tr, const lk_level& L) {
     if (!dom.has(tz(v.r), tz(v.c))) return fail;
   }
 
-  window_sum se(P.group_lanes);
+  window_sum se(P.group_lanes, ws);
   for (int w = 0, r = -hws; r <= hws; r++)
     for (int c = -hws; c <= hws; c++, w++) {
       f2 q{v.r + (float)r, v.c + (float)c};

@@ -127,22 +127,30 @@ def f32_fma(a, b, c):
 
 
 class Sum:
-    """A window sum in raster order (lanes 0) or in the lane-group order: lane
-    i % lanes accumulates, lanes are added pairwise in rows of 16, rows as
-    (r3 + r2) + (r1 + r0)."""
+    """A window sum in raster order (lanes 0) or in the lane-group order: for
+    16 lanes pixel (r, c) of the window goes to lane 8 (r % 2) + c, for 64
+    lanes pixel i to lane i % 64; lanes are added pairwise in rows of 16, rows
+    as (r3 + r2) + (r1 + r0)."""
 
-    def __init__(self, lanes):
-        self.lanes = lanes
+    def __init__(self, lanes, ws):
+        self.lanes, self.ws = lanes, ws
         self.acc = [f32(0)] * max(lanes, 1)
+
+    def lane(self, i):
+        if not self.lanes:
+            return 0
+        if self.lanes == 16:
+            return ((i // self.ws) % 2) * 8 + i % self.ws
+        return i % self.lanes
 
     def fma(self, i, a, b):
         if self.lanes:
-            self.acc[i % self.lanes] = f32_fma(a, b, self.acc[i % self.lanes])
+            self.acc[self.lane(i)] = f32_fma(a, b, self.acc[self.lane(i)])
         else:
             self.acc[0] = f32(self.acc[0] + f32(a * b))
 
     def add(self, i, x):
-        k = i % self.lanes if self.lanes else 0
+        k = self.lane(i)
         self.acc[k] = f32(self.acc[k] + x)
 
     def total(self):
@@ -179,7 +187,7 @@ def np_match(A, B, G, p, pred, ws, max_iters, eps, top_n, template_u8, lanes=0):
     mag = [np.sqrt(f32(f32(g[0] * g[0]) + f32(g[1] * g[1]))) for g in gs]
     order = sorted([i for i in range(n) if valid[i]], key=lambda i: (-mag[i], i))
     sel = set(order[:top_n])
-    s00, s01, s11 = Sum(lanes), Sum(lanes), Sum(lanes)
+    s00, s01, s11 = Sum(lanes, ws), Sum(lanes, ws), Sum(lanes, ws)
     for i in range(n):
         if i in sel:
             gx, gy = gs[i]
@@ -197,7 +205,7 @@ def np_match(A, B, G, p, pred, ws, max_iters, eps, top_n, template_u8, lanes=0):
     nk = (f32(1), f32(1))
     k, cpt = 0, len(sel)
     while k < max_iters and np.sqrt(f32(f32(nk[0] * nk[0]) + f32(nk[1] * nk[1]))) >= f32(eps):
-        sb0, sb1 = Sum(lanes), Sum(lanes)
+        sb0, sb1 = Sum(lanes, ws), Sum(lanes, ws)
         cpt = 0
         i = 0
         for r in range(-hws, hws + 1):
@@ -217,7 +225,7 @@ def np_match(A, B, G, p, pred, ws, max_iters, eps, top_n, template_u8, lanes=0):
         if not (3 <= int(v[0]) <= nr - 4 and 3 <= int(v[1]) <= nc - 4):
             return np.array([-1, -1], np.float32), f32(1000)
         k += 1
-    se = Sum(lanes)
+    se = Sum(lanes, ws)
     w = 0
     for r in range(-hws, hws + 1):
         for c in range(-hws, hws + 1):
