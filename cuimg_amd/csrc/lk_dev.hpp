@@ -61,9 +61,11 @@ __device__ __forceinline__ float mul_rn(float a, float b) { return a * b; }
 __device__ __forceinline__ float add_rn(float a, float b) { return a + b; }
 __device__ __forceinline__ float sub_rn(float a, float b) { return a - b; }
 
+// In-row permutations only (every source lane exists), so bound_ctrl changes no
+// value; with it set the compiler folds the move into the consuming add.
 template <int CTRL>
 __device__ __forceinline__ float mov_dpp(float x) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, true));
 }
 
 template <int CTRL, int ROW_MASK>
@@ -254,12 +256,22 @@ __device__ __forceinline__ void load_patch_buf(float* dst, int q, __amdgpu_buffe
   constexpr int MP = (MAXQ + RPP - 1) / RPP;
   const int col = lane % PS, row = lane / PS;
   const bool on = col < q;
-  const uint32_t cb = (uint32_t)clampi(c0 + col, 0, nc - 1) * 4u;
   float v[MP];
+  if (r0 >= 0 && r0 + q <= nr && c0 >= 0 && c0 + q <= nc) {
+    // inside the image: one offset, rows through the uniform SGPR offset
+    const uint32_t o = __umul24((uint32_t)(r0 + row), (uint32_t)pitch) + (uint32_t)(c0 + col) * 4u;
 #pragma unroll
-  for (int m = 0; m < MP; m++) {
-    const int i = row + m * RPP;
-    if (on && i < q) v[m] = ld_f32(r, __umul24((uint32_t)clampi(r0 + i, 0, nr - 1), (uint32_t)pitch) + cb, 0);
+    for (int m = 0; m < MP; m++) {
+      const int i = row + m * RPP;
+      if (on && i < q) v[m] = ld_f32(r, o, m * RPP * pitch);
+    }
+  } else {
+    const uint32_t cb = (uint32_t)clampi(c0 + col, 0, nc - 1) * 4u;
+#pragma unroll
+    for (int m = 0; m < MP; m++) {
+      const int i = row + m * RPP;
+      if (on && i < q) v[m] = ld_f32(r, __umul24((uint32_t)clampi(r0 + i, 0, nr - 1), (uint32_t)pitch) + cb, 0);
+    }
   }
 #pragma unroll
   for (int m = 0; m < MP; m++) {
@@ -590,6 +602,20 @@ __device__ __forceinline__ bool exact_axis(float x, int h) {
   return lo >= 0.f && hi < top;
 }
 
+// A sufficient interval for exact_axis around a start coordinate x0 > 0 in
+// binade [2^E, 2^(E+1)): every x in [max(2^E, h), 2^(E+1) - h) is exact (then
+// x + h < 2^(E+1) exactly).  Empty for x0 <= 0 or NaN.  Tested per step
+// instead of exact_axis; outside it the per-sample path gives the same bits.
+struct exact_iv {
+  float lo, hi;
+  __device__ __forceinline__ bool has(float x) const { return x >= lo && x < hi; }
+};
+__device__ __forceinline__ exact_iv exact_interval(float x0, int h) {
+  if (!(x0 > 0.f)) return exact_iv{1.f, 0.f};
+  const uint32_t e = __float_as_uint(x0) >> 23;
+  return exact_iv{fmaxf(__uint_as_float(e << 23), (float)h), sub_rn(__uint_as_float((e + 1u) << 23), (float)h)};
+}
+
 struct lk_lds16 {
   static constexpr int WSMAX = 7;
   static constexpr int QB = WSMAX + 1 + 2 * kMB;  // current-frame patch side (14)
@@ -678,16 +704,20 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
   for (int k = 0; k < NS; k++)
     if (W.in[k] && W.idx[k] < lk_lds16::TPL) S.tpl[W.idx[k]] = as[k];
 
+  // window sums with no lane masks: an unselected pixel has a zero gradient,
+  // and adding fma(0, finite, acc) leaves acc (never -0 here) unchanged
   float g00 = 0.f, g01 = 0.f, g11 = 0.f;
-  int cnt = 0;
+  int cnt = n;
 #pragma unroll
   for (int k = 0; k < NS; k++) {
-    if (sel[k]) {
-      g00 = fmaf(gs[k].x, gs[k].x, g00);
-      g01 = fmaf(gs[k].x, gs[k].y, g01);
-      g11 = fmaf(gs[k].y, gs[k].y, g11);
-    }
-    cnt += __popcll(group_ballot<GL>(sel[k]));
+    g00 = fmaf(gs[k].x, gs[k].x, g00);
+    g01 = fmaf(gs[k].x, gs[k].y, g01);
+    g11 = fmaf(gs[k].y, gs[k].y, g11);
+  }
+  if (!exact_t) {
+    cnt = 0;
+#pragma unroll
+    for (int k = 0; k < NS; k++) cnt += __popcll(group_ballot<GL>(sel[k]));
   }
   const float G00 = group_sum<GL>(g00), G01 = group_sum<GL>(g01), G11 = group_sum<GL>(g11);
   const float nsel = (float)cnt;
@@ -704,21 +734,25 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
   const float I00 = mul_rn(G11, invdet), I01 = mul_rn(-G01, invdet);
   const float I10 = mul_rn(-G10, invdet), I11 = mul_rn(G00, invdet);
 
-  // the window at v has all its corners inside the patch
+  // the window at v has all its corners inside the patch: tz(v) within
+  // [cov_r, cov_r + span] (and the same for columns)
+  constexpr int span = QB - 2 * hws - 4;
+  int cov_r = oBr + hws + 1, cov_c = oBc + hws + 1;
   auto ensure_covered = [&]() {
     const int vr = tz(v.x), vc = tz(v.y);
-    if (vr - hws - 1 - oBr >= 0 && vr + hws + 2 - oBr <= QB - 1 && vc - hws - 1 - oBc >= 0 &&
-        vc + hws + 2 - oBc <= QB - 1)
-      return;
+    if ((uint32_t)(vr - cov_r) <= (uint32_t)span && (uint32_t)(vc - cov_c) <= (uint32_t)span) return;
     oBr = vr - hws - kMB;
     oBc = vc - hws - kMB;
+    cov_r = oBr + hws + 1;
+    cov_c = oBc + hws + 1;
     wave_sync_lds();
     load_patch_buf<PW, GL, lk_lds16::QB>(S.B, QB, rB, L.pitchB, L.nr, L.nc, oBr, oBc, lane);
     wave_sync_lds();
   };
+  const exact_iv ivr = exact_interval(v.x, hws), ivc = exact_interval(v.y, hws);
   // current-frame samples at v + (r, c) for the lane's pixels
   auto sample_window = [&](float (&out)[NS]) {
-    if (exact_axis(v.x, hws) && exact_axis(v.y, hws)) {
+    if (ivr.has(v.x) && ivc.has(v.y)) {
       const bw w = weights(v.x, v.y);
       const float* b0 = &S.B[(tz(v.x) - hws + hr - oBr) * PW + (tz(v.y) - hws + hc - oBc)];
 #pragma unroll
@@ -747,14 +781,14 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
     ensure_covered();
     float smp[NS];
     sample_window(smp);
+    // no lane masks (zero gradients where unselected, finite samples)
     float b0 = 0.f, b1 = 0.f;
 #pragma unroll
-    for (int k = 0; k < NS; k++)
-      if (sel[k]) {
-        const float dt = sub_rn(as[k], smp[k]);
-        b0 = fmaf(gs[k].x, dt, b0);
-        b1 = fmaf(gs[k].y, dt, b1);
-      }
+    for (int k = 0; k < NS; k++) {
+      const float dt = sub_rn(as[k], smp[k]);
+      b0 = fmaf(gs[k].x, dt, b0);
+      b1 = fmaf(gs[k].y, dt, b1);
+    }
     b0 = group_sum<GL>(b0);
     b1 = group_sum<GL>(b1);
     nk0 = add_rn(mul_rn(I00, b0), mul_rn(I01, b1));
@@ -764,7 +798,7 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
     const float dr = sub_rn(prediction.x, v.x), dc = sub_rn(prediction.y, v.y);
     if (add_rn(mul_rn(dr, dr), mul_rn(dc, dc)) > P.far_sq) return fail;
     const int vr = tz(v.x), vc = tz(v.y);
-    if (!(vr >= rlo && vr <= rhi && vc >= clo && vc <= chi)) return fail;
+    if ((uint32_t)(vr - rlo) > (uint32_t)(rhi - rlo) || (uint32_t)(vc - clo) > (uint32_t)(chi - clo)) return fail;
   }
 
   // error with the reference's index (r+hws)*hws + (c+hws) and counter

@@ -715,9 +715,14 @@ static int build_maps(cuimg_hip_batch* b) {
     cnt[K_MR1].push_back(cnt[K_MSCAN].back());
   }
   // Matcher: one workgroup per kLkWaves * ppw particles of capacity, at most
-  // kLkMaxWgs per job (the workgroups loop over the chunks, see lk_body).
+  // kLkMaxWgs / 4^s for scale s (the workgroups loop over the chunks, see
+  // lk_body).  A coarser scale holds about a quarter of the particles of the
+  // next finer one; more workgroups than that only add nearly idle ones,
+  // each of which still sets up before it finds its chunks (1080p: 221 ->
+  // 203 us per launch against 1024 workgroups for every scale).
   for (int j = 0; j < b->njobs; j++)
-    cnt[K_LK].push_back((uint32_t)std::min(ceil_div((int)b->lv[j].cap, kLkWaves * b->lk_ppw), kLkMaxWgs));
+    cnt[K_LK].push_back((uint32_t)std::min(ceil_div((int)b->lv[j].cap, kLkWaves * b->lk_ppw),
+                                           std::max(16, kLkMaxWgs >> (2 * b->hj[j].s))));
   for (int k = 0; k < K_NKINDS; k++) {
     // prefix of block counts, then the block -> (job, block in job) map
     std::vector<uint32_t>& p = b->prefix_h[k];
