@@ -49,7 +49,7 @@ def lk_algorithmic_bytes(level_shapes, nparts):
     return img + 40 * sum(nparts)
 
 
-KERNELS = ("pyramid_build", "gaussian_sep", "fills", "lk_track", "particle_mask", "fast_saliency",
+KERNELS = ("pyramid_build", "gaussian_sep", "gradient", "fills", "lk_track", "particle_mask", "fast_saliency",
            "detect_count", "scatter_insert", "merge_scan", "merge_round1", "merge_resolve")
 
 

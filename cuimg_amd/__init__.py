@@ -52,6 +52,15 @@ class TrackerParams(ctypes.Structure):
                 ("blur_K", ctypes.c_int32)]
 
 
+class ScaleParams(ctypes.Structure):
+    """cuimg_scale_params: the settings the reference keeps per scale."""
+    _fields_ = [("fast_n", ctypes.c_int32), ("fast_th", ctypes.c_float),
+                ("detector_frequency", ctypes.c_int32), ("filtering_frequency", ctypes.c_int32),
+                ("k", ctypes.c_int32), ("with_merge", ctypes.c_int32), ("box_size", ctypes.c_int32),
+                ("pd_r1", ctypes.c_int32), ("pd_c1", ctypes.c_int32), ("pd_r2", ctypes.c_int32),
+                ("pd_c2", ctypes.c_int32)]
+
+
 def profile_params(profile, nscales, fast_th=20.0, **kw):
     """'R': the reference matcher defaults on u8 frames (11x11 window, <=30
     iterations, 25 strongest-gradient pixels); 'B': 7x7 window, <=5 iterations,
@@ -119,6 +128,12 @@ def lib():
         L.cuimg_hip_tracker_run_host.argtypes = [vp, vp, ctypes.c_size_t]
         L.cuimg_hip_tracker_run_host_rgb.argtypes = [vp, vp, ctypes.c_size_t, c_int]
         L.cuimg_hip_tracker_clear.argtypes = [vp]
+        L.cuimg_hip_tracker_clear_scale.argtypes = [vp, c_int]
+        L.cuimg_hip_tracker_update_input.argtypes = [vp, P(Img)]
+        L.cuimg_hip_tracker_update_input_host.argtypes = [vp, vp, ctypes.c_size_t]
+        L.cuimg_hip_tracker_step.argtypes = [vp]
+        L.cuimg_hip_tracker_set_scale_params.argtypes = [vp, c_int, P(ScaleParams)]
+        L.cuimg_hip_tracker_get_scale_params.argtypes = [vp, c_int, P(ScaleParams)]
         L.cuimg_hip_tracker_nparticles.argtypes = [vp, c_int, P(ctypes.c_uint32)]
         L.cuimg_hip_tracker_get_particles.argtypes = [vp, c_int, vp, ctypes.c_uint32, P(ctypes.c_uint32)]
         L.cuimg_hip_tracker_set_particles.argtypes = [vp, c_int, vp, ctypes.c_uint32]
@@ -409,6 +424,34 @@ class Tracker:
 
     def clear(self):
         self.ctx.check(self.L.cuimg_hip_tracker_clear(self.h), "tracker_clear")
+
+    def update_input(self, frame):
+        """tracker::update_input: the pyramid of the next frame (device image or
+        host array), without the frame step."""
+        if isinstance(frame, (DeviceImage, View)):
+            self.ctx.check(self.L.cuimg_hip_tracker_update_input(self.h, ctypes.byref(frame.d)), "update_input")
+            return
+        frame = host_frame(frame, self.shape)
+        if frame.ndim != 2:
+            raise ValueError("update_input takes a gray frame")
+        self.ctx.check(self.L.cuimg_hip_tracker_update_input_host(self.h, frame.ctypes.data, frame.strides[0]),
+                       "update_input")
+
+    def step(self):
+        """tracker::run(): the frame step on the input last given to update_input."""
+        self.ctx.check(self.L.cuimg_hip_tracker_step(self.h), "tracker_step")
+
+    def clear_scale(self, s):
+        """The strategy's clear() of scale s: its frame counter restarts."""
+        self.ctx.check(self.L.cuimg_hip_tracker_clear_scale(self.h, s), "clear_scale")
+
+    def scale_params(self, s):
+        p = ScaleParams()
+        self.ctx.check(self.L.cuimg_hip_tracker_get_scale_params(self.h, s, ctypes.byref(p)), "get_scale_params")
+        return p
+
+    def set_scale_params(self, s, p):
+        self.ctx.check(self.L.cuimg_hip_tracker_set_scale_params(self.h, s, ctypes.byref(p)), "set_scale_params")
 
     def nparticles(self, s):
         n = ctypes.c_uint32()

@@ -72,13 +72,15 @@ struct grad_job {
   int in_pitch, out_pitch;
   int nr, nc;
 };
-constexpr int kGradRows = 4;  // rows per gradient workgroup
+constexpr int kGradRows = 2;     // rows per gradient workgroup
+constexpr int kGradCols = 1024;  // columns per gradient workgroup (4 per thread)
 
 struct level_job {
   int stream, s;
   int nr, nc;
   job_params prm;
   level_counters* cnt;
+  level_counters* cnt0;  // the stream's scale 0 (its step count sets the frame slot)
   uint8_t* u8[2];
   int u8_pitch;
   uint8_t* sal;
@@ -152,7 +154,9 @@ struct cuimg_hip_batch {
   int lk_ppw = 4;     // particles per matcher wave (tuning)
   int blur_th = 0;    // blur tile height, 0 = heuristic (tuning)
   bool use_graph = true;
-  hipGraphExec_t exec = nullptr;
+  hipGraphExec_t exec = nullptr;        // frame step with the pyramid of the staged frame
+  hipGraphExec_t exec_step = nullptr;   // frame step on the input built by update_input
+  bool input_ready = false;             // update_input staged a frame the next step has not used
   std::vector<host_ingest> ingest;           // per stream, for run_host
   hipStream_t copy = nullptr;                // host-frame uploads (default priority)
   std::vector<void*> allocs;
@@ -194,7 +198,7 @@ __device__ __forceinline__ step_flags flags_of(const level_job& J) {
   const job_params prm = uni(J.prm);
   f.frame = uni(J.cnt->frame);
   f.cur = uni(J.cnt->cur) & 1u;
-  f.slot = f.frame & 1u;
+  f.slot = uni(J.cnt0->steps) & 1u;
   f.match = f.frame > 0;
   f.detect = f.frame % (uint32_t)prm.det_freq == 0;
   f.merge = prm.with_merge && f.frame % (uint32_t)prm.filt_freq == 0;
@@ -218,7 +222,7 @@ __global__ __launch_bounds__(kPyrThreads) void step_pyramid_kernel(const stream_
   const job_block jb = job_of_block(prefix, nstreams);
   const int st = jb.j;
   const stream_job& SJ = S[st];
-  const uint32_t slot = jobs[SJ.first].cnt->frame & 1u;
+  const uint32_t slot = jobs[SJ.first].cnt->steps & 1u;
   const int* src = reinterpret_cast<const int*>(&SJ.pyr[slot]);
   int* dst = reinterpret_cast<int*>(&a);
   for (int i = threadIdx.x; i < (int)(sizeof(pyr_args) / 4); i += blockDim.x) dst[i] = src[i];
@@ -237,7 +241,7 @@ template <int K>
 __global__ __launch_bounds__(64) void step_blur_kernel(const level_job* jobs, const uint32_t* prefix, int njobs,
                                                        coef_pack cp) {
   CUIMG_JOB(prefix, njobs);
-  const gauss_job G = uni(J.blur[J.cnt->frame & 1u]);
+  const gauss_job G = uni(J.blur[J.cnt0->steps & 1u]);
   gauss_tile<uint8_t, float, 4, K>(G, lb % G.tiles_x, lb / G.tiles_x, cp);
 }
 
@@ -245,9 +249,10 @@ __global__ __launch_bounds__(64) void step_blur_kernel(const level_job* jobs, co
 // new frame (the matcher reads it as the previous frame's gradient next step).
 __global__ __launch_bounds__(256) void step_grad_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
   CUIMG_JOB(prefix, njobs);
-  const grad_job g = uni(J.grad[J.cnt->frame & 1u]);
-  const int r0 = lb * kGradRows;
-  sobel_rows_body(g.in, g.in_pitch, g.out, g.out_pitch, g.nr, g.nc, r0, min(r0 + kGradRows, g.nr));
+  const grad_job g = uni(J.grad[J.cnt0->steps & 1u]);
+  const int tx = (g.nc + kGradCols - 1) / kGradCols;
+  const int r0 = (lb / tx) * kGradRows, c = (lb % tx) * kGradCols + 4 * (int)threadIdx.x;
+  sobel_rows_body(g.in, g.in_pitch, g.out, g.out_pitch, g.nr, g.nc, r0, min(r0 + kGradRows, g.nr), c);
 }
 
 // before_matching (sparse <- -1, particle_container.hpp:187-193) and the
@@ -341,6 +346,7 @@ __global__ __launch_bounds__(1024) void step_mres_kernel(const level_job* jobs) 
     J.cnt->cur = c;
     J.cnt->compact_has_run = f.detect ? 1u : 0u;
     J.cnt->frame = f.frame + 1;
+    if (J.s == 0) J.cnt->steps = J.cnt->steps + 1;  // read by the next step's kernels only
   }
 }
 
@@ -457,6 +463,7 @@ static void destroy_batch(cuimg_hip_batch* b) {
   drop_child(b->ctx, b);
   (void)hipStreamSynchronize(b->ctx->stream);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->exec_step) (void)hipGraphExecDestroy(b->exec_step);
   for (auto& R : b->lv) {
     for (int k = 0; k < 2; k++) {
       image_free(b->ctx, &R.u8[k]);
@@ -516,12 +523,19 @@ static int check_params(cuimg_hip_ctx* ctx, int nrows, int ncols, const cuimg_tr
   return 0;
 }
 
+// Detector boxes of a scale: (int)(n / box) per axis (fast_detector.hpp:225).
+static void set_boxes(level_res& R) {
+  R.nbr = (int)((float)R.nr / (float)R.sp.box_size);
+  R.nbc = (int)((float)R.nc / (float)R.sp.box_size);
+}
+
 static int alloc_level(cuimg_hip_batch* b, const cuimg_tracker_params* P, int r, int c, level_res& R) {
   cuimg_hip_ctx* ctx = b->ctx;
   R.nr = r;
   R.nc = c;
+  // particle_domain_ = domain - border(3) (pyrlk_tracker.hpp:792)
   R.sp = cuimg_scale_params{P->fast_n, P->fast_th, P->detector_frequency, P->filtering_frequency, P->k,
-                            P->with_merge};
+                            P->with_merge, P->box_size, 3, 3, r - 4, c - 4};
   for (int k = 0; k < 2; k++) {
     CUIMG_TRY(image_alloc(ctx, r, c, 3, CUIMG_U8, &R.u8[k]));
     if (P->profile == 1) {
@@ -531,13 +545,14 @@ static int alloc_level(cuimg_hip_batch* b, const cuimg_tracker_params* P, int r,
   }
   CUIMG_TRY(image_alloc(ctx, r, c, 1, CUIMG_U8, &R.sal));
   CUIMG_TRY(image_alloc(ctx, r, c, 7, CUIMG_U32, &R.sparse));
-  R.nbr = (int)((float)r / (float)P->box_size);
-  R.nbc = (int)((float)c / (float)P->box_size);
-  const size_t nboxes = (size_t)R.nbr * R.nbc;
+  set_boxes(R);
+  // detector buffers sized for the smallest box (1), so the box size of a
+  // scale can change at any frame
+  const size_t nboxes = (size_t)r * c;
   R.cap = (uint32_t)std::max<size_t>((size_t)r * c / 4, 4096);
   R.nblocks = ceil_div((int)R.cap, kCompactBlock);
   CUIMG_TRY(dev_alloc_n(b, std::max<size_t>(nboxes, 1), &R.boxcode));
-  CUIMG_TRY(dev_alloc_n(b, std::max<size_t>((size_t)R.nbr * P->box_size, 1), &R.subcnt));
+  CUIMG_TRY(dev_alloc_n(b, std::max<size_t>((size_t)r, 1), &R.subcnt));
   for (int k = 0; k < 2; k++) CUIMG_TRY(dev_alloc_n(b, R.cap, &R.parts[k]));
   CUIMG_TRY(dev_alloc_n(b, R.cap + nboxes, &R.matches));
   CUIMG_TRY(dev_alloc_n(b, R.cap, &R.flags));
@@ -585,6 +600,7 @@ static void build_job(cuimg_hip_batch* b, int i, int s) {
   J.nc = R.nc;
   J.prm = job_params{R.sp.detector_frequency, R.sp.filtering_frequency, R.sp.with_merge, R.sp.fast_n, R.sp.fast_th};
   J.cnt = b->cnt + j;
+  J.cnt0 = b->cnt + b->first[i];
   for (int k = 0; k < 2; k++) J.u8[k] = static_cast<uint8_t*>(R.u8[k].begin);
   J.u8_pitch = R.u8[0].pitch;
   J.sal = static_cast<uint8_t*>(R.sal.begin);
@@ -608,11 +624,10 @@ static void build_job(cuimg_hip_batch* b, int i, int s) {
       a.dec_accept = R.dec_accept;
       a.dec_tr = R.dec_tr;
       a.dec_dist = R.dec_dist;
-      // particle_domain_ = domain - border(3) (pyrlk_tracker.hpp:792)
-      a.pd_r1 = 3;
-      a.pd_c1 = 3;
-      a.pd_r2 = R.nr - 4;
-      a.pd_c2 = R.nc - 4;
+      a.pd_r1 = R.sp.pd_r1;
+      a.pd_c1 = R.sp.pd_c1;
+      a.pd_r2 = R.sp.pd_r2;
+      a.pd_c2 = R.sp.pd_c2;
       a.P = make_lk_params(P.lk_ws, P.lk_max_iters, std::min(P.lk_top_n, P.lk_ws * P.lk_ws), P.lk_eps, R.sp.k,
                            P.profile == 0 ? 1 : 0);
     }
@@ -620,7 +635,7 @@ static void build_job(cuimg_hip_batch* b, int i, int s) {
     detect_args& d = J.det[cur];
     d.nr = R.nr;
     d.nc = R.nc;
-    d.box = P.box_size;
+    d.box = R.sp.box_size;
     d.nbr = R.nbr;
     d.nbc = R.nbc;
     d.fast_th = R.sp.fast_th;
@@ -705,12 +720,12 @@ static int build_maps(cuimg_hip_batch* b) {
     const level_job& J = b->hj[j];
     const long cap = R.cap;
     cnt[K_BLUR].push_back(b->P0.profile == 1 ? (uint32_t)(ceil_div(R.nc, 256) * ceil_div(R.nr, th)) : 0u);
-    cnt[K_GRAD].push_back(b->P0.profile == 1 ? (uint32_t)ceil_div(R.nr, kGradRows) : 0u);
+    cnt[K_GRAD].push_back(b->P0.profile == 1 ? (uint32_t)(ceil_div(R.nr, kGradRows) * ceil_div(R.nc, kGradCols)) : 0u);
     cnt[K_FILL].push_back((uint32_t)J.fill_sparse_blocks);
     cnt[K_PMASK].push_back((uint32_t)std::max<long>(1, std::min<long>(512, (cap + 255) / 256)));
     cnt[K_FAST].push_back((uint32_t)(J.fast_tiles_x * ceil_div(R.nr, kFTH)));
     cnt[K_DETC].push_back((uint32_t)(R.nbr + R.nblocks));
-    cnt[K_SCATI].push_back((uint32_t)(R.nblocks + ceil_div(R.nbr * b->P[J.stream].box_size, 4)));
+    cnt[K_SCATI].push_back((uint32_t)(R.nblocks + ceil_div(R.nbr * R.sp.box_size, 4)));
     cnt[K_MSCAN].push_back((uint32_t)std::max<long>(1, std::min<long>(512, (cap + 255) / 256)));
     cnt[K_MR1].push_back(cnt[K_MSCAN].back());
   }
@@ -772,6 +787,10 @@ static int rebuild(cuimg_hip_batch* b) {
   if (b->exec) {
     (void)hipGraphExecDestroy(b->exec);
     b->exec = nullptr;
+  }
+  if (b->exec_step) {
+    (void)hipGraphExecDestroy(b->exec_step);
+    b->exec_step = nullptr;
   }
   return 0;
 }
@@ -850,15 +869,23 @@ static void launch_blur(cuimg_hip_batch* b) {
 }
 
 // The fixed launch sequence of one frame step of every stream.
-static int enqueue_step(cuimg_hip_batch* b, bool capturing) {
+static int enqueue_pyramid(cuimg_hip_batch* b) {
+  hipStream_t st = b->ctx->stream;
+  return launch(b->ctx, "pyramid_build", [&] {
+    step_pyramid_kernel<<<b->total[K_PYR], kPyrThreads, 0, st>>>(b->ds, b->dj, b->din, b->prefix_d[K_PYR], b->nstreams);
+  });
+}
+
+// with_pyramid: the step builds the pyramid of the frames in the input table
+// first (tracker::run(in)); otherwise it steps on the pyramid already in the
+// frame slot (tracker::run()).
+static int enqueue_step(cuimg_hip_batch* b, bool capturing, bool with_pyramid = true) {
   cuimg_hip_ctx* ctx = b->ctx;
   hipStream_t st = ctx->stream;
   const int nj = b->njobs;
   const level_job* J = b->dj;
   uint32_t* const* pf = b->prefix_d;
-  CUIMG_TRY(launch(ctx, "pyramid_build", [&] {
-    step_pyramid_kernel<<<b->total[K_PYR], kPyrThreads, 0, st>>>(b->ds, J, b->din, pf[K_PYR], b->nstreams);
-  }));
+  if (with_pyramid) CUIMG_TRY(enqueue_pyramid(b));
   // FAST needs only the pyramid: in a captured step it runs beside the matcher
   // on the second stream; plain launches (and per-launch timing on the context
   // stream) keep it in order, where two more stream calls per step would only
@@ -900,8 +927,15 @@ static int enqueue_step(cuimg_hip_batch* b, bool capturing) {
   return launch(ctx, "merge_resolve", [&] { step_mres_kernel<<<nj, 1024, 0, st>>>(J); });
 }
 
-static int batch_step(cuimg_hip_batch* b, const cuimg_img* frames) {
-  if (!b || !frames) return CUIMG_E_INVALID;
+static int read_counters(cuimg_hip_batch* b, int j, level_counters* c) {
+  CUIMG_HIP(b->ctx, hipMemcpyAsync(c, b->cnt + j, sizeof(level_counters), hipMemcpyDeviceToHost, b->ctx->stream));
+  CUIMG_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+  return 0;
+}
+
+// Writes the frames of the next step into the device input table (through
+// the pinned ring).
+static int write_inputs(cuimg_hip_batch* b, const cuimg_img* frames) {
   cuimg_hip_ctx* ctx = b->ctx;
   const int k = (int)(b->steps % kRing);
   frame_in* row = b->hin + (size_t)k * b->nstreams;
@@ -915,34 +949,75 @@ static int batch_step(cuimg_hip_batch* b, const cuimg_img* frames) {
   }
   CUIMG_HIP(ctx, hipMemcpyAsync(b->din, row, sizeof(frame_in) * b->nstreams, hipMemcpyHostToDevice, ctx->stream));
   CUIMG_HIP(ctx, hipEventRecord(b->ring_ev[k], ctx->stream));
+  return 0;
+}
+
+// One frame step, replayed from its captured graph (or launched).
+static int launch_step(cuimg_hip_batch* b, bool with_pyramid) {
+  cuimg_hip_ctx* ctx = b->ctx;
   if (b->use_graph && !ctx->profiling) {
-    if (!b->exec) {
+    hipGraphExec_t& exec = with_pyramid ? b->exec : b->exec_step;
+    if (!exec) {
       hipGraph_t g = nullptr;
       CUIMG_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      const int rc = enqueue_step(b, true);
+      const int rc = enqueue_step(b, true, with_pyramid);
       const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
       if (rc) {
         if (g) (void)hipGraphDestroy(g);
         return rc;
       }
       CUIMG_HIP(ctx, e);
-      const hipError_t ei = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+      const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       CUIMG_HIP(ctx, ei);
     }
-    CUIMG_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
+    CUIMG_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
   } else {
-    CUIMG_TRY(enqueue_step(b, false));
+    CUIMG_TRY(enqueue_step(b, false, with_pyramid));
   }
   b->steps++;
+  b->input_ready = false;
   return 0;
 }
 
-static int read_counters(cuimg_hip_batch* b, int j, level_counters* c) {
-  CUIMG_HIP(b->ctx, hipMemcpyAsync(c, b->cnt + j, sizeof(level_counters), hipMemcpyDeviceToHost, b->ctx->stream));
-  CUIMG_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+static int batch_step(cuimg_hip_batch* b, const cuimg_img* frames) {
+  if (!b || !frames) return CUIMG_E_INVALID;
+  CUIMG_TRY(write_inputs(b, frames));
+  return launch_step(b, true);
+}
+
+// tracker::update_input: the pyramid of the frames into the next frame slot.
+static int batch_update_input(cuimg_hip_batch* b, const cuimg_img* frames) {
+  if (!b || !frames) return CUIMG_E_INVALID;
+  CUIMG_TRY(write_inputs(b, frames));
+  CUIMG_TRY(enqueue_pyramid(b));
+  b->input_ready = true;
   return 0;
 }
+
+// tracker::run(): without a newly staged frame the last input is used again
+// (its pyramid is copied into the next frame slot, borders included).
+static int batch_run_step(cuimg_hip_batch* b) {
+  if (!b) return CUIMG_E_INVALID;
+  if (!b->input_ready) {
+    for (int i = 0; i < b->nstreams; i++) {
+      level_counters c;
+      CUIMG_TRY(read_counters(b, b->first[i], &c));
+      const int slot = (int)(c.steps & 1u);
+      for (int s = 0; s < b->P[i].nscales; s++) {
+        const level_res& R = b->lv[b->first[i] + s];
+        const cuimg_img& from = R.u8[slot ^ 1];
+        const cuimg_img& to = R.u8[slot];
+        const size_t off = (size_t)from.border * from.pitch + from.border;
+        CUIMG_HIP(b->ctx, hipMemcpyAsync(static_cast<uint8_t*>(to.begin) - off, static_cast<const uint8_t*>(from.begin) - off,
+                                         (size_t)from.pitch * (from.nrows + 2 * from.border), hipMemcpyDeviceToDevice,
+                                         b->ctx->stream));
+      }
+    }
+  }
+  return launch_step(b, false);
+}
+
 
 static int job_of(cuimg_hip_tracker* t, int s, int* j) {
   if (!t || !t->b) return CUIMG_E_INVALID;
@@ -1163,6 +1238,43 @@ int cuimg_hip_tracker_run_host(cuimg_hip_tracker* t, const uint8_t* frame, size_
   return cuimg_hip_tracker_run_host_rgb(t, frame, pitch, 1);
 }
 
+int cuimg_hip_tracker_update_input(cuimg_hip_tracker* t, const cuimg_img* frame) {
+  if (!t || !frame) return CUIMG_E_INVALID;
+  if (t->b->nstreams != 1)
+    return set_error(t->b->ctx, CUIMG_E_INVALID, "tracker_update_input: a stream of a batch steps with batch_step");
+  return batch_update_input(t->b, frame);
+}
+
+int cuimg_hip_tracker_update_input_host(cuimg_hip_tracker* t, const uint8_t* frame, size_t pitch) {
+  if (!t || !frame) return CUIMG_E_INVALID;
+  cuimg_hip_batch* b = t->b;
+  if (b->nstreams != 1)
+    return set_error(b->ctx, CUIMG_E_INVALID, "tracker_update_input: a stream of a batch steps with batch_step");
+  const cuimg_img* dev = nullptr;
+  CUIMG_TRY(stage_host_frame(b, t->stream, frame, pitch, 1, &dev));
+  const int rc = batch_update_input(b, dev);
+  CUIMG_TRY(host_frame_consumed(b, t->stream));
+  return rc;
+}
+
+int cuimg_hip_tracker_step(cuimg_hip_tracker* t) {
+  if (!t) return CUIMG_E_INVALID;
+  if (t->b->nstreams != 1)
+    return set_error(t->b->ctx, CUIMG_E_INVALID, "tracker_step: a stream of a batch steps with batch_step");
+  return batch_run_step(t->b);
+}
+
+int cuimg_hip_tracker_clear_scale(cuimg_hip_tracker* t, int scale) {
+  int j;
+  CUIMG_TRY(job_of(t, scale, &j));
+  // level_counters::frame of the scale (its particles and the stream's step
+  // count, which picks the frame slots, stay)
+  cuimg_hip_batch* b = t->b;
+  CUIMG_HIP(b->ctx, hipMemsetAsync(&b->cnt[j].frame, 0, sizeof(uint32_t), b->ctx->stream));
+  CUIMG_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+  return 0;
+}
+
 int cuimg_hip_tracker_set_tuning(cuimg_hip_tracker* t, int lk_ppw, int blur_th) {
   if (!t || !t->b) return CUIMG_E_INVALID;
   if (t->b->nstreams != 1)
@@ -1181,7 +1293,18 @@ int cuimg_hip_tracker_set_scale_params(cuimg_hip_tracker* t, int scale, const cu
   cuimg_hip_batch* b = t->b;
   if (!in || in->fast_n < 1 || in->fast_th < 0.f || in->detector_frequency < 1 || in->filtering_frequency < 1)
     return set_error(b->ctx, CUIMG_E_INVALID, "set_scale_params: bad settings");
-  b->lv[j].sp = *in;
+  if (in->box_size < 1 || in->box_size > kMaxBox)
+    return set_error(b->ctx, CUIMG_E_UNSUPPORTED, "set_scale_params: detector box %d not in 1..%d", in->box_size,
+                     kMaxBox);
+  level_res& R = b->lv[j];
+  const bool new_box = in->box_size != R.sp.box_size;
+  R.sp = *in;
+  if (new_box) {
+    // the box size changes the launch shape of the detector kernels
+    set_boxes(R);
+    CUIMG_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+    return rebuild(b);
+  }
   build_job(b, t->stream, scale);
   CUIMG_HIP(b->ctx, hipMemcpyAsync(b->dj + j, &b->hj[j], sizeof(level_job), hipMemcpyHostToDevice, b->ctx->stream));
   CUIMG_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
@@ -1283,8 +1406,8 @@ int cuimg_hip_tracker_level(cuimg_hip_tracker* t, int scale, cuimg_img* view) {
   CUIMG_TRY(job_of(t, scale, &j));
   if (!view) return CUIMG_E_INVALID;
   level_counters c;
-  CUIMG_TRY(read_counters(t->b, j, &c));
-  *view = t->b->lv[j].u8[(c.frame + 1) & 1];  // slot of the last frame
+  CUIMG_TRY(read_counters(t->b, t->b->first[t->stream], &c));
+  *view = t->b->lv[j].u8[(c.steps + 1) & 1];  // slot of the last frame
   return 0;
 }
 
@@ -1319,9 +1442,9 @@ int cuimg_hip_tracker_matcher_images(cuimg_hip_tracker* t, int scale, cuimg_img*
   int j;
   CUIMG_TRY(job_of(t, scale, &j));
   level_counters c;
-  CUIMG_TRY(read_counters(t->b, j, &c));
+  CUIMG_TRY(read_counters(t->b, t->b->first[t->stream], &c));
   const level_res& R = t->b->lv[j];
-  const int cur = (c.frame + 1) & 1, prev = cur ^ 1;
+  const int cur = (c.steps + 1) & 1, prev = cur ^ 1;
   const bool u8 = t->b->P0.profile == 0;
   if (A) *A = u8 ? R.u8[prev] : R.blur[prev];
   if (B) *B = u8 ? R.u8[cur] : R.blur[cur];

@@ -25,7 +25,8 @@ struct level_counters {
   uint32_t cur;      // index (0/1) of the current dense vector
   uint32_t compact_has_run;  // the last frame step ran the detector
   uint32_t pend1;    // merge: actors left open by round 1 (in merge_args::pend)
-  uint32_t pad;
+  uint32_t steps;    // scale 0 only: frame steps of the stream since clear (its
+                     // parity picks the frame slot of every scale)
   unsigned long long live_total;      // particles (age > 0) run through the matcher
   unsigned long long accepted_total;  // of which accepted (moved)
 };

@@ -1,6 +1,6 @@
 // tracker<F> for the GPU strategy (tracking2/tracker.h:10-47, .hpp:12-112):
-// same construction, run(), scale(s), pset(), strategy(), clear() and
-// nparticles().  All scales live in one device tracker; scale(s) returns the
+// same construction, run(in), update_input(in) + run(), input(), scale(s),
+// pset(), strategy(), clear() and nparticles().  All scales live in one device tracker; scale(s) returns the
 // facade of scale s.  The device state is created at the first run(), after
 // the setters the caller applies to the freshly built tracker.
 #ifndef CUIMG_TRACKING2_TRACKER_H_
@@ -48,17 +48,23 @@ namespace cuimg
       P.with_merge = 1;
       P.blur_sigma = 1;
       P.blur_K = 3;
-      S.h.sp.assign(nscales, cuimg_scale_params{P.fast_n, P.fast_th, P.detector_frequency,
-                                                P.filtering_frequency, P.k, P.with_merge});
       obox2d ds = d;
       for (int s = 0; s < nscales; s++)
       {
         S.domains.push_back(ds);
+        // particle domain: domain - border(3) (pyrlk_tracker.hpp:792)
+        S.h.sp.push_back(cuimg_scale_params{P.fast_n, P.fast_th, P.detector_frequency, P.filtering_frequency, P.k,
+                                            P.with_merge, P.box_size, 3, 3, ds.nrows() - 4, ds.ncols() - 4});
         ds = ds / 2;
       }
+      S.h.domains = S.domains;
       S.scales.push_back(this);
       for (int s = 1; s < nscales; s++)
         S.owned.emplace_back(new tracker(st_, s, S.domains[s])), S.scales.push_back(S.owned.back().get());
+      // tracker<F> ctor (tracker.hpp:12-26): each strategy knows its coarser
+      // neighbour, then init()
+      for (int s = 0; s + 1 < nscales; s++) S.scales[s]->strategy_.set_upper(&S.scales[s + 1]->strategy_);
+      for (int s = 0; s < nscales; s++) S.scales[s]->strategy_.init();
     }
 
     tracker(const tracker&) = delete;
@@ -87,6 +93,54 @@ namespace cuimg
       ensure();
       st_->h.ctx->check(cuimg_hip_tracker_run(st_->h.t, &in.kernel()), "tracker run");
       return *this;
+    }
+
+    // tracker::update_input (tracker.hpp:53-69): the pyramid of the next
+    // frame is built on the device; run() then steps on it (tracker.hpp:83-91;
+    // without a new input it steps on the previous one again).
+    tracker& update_input(const I& in)
+    {
+      if (s_ != 0) throw std::runtime_error("cuimg: update_input() is called on the finest scale");
+      ensure();
+      st_->h.ctx->check(cuimg_hip_tracker_update_input_host(st_->h.t, reinterpret_cast<const uint8_t*>(in.begin()),
+                                                            in.pitch()), "update_input");
+      return *this;
+    }
+
+    tracker& update_input(const device_image2d<gl8u>& in)
+    {
+      if (s_ != 0) throw std::runtime_error("cuimg: update_input() is called on the finest scale");
+      ensure();
+      st_->h.ctx->check(cuimg_hip_tracker_update_input(st_->h.t, &in.kernel()), "update_input");
+      return *this;
+    }
+
+    tracker& run()
+    {
+      if (s_ != 0) throw std::runtime_error("cuimg: run() is called on the finest scale");
+      ensure();
+      st_->h.ctx->check(cuimg_hip_tracker_step(st_->h.t), "tracker step");
+      return *this;
+    }
+
+    // input(): this scale's input level of the last frame (border 3,
+    // clamp-filled), copied from the device.
+    const I& input() const
+    {
+      if (!st_->h.t) return input_;
+      cuimg_img v;
+      st_->h.ctx->check(cuimg_hip_tracker_level(st_->h.t, s_, &v), "input");
+      const int b = v.border;
+      if (input_.nrows() != v.nrows || input_.ncols() != v.ncols || input_.border() != b)
+        input_ = I(obox2d(v.nrows, v.ncols), b);
+      cuimg_img all = v;  // domain plus border
+      all.begin = static_cast<uint8_t*>(v.begin) - (ptrdiff_t)b * v.pitch - b;
+      all.nrows = v.nrows + 2 * b;
+      all.ncols = v.ncols + 2 * b;
+      all.border = 0;
+      uint8_t* host = reinterpret_cast<uint8_t*>(input_.begin()) - (ptrdiff_t)b * input_.pitch() - b;
+      st_->h.ctx->check(cuimg_hip_image_download(st_->h.ctx->get(), &all, host, input_.pitch()), "input");
+      return input_;
     }
 
     const obox2d& domain() const { return domain_; }
@@ -130,6 +184,7 @@ namespace cuimg
       for (size_t s = 0; s < h.sp.size(); s++) h.push((int)s);
     }
 
+    mutable I input_;               // input() cache
     std::unique_ptr<state> owner_;  // the finest scale owns the shared state
     state* st_;
     int s_;

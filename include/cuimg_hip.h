@@ -196,9 +196,10 @@ typedef struct {
 typedef struct cuimg_hip_tracker cuimg_hip_tracker;
 
 /* Settings the reference keeps per scale (one pyrlk_cpu strategy and one
- * fast_detector per scale): fast_detector::set_n / set_fast_threshold
- * (tracking2/fast_detector.hpp:170-193), pyrlk_cpu::set_detector_frequency /
- * set_filtering_frequency / set_k / set_with_merge (pyrlk_tracker.h:45-58). */
+ * fast_detector per scale): fast_detector::set_n / set_fast_threshold /
+ * set_box_size (tracking2/fast_detector.hpp:170-199), pyrlk_cpu::
+ * set_detector_frequency / set_filtering_frequency / set_k / set_with_merge /
+ * set_particle_domain (pyrlk_tracker.h:45-58, pyrlk_tracker.hpp:894-897). */
 typedef struct {
   int32_t fast_n;
   float fast_th;
@@ -206,6 +207,10 @@ typedef struct {
   int32_t filtering_frequency;
   int32_t k;
   int32_t with_merge;
+  int32_t box_size;             /* 1..15 */
+  /* particle domain, an inclusive (row, col) box: particles outside it are
+   * removed and matches must land in it; default domain - border(3) */
+  int32_t pd_r1, pd_c1, pd_r2, pd_c2;
 } cuimg_scale_params;
 
 /* tracker<pyrlk_cpu<fast_detector<cpu>, iterative_lk_match_5x5>>(domain, nscales)
@@ -232,6 +237,18 @@ int cuimg_hip_tracker_set_scale_params(cuimg_hip_tracker* t, int scale, const cu
 int cuimg_hip_tracker_get_scale_params(cuimg_hip_tracker* t, int scale, cuimg_scale_params* out);
 /* tracker::clear (tracking2/tracker.hpp:95-102). */
 int cuimg_hip_tracker_clear(cuimg_hip_tracker* t);
+/* pyrlk_cpu::clear of scale s (pyrlk_tracker.hpp:1015-1019): the scale's
+ * frame counter restarts at 0 (its next frame matches nothing and detects);
+ * its particles stay. */
+int cuimg_hip_tracker_clear_scale(cuimg_hip_tracker* t, int scale);
+/* tracker::update_input (tracking2/tracker.hpp:53-69): the pyramid of the
+ * next frame (device or host gray frame, staged like tracker_run_host), and
+ * tracker::run() (:83-91): the frame step of every scale on the input last
+ * staged (on the previous input again when none was staged since the last
+ * step).  tracker_run = update_input + step, in one graph replay. */
+int cuimg_hip_tracker_update_input(cuimg_hip_tracker* t, const cuimg_img* frame);
+int cuimg_hip_tracker_update_input_host(cuimg_hip_tracker* t, const uint8_t* frame, size_t pitch);
+int cuimg_hip_tracker_step(cuimg_hip_tracker* t);
 /* Number of entries of scale(s).pset().dense_particles(); synchronises. */
 int cuimg_hip_tracker_nparticles(cuimg_hip_tracker* t, int scale, uint32_t* n);
 /* Copies dense_particles() of scale s to host (cap entries); synchronises. */

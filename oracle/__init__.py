@@ -138,6 +138,10 @@ class Oracle:
         for name in ("orc_tracker_destroy", "orc_tracker_clear"):
             getattr(L, name).argtypes = [vp]
         L.orc_tracker_run.argtypes = [vp, P(ctypes.c_uint8)]
+        L.orc_tracker_update_input.argtypes = [vp, P(ctypes.c_uint8)]
+        L.orc_tracker_step.argtypes = [vp]
+        L.orc_tracker_clear_scale.argtypes = [vp, c_int]
+        L.orc_tracker_set_scale.argtypes = [vp, c_int, c_int, c_int, c_int, c_int, c_int]
         L.orc_tracker_nscales.argtypes = [vp]
         L.orc_tracker_level_dims.argtypes = [vp, c_int, P(c_int), P(c_int)]
         L.orc_tracker_nparticles.argtypes = [vp, c_int]
@@ -366,6 +370,21 @@ class OracleTracker:
 
     def clear(self):
         self.L.orc_tracker_clear(self.h)
+
+    def update_input(self, frame):
+        frame = np.ascontiguousarray(frame, np.uint8)
+        self.L.orc_tracker_update_input(self.h, _ptr(frame, ctypes.c_uint8))
+
+    def step(self):
+        self.L.orc_tracker_step(self.h)
+
+    def clear_scale(self, s):
+        self.L.orc_tracker_clear_scale(self.h, s)
+
+    def set_scale(self, s, box_size, domain):
+        """fast_detector::set_box_size and pyrlk_cpu::set_particle_domain of
+        scale s; domain = (r1, c1, r2, c2) inclusive."""
+        self.L.orc_tracker_set_scale(self.h, s, box_size, *domain)
 
     @property
     def nscales(self):

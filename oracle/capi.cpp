@@ -131,6 +131,14 @@ void* orc_tracker_create(int nr, int nc, const orc_tracker_params* p) {
 void orc_tracker_destroy(void* t) { delete static_cast<tracker*>(t); }
 void orc_tracker_clear(void* t) { static_cast<tracker*>(t)->clear(); }
 void orc_tracker_run(void* t, const uint8_t* frame) { static_cast<tracker*>(t)->run(frame); }
+void orc_tracker_update_input(void* t, const uint8_t* frame) { static_cast<tracker*>(t)->update_input(frame); }
+void orc_tracker_step(void* t) { static_cast<tracker*>(t)->step(); }
+void orc_tracker_clear_scale(void* t, int s) { static_cast<tracker*>(t)->levels[s].frame_cpt = 0; }
+void orc_tracker_set_scale(void* t, int s, int box_size, int r1, int c1, int r2, int c2) {
+  level_state& L = static_cast<tracker*>(t)->levels[s];
+  L.box_size = box_size;
+  L.pdomain = box{r1, c1, r2, c2};
+}
 int orc_tracker_nscales(void* t) { return (int)static_cast<tracker*>(t)->levels.size(); }
 
 void orc_tracker_level_dims(void* t, int s, int* nr, int* nc) {

@@ -66,6 +66,13 @@ void* orc_tracker_create(int nr, int nc, const orc_tracker_params* p);
 void orc_tracker_destroy(void* t);
 void orc_tracker_clear(void* t);
 void orc_tracker_run(void* t, const uint8_t* frame);
+/* tracker::update_input and tracker::run() (tracking2/tracker.hpp:53-91). */
+void orc_tracker_update_input(void* t, const uint8_t* frame);
+void orc_tracker_step(void* t);
+/* pyrlk_cpu::clear of one scale (frame counter only, pyrlk_tracker.hpp:1015-1019). */
+void orc_tracker_clear_scale(void* t, int s);
+/* fast_detector::set_box_size and pyrlk_cpu::set_particle_domain of one scale. */
+void orc_tracker_set_scale(void* t, int s, int box_size, int r1, int c1, int r2, int c2);
 int orc_tracker_nscales(void* t);
 void orc_tracker_level_dims(void* t, int s, int* nr, int* nc);
 int orc_tracker_nparticles(void* t, int s);

@@ -40,7 +40,9 @@ struct level_state {
   std::vector<lk_decision> last_decisions;
   bool compact_has_run = false;
   int frame_cpt = 0;
-  box pdomain{0, 0, -1, -1};
+  bool seen = false;  // a frame went through this scale since the last tracker clear
+  box pdomain{0, 0, -1, -1};  // pyrlk_cpu::particle_domain_
+  int box_size = 3;             // fast_detector::set_box_size of this scale
 };
 
 // Detector mask of a dense vector (255, 0 on the 3x3 block of every particle).
@@ -60,6 +62,7 @@ struct tracker {
 
   tracker(int nr, int nc, const tracker_params& p);
   void run(const uint8_t* frame);
+  void step();  // tracker::run() on the current input
   void clear();
 
   void update_input(const uint8_t* frame);

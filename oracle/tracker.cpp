@@ -20,6 +20,7 @@ tracker::tracker(int nr, int nc, const tracker_params& p) : P(p) {
     L.nc = c;
     L.sparse = image<int32_t>(r, c, -1);  // particle_container ctor memset -1
     L.pdomain = inner(r, c, 3);           // pyrlk_cpu ctor: d - border(3)
+    L.box_size = P.box_size;
     levels.push_back(std::move(L));
     r = r / 2;  // obox2d / 2 (tracking2/tracker.hpp:18-26)
     c = c / 2;
@@ -34,6 +35,7 @@ void tracker::clear() {
   // container (sparse <- -1, vectors emptied).
   for (auto& L : levels) {
     L.frame_cpt = 0;
+    L.seen = false;
     L.parts.clear();
     L.matches.clear();
     std::fill(L.sparse.px.begin(), L.sparse.px.end(), -1);
@@ -235,7 +237,7 @@ void container_add(std::vector<particle>& parts, image<int32_t>& sparse,
 }
 
 void tracker::new_particles(level_state& L) {
-  container_add(L.parts, L.sparse, fast_select(L.sal, L.sparse, P.box_size, P.fast_th));
+  container_add(L.parts, L.sparse, fast_select(L.sal, L.sparse, L.box_size, P.fast_th));
 }
 
 // particle_container::compact(cpu) (tracking2/particle_container.hpp:216-252):
@@ -302,16 +304,21 @@ void merge_state(std::vector<particle>& parts, const image<int32_t>& sparse) {
 // filter() is a no-op: its multiscale counts are never set to a non-zero value
 // (the fusion kernel is commented out, :958-968) and the finest level's counts
 // are uninitialised memory, defined here as zero.
+// After a scale's own clear() (frame counter 0, tracker not cleared) the
+// reference copies the current frame into prev_input_ and leaves its gradient
+// from two frames before, which finer scales then see through get_flow_at;
+// here the scale keeps its two most recent frames instead (the matcher images
+// of the previous and the current frame), which is what its frame counter
+// restarting changes nothing about.
 void tracker::update_level(int s) {
   level_state& L = levels[s];
-  if (L.frame_cpt > 0) L.prev_u8 = L.cur_u8;
+  if (L.frame_cpt > 0 || L.seen) L.prev_u8 = L.cur_u8;
   else L.prev_u8 = L.in_u8;
   L.cur_u8 = L.in_u8;
+  if (L.seen) prepare_matcher_images(L);
+  L.seen = true;
 
-  if (L.frame_cpt > 0) {
-    prepare_matcher_images(L);
-    match_particles(s);
-  }
+  if (L.frame_cpt > 0) match_particles(s);
   if (!(L.frame_cpt % P.detector_frequency)) {
     detector_mask(L);
     fast_saliency(L.cur_u8, L.mask, P.fast_n, (int)P.fast_th, L.sal);
@@ -327,6 +334,10 @@ void tracker::update_level(int s) {
 // tracker::run (tracking2/tracker.hpp:73-91): coarsest level first.
 void tracker::run(const uint8_t* frame) {
   update_input(frame);
+  step();
+}
+
+void tracker::step() {
   for (int s = (int)levels.size() - 1; s >= 0; s--) update_level(s);
 }
 

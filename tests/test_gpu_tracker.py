@@ -9,6 +9,8 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 from lockstep import TOL_PX, run_lockstep, to_cu  # noqa: F401
+from lockstep import compare_frame as lockstep_compare
+from lockstep import new_stats as lockstep_new_stats
 
 
 def assert_bit_exact(st, nframes):
@@ -277,3 +279,71 @@ def test_free_run_matches_committed_fixture(ctx, profile):
         t.run(v[f])
         assert m.state(t, m.NSCALES) == want[f], (profile, f)
     t.close()
+
+
+def test_per_scale_box_size_and_particle_domain(ctx, orc):
+    """fast_detector::set_box_size and pyrlk_cpu::set_particle_domain per scale
+    (changed before the first frame and again mid-run) in lockstep with the
+    oracle: bit-exact."""
+    video = cu.synth_video(0xC0FFEE01, 240, 320, 10, 1)
+    p_o = oracle.profile_params("B", 3, 12.0, sum_lanes=oracle.gpu_sum_lanes(7))
+    o = orc.tracker(240, 320, p_o)
+    g = cu.Tracker(ctx, 240, 320, cu.profile_params("B", 3, 12.0))
+    st = lockstep_new_stats()
+    settings = {0: [(0, 3, (20, 30, 200, 280)), (1, 5, (3, 3, 116, 156))],
+                5: [(0, 2, (3, 3, 236, 316)), (2, 4, (10, 10, 50, 70))]}
+    for f in range(10):
+        for s, box, dom in settings.get(f, []):
+            o.set_scale(s, box, dom)
+            sp = g.scale_params(s)
+            sp.box_size = box
+            sp.pd_r1, sp.pd_c1, sp.pd_r2, sp.pd_c2 = dom
+            g.set_scale_params(s, sp)
+        pre = [o.particles(s) for s in range(3)]
+        for s in range(3):
+            g.set_particles(s, to_cu(pre[s]))
+        o.run(video[f])
+        g.run(video[f])
+        lockstep_compare(o, g, 3, st, pre)
+    print(st)
+    assert st["tracked"] > 500
+    assert_bit_exact(st, 10)
+    # particles outside a scale's domain are removed
+    p = g.particles(2)
+    live = p[p["age"] > 1]
+    assert len(live) and (live["pos_r"] >= 10).all() and (live["pos_r"] < 51).all()
+
+
+def test_update_input_then_step_equals_run(ctx, orc):
+    """tracker::update_input + tracker::run() (and run() with no new input,
+    which steps again on the last input), plus the strategy's clear() of one
+    scale, against the oracle doing the same: bit-exact."""
+    video = cu.synth_video(0xC0FFEE03, 240, 320, 8, 1)
+    p_o = oracle.profile_params("B", 3, 12.0, sum_lanes=oracle.gpu_sum_lanes(7))
+    o = orc.tracker(240, 320, p_o)
+    g = cu.Tracker(ctx, 240, 320, cu.profile_params("B", 3, 12.0))
+    dev = ctx.image(240, 320, cu.U8)
+    for f in range(8):
+        if f == 4:  # run() on the previous input again
+            o.step()
+            g.step()
+        else:
+            o.update_input(video[f])
+            if f % 2:
+                dev.upload(video[f])
+                g.update_input(dev)
+            else:
+                g.update_input(video[f])
+            o.step()
+            g.step()
+        if f == 5:
+            o.clear_scale(1)
+            g.clear_scale(1)
+        for s in range(3):
+            assert np.array_equal(o.level(s), g.level(s)), (f, s)
+            po, pg = o.particles(s), g.particles(s)
+            assert len(po) == len(pg), (f, s)
+            for k in ("speed_r", "speed_c", "pos_r", "pos_c", "age"):
+                assert np.array_equal(po[k], pg[k]), (f, s, k)
+            assert np.array_equal(o.sparse(s), g.sparse(s)), (f, s)
+    assert sum(len(g.particles(s)) for s in range(3)) > 300
