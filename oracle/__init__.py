@@ -44,7 +44,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle_cuimg.so")
+# CUIMG_ORACLE_LIB selects another build of the same library (the
+# UndefinedBehaviorSanitizer build of `make -C oracle ubsan-lib`).
+LIB_PATH = os.environ.get("CUIMG_ORACLE_LIB") or os.path.join(_HERE, "liboracle_cuimg.so")
 
 
 class Particle(ctypes.Structure):

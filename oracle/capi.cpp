@@ -14,10 +14,15 @@ double lk_loss_grad_f64(const image<double>& I0, const image<double>& I1, const 
 
 using namespace orc;
 
+// memcpy of `bytes` bytes; an empty copy may have null pointers (memcpy's may not).
+static void copy_bytes(void* dst, const void* src, size_t bytes) {
+  if (bytes) std::memcpy(dst, src, bytes);
+}
+
 template <typename T>
 static image<T> wrap(const T* p, int nr, int nc) {
   image<T> im(nr, nc);
-  std::memcpy(im.px.data(), p, sizeof(T) * (size_t)nr * nc);
+  copy_bytes(im.px.data(), p, sizeof(T) * (size_t)nr * nc);
   return im;
 }
 
@@ -42,7 +47,7 @@ void orc_gaussian_sep_f32(const float* in, int nr, int nc, int ch, const float* 
 void orc_gaussian_sep_u8(const uint8_t* in, int nr, int nc, const float* coef, int K, int fused, float* out) {
   image<float> o;
   gaussian_sep_f32<uint8_t>(wrap(in, nr, nc), coef, K, o, fused != 0);
-  std::memcpy(out, o.px.data(), sizeof(float) * o.px.size());
+  copy_bytes(out, o.px.data(), sizeof(float) * o.px.size());
 }
 
 void orc_gaussian_sep_f64(const double* in, int nr, int nc, int ch, const double* coef, int K,
@@ -58,13 +63,13 @@ void orc_gaussian_sep_f64(const double* in, int nr, int nc, int ch, const double
 void orc_pyr_down_u8(const uint8_t* in, int nr, int nc, uint8_t* out) {
   image<uint8_t> o;
   pyr_down_u8(wrap(in, nr, nc), o);
-  std::memcpy(out, o.px.data(), o.px.size());
+  copy_bytes(out, o.px.data(), o.px.size());
 }
 
 void orc_pyr_down_f32(const float* in, int nr, int nc, float* out) {
   image<float> o;
   pyr_down_f32(wrap(in, nr, nc), o);
-  std::memcpy(out, o.px.data(), sizeof(float) * o.px.size());
+  copy_bytes(out, o.px.data(), sizeof(float) * o.px.size());
 }
 
 uint8_t orc_fast_score(const uint8_t* in, int nr, int nc, int r, int c, int n, int th) {
@@ -75,13 +80,13 @@ void orc_fast_saliency(const uint8_t* in, int nr, int nc, const uint8_t* mask, i
                        uint8_t* out) {
   image<uint8_t> s;
   fast_saliency(wrap(in, nr, nc), wrap(mask, nr, nc), n, th, s);
-  std::memcpy(out, s.px.data(), s.px.size());
+  copy_bytes(out, s.px.data(), s.px.size());
 }
 
 void orc_sobel_gradient(const float* prev, int nr, int nc, int integer_source, float* out2) {
   image<f2> g;
   sobel_gradient(wrap(prev, nr, nc), integer_source != 0, g);
-  std::memcpy(out2, g.px.data(), sizeof(f2) * g.px.size());
+  copy_bytes(out2, g.px.data(), sizeof(f2) * g.px.size());
 }
 
 void orc_lk_match(const float* A, const float* B, const float* G, int nr, int nc, int ws,
@@ -155,14 +160,14 @@ int orc_tracker_get_particles(void* t, int s, orc_particle* out, int cap) {
   const level_state& L = static_cast<tracker*>(t)->levels[s];
   int n = (int)L.parts.size();
   if (n > cap) return -n;
-  std::memcpy(out, L.parts.data(), sizeof(particle) * n);
+  copy_bytes(out, L.parts.data(), sizeof(particle) * n);
   return n;
 }
 
 int orc_tracker_set_particles(void* t, int s, const orc_particle* in, int n) {
   level_state& L = static_cast<tracker*>(t)->levels[s];
   L.parts.resize(n);
-  std::memcpy(L.parts.data(), in, sizeof(particle) * n);
+  copy_bytes(L.parts.data(), in, sizeof(particle) * n);
   return n;
 }
 
@@ -170,7 +175,7 @@ int orc_tracker_get_matches(void* t, int s, uint32_t* out, int cap) {
   const level_state& L = static_cast<tracker*>(t)->levels[s];
   int n = (int)L.matches.size();
   if (n > cap) return -n;
-  std::memcpy(out, L.matches.data(), sizeof(uint32_t) * n);
+  copy_bytes(out, L.matches.data(), sizeof(uint32_t) * n);
   return n;
 }
 
@@ -180,25 +185,25 @@ int orc_tracker_compact_has_run(void* t, int s) {
 
 void orc_tracker_get_sparse(void* t, int s, int32_t* out) {
   const level_state& L = static_cast<tracker*>(t)->levels[s];
-  std::memcpy(out, L.sparse.px.data(), sizeof(int32_t) * L.sparse.px.size());
+  copy_bytes(out, L.sparse.px.data(), sizeof(int32_t) * L.sparse.px.size());
 }
 
 void orc_tracker_get_saliency(void* t, int s, uint8_t* out) {
   const level_state& L = static_cast<tracker*>(t)->levels[s];
   if (L.sal.px.empty()) std::memset(out, 0, (size_t)L.nr * L.nc);
-  else std::memcpy(out, L.sal.px.data(), L.sal.px.size());
+  else copy_bytes(out, L.sal.px.data(), L.sal.px.size());
 }
 
 void orc_tracker_get_level(void* t, int s, uint8_t* out) {
   const level_state& L = static_cast<tracker*>(t)->levels[s];
-  std::memcpy(out, L.in_u8.px.data(), L.in_u8.px.size());
+  copy_bytes(out, L.in_u8.px.data(), L.in_u8.px.size());
 }
 
 void orc_tracker_get_matcher_images(void* t, int s, float* A, float* B, float* G2) {
   const level_state& L = static_cast<tracker*>(t)->levels[s];
-  if (A) std::memcpy(A, L.A.px.data(), sizeof(float) * L.A.px.size());
-  if (B) std::memcpy(B, L.B.px.data(), sizeof(float) * L.B.px.size());
-  if (G2) std::memcpy(G2, L.G.px.data(), sizeof(f2) * L.G.px.size());
+  if (A) copy_bytes(A, L.A.px.data(), sizeof(float) * L.A.px.size());
+  if (B) copy_bytes(B, L.B.px.data(), sizeof(float) * L.B.px.size());
+  if (G2) copy_bytes(G2, L.G.px.data(), sizeof(f2) * L.G.px.size());
 }
 
 static int copy_decisions(const std::vector<lk_decision>& d, int32_t* accept, float* tr2,
@@ -226,9 +231,9 @@ int orc_tracker_lk_decide(void* t, int s, int32_t* accept, float* tr2, float* di
 
 void orc_particle_mask(const orc_particle* parts, int n, int nr, int nc, uint8_t* mask) {
   std::vector<particle> v(n);
-  if (n) std::memcpy(v.data(), parts, sizeof(particle) * n);
+  if (n) copy_bytes(v.data(), parts, sizeof(particle) * n);
   const image<uint8_t> m = particle_mask(v, nr, nc);
-  std::memcpy(mask, m.px.data(), m.px.size());
+  copy_bytes(mask, m.px.data(), m.px.size());
 }
 
 int orc_fast_select(const uint8_t* sal, const int32_t* sparse, int nr, int nc, int box, float th, int32_t* pts,
@@ -247,33 +252,34 @@ int orc_max_threads() { return omp_get_max_threads(); }
 int orc_particles_add(orc_particle* parts, int n, int cap, int32_t* sparse, int nr, int nc, const int32_t* pts,
                       int npts) {
   std::vector<particle> v(n);
-  if (n) std::memcpy(v.data(), parts, sizeof(particle) * n);
+  if (n) copy_bytes(v.data(), parts, sizeof(particle) * n);
   image<int32_t> sp = wrap(sparse, nr, nc);
   std::vector<std::pair<int, int>> q;
   for (int k = 0; k < npts && n + k < cap; k++) q.emplace_back(pts[2 * k], pts[2 * k + 1]);
   container_add(v, sp, q);
-  if (!v.empty()) std::memcpy(parts, v.data(), sizeof(particle) * v.size());
-  std::memcpy(sparse, sp.px.data(), sizeof(int32_t) * sp.px.size());
+  if (!v.empty()) copy_bytes(parts, v.data(), sizeof(particle) * v.size());
+  copy_bytes(sparse, sp.px.data(), sizeof(int32_t) * sp.px.size());
   return (int)v.size();
 }
 
 int orc_particles_compact(orc_particle* parts, int n, int32_t* sparse, int nr, int nc, uint32_t* matches) {
   std::vector<particle> v(n);
-  if (n) std::memcpy(v.data(), parts, sizeof(particle) * n);
+  if (n) copy_bytes(v.data(), parts, sizeof(particle) * n);
   image<int32_t> sp = wrap(sparse, nr, nc);
   std::vector<uint32_t> m;
   container_compact(v, sp, m);
-  if (!v.empty()) std::memcpy(parts, v.data(), sizeof(particle) * v.size());
-  if (n) std::memcpy(matches, m.data(), sizeof(uint32_t) * n);
-  std::memcpy(sparse, sp.px.data(), sizeof(int32_t) * sp.px.size());
+  if (!v.empty()) copy_bytes(parts, v.data(), sizeof(particle) * v.size());
+  if (n) copy_bytes(matches, m.data(), sizeof(uint32_t) * n);
+  copy_bytes(sparse, sp.px.data(), sizeof(int32_t) * sp.px.size());
   return (int)v.size();
 }
 
 void orc_merge(orc_particle* parts, int n, const int32_t* sparse, int nr, int nc) {
+  if (n <= 0) return;
   std::vector<particle> v(n);
-  std::memcpy(v.data(), parts, sizeof(particle) * n);
+  copy_bytes(v.data(), parts, sizeof(particle) * n);
   merge_state(v, wrap(sparse, nr, nc));
-  std::memcpy(parts, v.data(), sizeof(particle) * n);
+  copy_bytes(parts, v.data(), sizeof(particle) * n);
 }
 
 double orc_lk_loss_grad_f64_d(const double* I0, const double* I1, int nr, int nc,
@@ -281,7 +287,7 @@ double orc_lk_loss_grad_f64_d(const double* I0, const double* I1, int nr, int nc
                               int np, int ws, double* dL_dd2, double* dL_dI1) {
   image<double> a = wrap(I0, nr, nc), b = wrap(I1, nr, nc), g;
   double L = lk_loss_grad_f64(a, b, coef, K, pos2, d2, np, ws, dL_dd2, dL_dI1 ? &g : nullptr);
-  if (dL_dI1) std::memcpy(dL_dI1, g.px.data(), sizeof(double) * g.px.size());
+  if (dL_dI1) copy_bytes(dL_dI1, g.px.data(), sizeof(double) * g.px.size());
   return L;
 }
 
