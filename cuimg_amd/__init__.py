@@ -64,13 +64,17 @@ class ScaleParams(ctypes.Structure):
 def profile_params(profile, nscales, fast_th=20.0, **kw):
     """'R': the reference matcher defaults on u8 frames (11x11 window, <=30
     iterations, 25 strongest-gradient pixels); 'B': 7x7 window, <=5 iterations,
-    all pixels, on sigma=1 / 7-tap blurred FP32 frames."""
+    all pixels, on sigma=1 / 7-tap blurred FP32 frames; 'C': the bc2s
+    descriptor with the gradient-descent matcher (generic_strategy, the
+    reference sample's bc2s_fast_gradient_cpu; the LK settings are unused)."""
     p = TrackerParams()
     p.nscales = nscales
     if profile == "R":
         p.profile, p.lk_ws, p.lk_max_iters, p.lk_top_n = 0, 11, 30, 25
     elif profile == "B":
         p.profile, p.lk_ws, p.lk_max_iters, p.lk_top_n = 1, 7, 5, 49
+    elif profile == "C":
+        p.profile, p.lk_ws, p.lk_max_iters, p.lk_top_n = 2, 7, 5, 49
     else:
         raise ValueError(profile)
     p.lk_eps, p.fast_n, p.fast_th, p.box_size = 0.1, 9, fast_th, 3
@@ -158,6 +162,14 @@ def lib():
         L.cuimg_hip_lk_track.argtypes = [vp, P(Img), P(Img), c_int, c_int, vp, ctypes.c_uint32, P(LkParams),
                                          P(Img), vp, vp, vp]
         L.cuimg_hip_lk_track_noalloc.argtypes = L.cuimg_hip_lk_track.argtypes
+        L.cuimg_hip_tracker_get_features.argtypes = [vp, c_int, vp, ctypes.c_uint32, P(ctypes.c_uint32)]
+        L.cuimg_hip_tracker_bc2s_images.argtypes = [vp, c_int, P(Img), P(Img), P(Img)]
+        L.cuimg_hip_tracker_flow.argtypes = [vp, c_int, P(Img), P(Img)]
+        L.cuimg_hip_bc2s_images.argtypes = [vp, P(Img), P(Img), P(Img), P(Img)]
+        L.cuimg_hip_blur121.argtypes = [vp, P(Img), P(Img)]
+        L.cuimg_hip_gaussian_int.argtypes = [vp, P(Img), P(Img), ctypes.c_float, c_int]
+        L.cuimg_hip_bc2s_features.argtypes = [vp, P(Img), P(Img), vp, ctypes.c_uint32, vp]
+        L.cuimg_hip_bc2s_match.argtypes = [vp, P(Img), P(Img), vp, vp, ctypes.c_uint32, c_int, vp, vp]
         L.cuimg_hip_batch_create.argtypes = [vp, c_int, P(c_int), P(c_int), P(TrackerParams), P(vp)]
         L.cuimg_hip_batch_destroy.argtypes = [vp]
         L.cuimg_hip_batch_step.argtypes = [vp, P(Img)]
@@ -288,6 +300,65 @@ class Context:
 
     def fill_border(self, img):
         self.check(self.L.cuimg_hip_fill_border_clamp(self.h, ctypes.byref(img.d)), "fill_border_clamp")
+
+    # ---------------------------------------------------------------- bc2s
+    def bc2s_images(self, src, s1, s2, contrast=None):
+        """s1 = blur121(src), s2 = blur121(s1) (borders clamp-filled), contrast
+        (U32, U8 saturated, or None)."""
+        c = ctypes.byref(contrast.d) if contrast is not None else None
+        self.check(self.L.cuimg_hip_bc2s_images(self.h, ctypes.byref(src.d), ctypes.byref(s1.d), ctypes.byref(s2.d),
+                                                c), "bc2s_images")
+
+    def blur121(self, src, dst):
+        self.check(self.L.cuimg_hip_blur121(self.h, ctypes.byref(src.d), ctypes.byref(dst.d)), "blur121")
+
+    def gaussian_int(self, src, dst, sigma, hw):
+        self.check(self.L.cuimg_hip_gaussian_int(self.h, ctypes.byref(src.d), ctypes.byref(dst.d),
+                                                 ctypes.c_float(sigma), int(hw)), "gaussian_int")
+
+    def _dev(self, arr):
+        arr = np.ascontiguousarray(arr)
+        p = ctypes.c_void_p()
+        self.check(self.L.cuimg_hip_malloc(self.h, max(arr.nbytes, 16), ctypes.byref(p)), "malloc")
+        if arr.nbytes:
+            self.check(self.L.cuimg_hip_memcpy_h2d(self.h, p, arr.ctypes.data, arr.nbytes), "memcpy_h2d")
+        return p
+
+    def _host(self, p, shape, dtype):
+        out = np.zeros(shape, dtype)
+        if out.nbytes:
+            self.check(self.L.cuimg_hip_memcpy_d2h(self.h, out.ctypes.data, p, out.nbytes), "memcpy_d2h")
+        return out
+
+    def bc2s_features(self, s1, s2, pts):
+        """Descriptors (n x 16 u8) at (row, col) points of the feature images."""
+        pts = np.ascontiguousarray(pts, np.int32).reshape(-1, 2)
+        n = len(pts)
+        dp, do = self._dev(pts), self._dev(np.zeros((n, 16), np.uint8))
+        try:
+            self.check(self.L.cuimg_hip_bc2s_features(self.h, ctypes.byref(s1.d), ctypes.byref(s2.d), dp, n, do),
+                       "bc2s_features")
+            return self._host(do, (n, 16), np.uint8)
+        finally:
+            self.L.cuimg_hip_free(self.h, dp)
+            self.L.cuimg_hip_free(self.h, do)
+
+    def bc2s_match(self, s1, s2, feats, preds, scale=0):
+        """Two-step descent (scale 0) or one descent at scale 1 / 2: (n x 2
+        int32 displacements, n distances)."""
+        feats = np.ascontiguousarray(feats, np.uint8).reshape(-1, 16)
+        preds = np.ascontiguousarray(preds, np.int32).reshape(-1, 2)
+        n = len(preds)
+        assert len(feats) == n
+        df, dp = self._dev(feats), self._dev(preds)
+        dd, ds = self._dev(np.zeros((n, 2), np.int32)), self._dev(np.zeros(n, np.float32))
+        try:
+            self.check(self.L.cuimg_hip_bc2s_match(self.h, ctypes.byref(s1.d), ctypes.byref(s2.d), df, dp, n, scale,
+                                                   dd, ds), "bc2s_match")
+            return self._host(dd, (n, 2), np.int32), self._host(ds, n, np.float32)
+        finally:
+            for q in (df, dp, dd, ds):
+                self.L.cuimg_hip_free(self.h, q)
 
     def fast(self, src, mask, dst, n=9, th=20):
         m = ctypes.byref(mask.d) if mask is not None else None
@@ -518,6 +589,29 @@ class Tracker:
                                                                   dist.ctypes.data, n.value, ctypes.byref(n)),
                            "get_decisions")
         return acc, tr, dist
+
+    def features(self, s):
+        """bc2s descriptors of the dense vector (n x 16 u8), profile 'C'."""
+        n = ctypes.c_uint32()
+        self.ctx.check(self.L.cuimg_hip_tracker_get_features(self.h, s, None, 0, ctypes.byref(n)), "get_features")
+        out = np.zeros((n.value, 16), np.uint8)
+        if n.value:
+            self.ctx.check(self.L.cuimg_hip_tracker_get_features(self.h, s, out.ctypes.data, n.value,
+                                                                 ctypes.byref(n)), "get_features")
+        return out
+
+    def bc2s_images(self, s):
+        """(s1, s2, contrast saturated at 255) of the scale's last frame."""
+        a, b, c = Img(), Img(), Img()
+        self.ctx.check(self.L.cuimg_hip_tracker_bc2s_images(self.h, s, ctypes.byref(a), ctypes.byref(b),
+                                                            ctypes.byref(c)), "bc2s_images")
+        return View(self.ctx, a).download(), View(self.ctx, b).download(), View(self.ctx, c).download()
+
+    def flow(self, s):
+        """(flow (cells x 2 float32, -500 = none), multiscale count) of the last frame."""
+        a, b = Img(), Img()
+        self.ctx.check(self.L.cuimg_hip_tracker_flow(self.h, s, ctypes.byref(a), ctypes.byref(b)), "flow")
+        return View(self.ctx, a).download(), View(self.ctx, b).download()
 
     def work(self, s):
         """(particles run through the matcher, accepted) at scale s since creation."""

@@ -65,11 +65,13 @@ __device__ __forceinline__ uint8_t fast_score_lds(const uint8_t* t, int stride, 
   return (uint8_t)(sb > sd ? sb : sd);
 }
 
-// One (16 x 64)-pixel tile per 256-thread workgroup.
+// One (16 x 64)-pixel tile per 256-thread workgroup.  A pixel is scored where
+// its mask byte is at least mask_min (a 0 / 255 detector mask: 1; the bc2s
+// strategy's contrast, saturated to 8 bits, as its mask: 5).
 template <bool CLAMP>
 __device__ inline void fast_tile_body(const uint8_t* in, int in_pitch, const uint8_t* mask, int mask_pitch,
                                       uint8_t* out, int out_pitch, int nr, int nc, int tile, int tiles_x, int n,
-                                      int th) {
+                                      int th, int mask_min = 1) {
   constexpr int SW = kFTW + 2 * kFH, SH = kFTH + 2 * kFH;
   __shared__ uint8_t t[SH * SW];
   const int r0 = (tile / tiles_x) * kFTH, c0 = (tile % tiles_x) * kFTW;
@@ -81,7 +83,7 @@ __device__ inline void fast_tile_body(const uint8_t* in, int in_pitch, const uin
 #pragma unroll
   for (int k = 0; k < kFTH / 4; k++) {
     const int r = r0 + lr0 + 4 * k;
-    mk[k] = (!mask || r >= nr || c >= nc) ? 1 : *px<uint8_t>(mask, mask_pitch, r, c);
+    mk[k] = (!mask || r >= nr || c >= nc) ? 255 : *px<uint8_t>(mask, mask_pitch, r, c);
   }
   // the tile with its halo: every load of the thread issued before the first
   // LDS write (one memory round trip)
@@ -114,7 +116,7 @@ __device__ inline void fast_tile_body(const uint8_t* in, int in_pitch, const uin
     const int lr = lr0 + 4 * k, r = r0 + lr;
     if (r >= nr) break;
     uint8_t s = 0;
-    if (mk[k] != 0) s = fast_score_lds(&t[(lr + kFH) * SW + lc + kFH], SW, n, th);
+    if (mk[k] >= mask_min) s = fast_score_lds(&t[(lr + kFH) * SW + lc + kFH], SW, n, th);
     *px<uint8_t>(out, out_pitch, r, c) = s;
   }
 }

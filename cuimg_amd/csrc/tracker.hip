@@ -27,6 +27,7 @@
 // detector's mask and saliency, the u32 sparse index image (border 7), two
 // dense particle vectors (compaction ping-pongs between them), the matches
 // vector, and per-particle scratch for compaction and the merge.
+#include "bc2s_dev.hpp"
 #include "fast_dev.hpp"
 #include "gaussian_dev.hpp"
 #include "lk_dev.hpp"
@@ -58,11 +59,20 @@ struct level_res {
   uint32_t cap = 0;
   int nblocks = 0, nbr = 0, nbc = 0;
   cuimg_scale_params sp{};
+  // bc2s strategy: contrast (saturated u8), feature images, descriptors of
+  // both particle vectors, per-cell statistics / flow / multiscale count
+  cuimg_img con{}, s1{}, s2{};
+  uint4* feat[2] = {nullptr, nullptr};
+  int cr = 0, cc = 0;
+  uint4* stats = nullptr;
+  float2* flow = nullptr;
+  uint32_t* msc = nullptr;
 };
 
 struct job_params {
   int det_freq, filt_freq, with_merge, fast_n;
   float fast_th;
+  int bc2s;  // generic_strategy with the bc2s descriptor (else pyrlk)
 };
 
 // Sobel/8 gradient of a blurred level: rows of `in` into `out`.
@@ -94,6 +104,7 @@ struct level_job {
   merge_args mg[2];     // [current vector after the detector]
   gauss_job blur[2];    // [slot]
   grad_job grad[2];     // [slot]
+  bc2s_job bc;          // bc2s strategy
 };
 
 struct stream_job {
@@ -107,7 +118,10 @@ struct frame_in {
   int vec_ok;
 };
 
-enum { K_PYR, K_BLUR, K_GRAD, K_FILL, K_LK, K_PMASK, K_FAST, K_DETC, K_SCATI, K_MSCAN, K_MR1, K_NKINDS };
+enum {
+  K_PYR, K_BLUR, K_GRAD, K_FILL, K_LK, K_PMASK, K_FAST, K_DETC, K_SCATI, K_MSCAN, K_MR1,
+  K_BPREP, K_BMATCH, K_FSTAT, K_FUSE, K_FILT, K_NKINDS
+};
 constexpr int kRing = 8;
 
 }  // namespace cuimg
@@ -190,17 +204,21 @@ __device__ __forceinline__ job_block job_of_block(const uint32_t* table, int njo
 
 struct step_flags {
   uint32_t frame, cur, slot;
-  bool match, detect, merge;
+  bool match, detect, filter, merge;
 };
 
+// pyrlk_cpu::update matches from the second frame on; generic_strategy::update
+// matches (and resets the sparse image) at every frame, and filters on the
+// filtering frames.
 __device__ __forceinline__ step_flags flags_of(const level_job& J) {
   step_flags f;
   const job_params prm = uni(J.prm);
   f.frame = uni(J.cnt->frame);
   f.cur = uni(J.cnt->cur) & 1u;
   f.slot = uni(J.cnt0->steps) & 1u;
-  f.match = f.frame > 0;
+  f.match = prm.bc2s || f.frame > 0;
   f.detect = f.frame % (uint32_t)prm.det_freq == 0;
+  f.filter = prm.bc2s && f.frame % (uint32_t)prm.filt_freq == 0;
   f.merge = prm.with_merge && f.frame % (uint32_t)prm.filt_freq == 0;
   return f;
 }
@@ -287,12 +305,76 @@ __global__ __launch_bounds__(256) void step_pmask_kernel(const level_job* jobs, 
   particle_mask_core(d.parts, d.cnt->n, J.sal, J.sal_pitch, d.nr, d.nc, lb, nb);
 }
 
+// The bc2s strategy's detector mask also holds 0 where the contrast is below
+// 5 (flow_mask_kernel, tracking_strategies.hpp:228-246): the contrast image is
+// the FAST mask with threshold 5.
 __global__ __launch_bounds__(256) void step_fast_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
   CUIMG_JOB(prefix, njobs);
   const step_flags f = flags_of(J);
   if (!f.detect) return;
-  fast_tile_body<false>(J.u8[f.slot], J.u8_pitch, nullptr, 0, J.sal, J.sal_pitch, J.nr, J.nc, lb,
-                        J.fast_tiles_x, J.prm.fast_n, (int)J.prm.fast_th);
+  const bool bc = J.prm.bc2s;
+  fast_tile_body<false>(J.u8[f.slot], J.u8_pitch, bc ? J.bc.con : nullptr, bc ? J.bc.con_pitch : 0, J.sal,
+                        J.sal_pitch, J.nr, J.nc, lb, J.fast_tiles_x, J.prm.fast_n, (int)J.prm.fast_th, bc ? 5 : 1);
+}
+
+// bc2s strategy: contrast and feature images of the new frame.
+__global__ __launch_bounds__(256) void step_bprep_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
+  CUIMG_JOB(prefix, njobs);
+  bc2s_prep_tile<false>(J.bc.prep[J.cnt0->steps & 1u], lb);
+}
+
+// bc2s strategy: match_particles_kernel over the current vector.
+__global__ __launch_bounds__(256) void step_bmatch_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
+  CUIMG_JOB(prefix, njobs);
+  const step_flags f = flags_of(J);
+  if (!f.match) return;
+  __shared__ uint32_t blk_live, blk_acc;
+  if (threadIdx.x == 0) {
+    blk_live = 0;
+    blk_acc = 0;
+  }
+  __syncthreads();
+  const detect_args& d = J.det[f.cur];
+  const uint32_t n = J.cnt->n;
+  if (lb == 0 && threadIdx.x == 0) J.cnt->n_matched = n;
+  const uint32_t first = (uint32_t)lb * blockDim.x + threadIdx.x, stride = (uint32_t)nb * blockDim.x;
+  uint32_t live = 0, acc = 0;
+  bc2s_match_body(J.bc, d.parts, J.bc.feat[f.cur], n, J.sparse, J.sparse_pitch, J.bc.dec_accept, J.bc.dec_tr,
+                  J.bc.dec_dist, first, stride, &live, &acc);
+  if (live) atomicAdd(&blk_live, live);
+  if (acc) atomicAdd(&blk_acc, acc);
+  __syncthreads();
+  if (threadIdx.x == 0 && (blk_live | blk_acc)) {
+    atomicAdd(&J.cnt->live_total, (unsigned long long)blk_live);
+    atomicAdd(&J.cnt->accepted_total, (unsigned long long)blk_acc);
+  }
+}
+
+// bc2s strategy: per-cell flow statistics of the matched particles.
+__global__ __launch_bounds__(256) void step_fstat_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
+  CUIMG_JOB(prefix, njobs);
+  const step_flags f = flags_of(J);
+  bc2s_stats_body(J.bc, J.det[f.cur].parts, J.sparse, J.sparse_pitch, (uint32_t)lb * blockDim.x + threadIdx.x,
+                  (uint32_t)nb * blockDim.x);
+}
+
+// bc2s strategy: the statistics fused over the scales into flow and
+// multiscale count (the stream's scales are consecutive jobs).
+__global__ __launch_bounds__(256) void step_fuse_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
+  CUIMG_JOB(prefix, njobs);
+  const int j0 = j - J.s;
+  bc2s_fuse_body(J.bc, J.s, [&](int t) -> const bc2s_job& { return jobs[j0 + t].bc; },
+                 (uint32_t)lb * blockDim.x + threadIdx.x, (uint32_t)nb * blockDim.x);
+}
+
+// bc2s strategy: filter_bad_particles_kernel over the vector after the detector.
+__global__ __launch_bounds__(256) void step_filter_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
+  CUIMG_JOB(prefix, njobs);
+  const step_flags f = flags_of(J);
+  if (!f.filter) return;
+  const uint32_t c = f.cur ^ (f.detect ? 1u : 0u);
+  bc2s_filter_body(J.bc, J.det[c].parts, J.cnt->n, (uint32_t)lb * blockDim.x + threadIdx.x,
+                   (uint32_t)nb * blockDim.x);
 }
 
 // Box selection with sub-row counts, and the compaction counts.
@@ -507,8 +589,8 @@ static int check_params(cuimg_hip_ctx* ctx, int nrows, int ncols, const cuimg_tr
   if (P->lk_top_n < 1 || P->lk_max_iters < 0)
     return set_error(ctx, CUIMG_E_INVALID, "tracker: LK top_n %d (>= 1) / max_iters %d (>= 0)", P->lk_top_n,
                      P->lk_max_iters);
-  if (P->profile != 0 && P->profile != 1)
-    return set_error(ctx, CUIMG_E_INVALID, "tracker: profile %d not in {0,1}", P->profile);
+  if (P->profile != 0 && P->profile != 1 && P->profile != 2)
+    return set_error(ctx, CUIMG_E_INVALID, "tracker: profile %d not in {0,1,2}", P->profile);
   if (P->profile == 1 && (P->blur_K < 1 || P->blur_K > kMaxK || P->blur_sigma < 1))
     return set_error(ctx, CUIMG_E_UNSUPPORTED, "tracker: blur sigma %d / half width %d", P->blur_sigma, P->blur_K);
   int r = nrows, c = ncols;
@@ -560,6 +642,20 @@ static int alloc_level(cuimg_hip_batch* b, const cuimg_tracker_params* P, int r,
   CUIMG_TRY(dev_alloc_n(b, R.cap, &R.dec_accept));
   CUIMG_TRY(dev_alloc_n(b, R.cap, &R.dec_tr));
   CUIMG_TRY(dev_alloc_n(b, R.cap, &R.dec_dist));
+  if (P->profile == 2) {
+    // generic_strategy: contrast_ (saturated), feature_ (s1 border 3, s2
+    // border 6), descriptors, and the flow cells (domain_div_up(d, 8))
+    CUIMG_TRY(image_alloc(ctx, r, c, 0, CUIMG_U8, &R.con));
+    CUIMG_TRY(image_alloc(ctx, r, c, 3, CUIMG_U8, &R.s1));
+    CUIMG_TRY(image_alloc(ctx, r, c, 6, CUIMG_U8, &R.s2));
+    for (int k = 0; k < 2; k++) CUIMG_TRY(dev_alloc_n(b, R.cap, &R.feat[k]));
+    R.cr = ceil_div(r, kCell);
+    R.cc = ceil_div(c, kCell);
+    const size_t ncell = (size_t)R.cr * R.cc;
+    CUIMG_TRY(dev_alloc_n(b, ncell, &R.stats));
+    CUIMG_TRY(dev_alloc_n(b, ncell, &R.flow));
+    CUIMG_TRY(dev_alloc_n(b, ncell, &R.msc));
+  }
   void* m = nullptr;
   CUIMG_TRY(dev_alloc(b, merge_scratch_bytes(R.cap), &m));
   R.merge = merge_scratch_carve(m, R.cap);
@@ -587,6 +683,8 @@ static int blur_th(const cuimg_hip_batch* b) {
   return (int)std::max<long>(4, std::min<long>(128, (strips + 4095) / 4096));
 }
 
+static void build_bc2s(cuimg_hip_batch* b, int i, int s, level_job& J);
+
 // Job descriptors of job j from its resources and settings.
 static void build_job(cuimg_hip_batch* b, int i, int s) {
   const int j = b->first[i] + s;
@@ -598,7 +696,8 @@ static void build_job(cuimg_hip_batch* b, int i, int s) {
   J.s = s;
   J.nr = R.nr;
   J.nc = R.nc;
-  J.prm = job_params{R.sp.detector_frequency, R.sp.filtering_frequency, R.sp.with_merge, R.sp.fast_n, R.sp.fast_th};
+  J.prm = job_params{R.sp.detector_frequency, R.sp.filtering_frequency, R.sp.with_merge, R.sp.fast_n, R.sp.fast_th,
+                     P.profile == 2 ? 1 : 0};
   J.cnt = b->cnt + j;
   J.cnt0 = b->cnt + b->first[i];
   for (int k = 0; k < 2; k++) J.u8[k] = static_cast<uint8_t*>(R.u8[k].begin);
@@ -653,6 +752,10 @@ static void build_job(cuimg_hip_batch* b, int i, int s) {
     d.cnt = J.cnt;
     d.cap = R.cap;
     d.nblocks = R.nblocks;
+    d.feat = R.feat[cur];
+    d.feat_out = R.feat[cur ^ 1];
+    d.F = bc2s_view{static_cast<const uint8_t*>(R.s1.begin), static_cast<const uint8_t*>(R.s2.begin), R.s1.pitch,
+                    R.s2.pitch, R.nr, R.nc};
     merge_args& m = J.mg[cur];
     m = R.merge;
     m.nr = R.nr;
@@ -685,6 +788,48 @@ static void build_job(cuimg_hip_batch* b, int i, int s) {
     g.vec_ok = (R.u8[slot].pitch % 4 == 0) && ((uintptr_t)R.u8[slot].begin % 4 == 0) &&
                (R.blur[slot].pitch % 16 == 0) && ((uintptr_t)R.blur[slot].begin % 16 == 0);
   }
+  build_bc2s(b, i, s, J);
+}
+
+// The bc2s strategy's job of scale s (profile 2).
+static void build_bc2s(cuimg_hip_batch* b, int i, int s, level_job& J) {
+  const level_res& R = b->lv[b->first[i] + s];
+  bc2s_job& B = J.bc;
+  B = bc2s_job{};
+  if (b->P[i].profile != 2) return;
+  for (int slot = 0; slot < 2; slot++) {
+    bc2s_prep_args& a = B.prep[slot];
+    a.in = static_cast<const uint8_t*>(R.u8[slot].begin);
+    a.in_pitch = R.u8[slot].pitch;
+    a.con = static_cast<uint8_t*>(R.con.begin);
+    a.con_pitch = R.con.pitch;
+    a.con32 = nullptr;
+    a.con32_pitch = 0;
+    a.s1 = static_cast<uint8_t*>(R.s1.begin);
+    a.s1_pitch = R.s1.pitch;
+    a.s1_border = R.s1.border;
+    a.s2 = static_cast<uint8_t*>(R.s2.begin);
+    a.s2_pitch = R.s2.pitch;
+    a.s2_border = R.s2.border;
+    a.nr = R.nr;
+    a.nc = R.nc;
+    a.tiles_x = ceil_div(R.nc, kB2W);
+  }
+  B.F = J.det[0].F;
+  B.con = static_cast<const uint8_t*>(R.con.begin);
+  B.con_pitch = R.con.pitch;
+  B.feat[0] = R.feat[0];
+  B.feat[1] = R.feat[1];
+  B.cr = R.cr;
+  B.cc = R.cc;
+  B.stats = R.stats;
+  B.flow = R.flow;
+  B.msc = R.msc;
+  B.k = R.sp.k;
+  B.last = b->P[i].nscales - 1;
+  B.dec_accept = R.dec_accept;
+  B.dec_tr = R.dec_tr;
+  B.dec_dist = R.dec_dist;
 }
 
 static void build_stream(cuimg_hip_batch* b, int i) {
@@ -728,6 +873,13 @@ static int build_maps(cuimg_hip_batch* b) {
     cnt[K_SCATI].push_back((uint32_t)(R.nblocks + ceil_div(R.nbr * R.sp.box_size, 4)));
     cnt[K_MSCAN].push_back((uint32_t)std::max<long>(1, std::min<long>(512, (cap + 255) / 256)));
     cnt[K_MR1].push_back(cnt[K_MSCAN].back());
+    const bool bc = b->P0.profile == 2;
+    const long cells = (long)R.cr * R.cc;
+    cnt[K_BPREP].push_back(bc ? (uint32_t)(ceil_div(R.nc, kB2W) * ceil_div(R.nr, kB2H)) : 0u);
+    cnt[K_BMATCH].push_back(bc ? (uint32_t)std::max<long>(1, std::min<long>(1024, (cap + 255) / 256)) : 0u);
+    cnt[K_FSTAT].push_back(bc ? (uint32_t)std::max<long>(1, (cells + 255) / 256) : 0u);
+    cnt[K_FUSE].push_back(cnt[K_FSTAT].back());
+    cnt[K_FILT].push_back(bc ? (uint32_t)std::max<long>(1, std::min<long>(512, (cap + 255) / 256)) : 0u);
   }
   // Matcher: one workgroup per kLkWaves * ppw particles of capacity, at most
   // kLkMaxWgs / 4^s for scale s (the workgroups loop over the chunks, see
@@ -736,8 +888,9 @@ static int build_maps(cuimg_hip_batch* b) {
   // each of which still sets up before it finds its chunks (1080p: 221 ->
   // 203 us per launch against 1024 workgroups for every scale).
   for (int j = 0; j < b->njobs; j++)
-    cnt[K_LK].push_back((uint32_t)std::min(ceil_div((int)b->lv[j].cap, kLkWaves * b->lk_ppw),
-                                           std::max(16, kLkMaxWgs >> (2 * b->hj[j].s))));
+    cnt[K_LK].push_back(b->P0.profile == 2 ? 0u
+                                           : (uint32_t)std::min(ceil_div((int)b->lv[j].cap, kLkWaves * b->lk_ppw),
+                                                                std::max(16, kLkMaxWgs >> (2 * b->hj[j].s))));
   for (int k = 0; k < K_NKINDS; k++) {
     // prefix of block counts, then the block -> (job, block in job) map
     std::vector<uint32_t>& p = b->prefix_h[k];
@@ -879,6 +1032,39 @@ static int enqueue_pyramid(cuimg_hip_batch* b) {
 // with_pyramid: the step builds the pyramid of the frames in the input table
 // first (tracker::run(in)); otherwise it steps on the pyramid already in the
 // frame slot (tracker::run()).
+// generic_strategy::update of every scale (bc2s descriptor): feature images,
+// FAST beside the matcher, flow statistics and fusion, detector, filter, merge.
+static int enqueue_step_bc2s(cuimg_hip_batch* b, bool capturing) {
+  cuimg_hip_ctx* ctx = b->ctx;
+  hipStream_t st = ctx->stream;
+  const int nj = b->njobs;
+  const level_job* J = b->dj;
+  uint32_t* const* pf = b->prefix_d;
+  CUIMG_TRY(launch(ctx, "bc2s_images", [&] { step_bprep_kernel<<<b->total[K_BPREP], 256, 0, st>>>(J, pf[K_BPREP], nj); }));
+  const bool fork = capturing;
+  hipStream_t fst = fork ? b->aux : st;
+  if (fork) {
+    CUIMG_HIP(ctx, hipEventRecord(b->fork_ev, st));
+    CUIMG_HIP(ctx, hipStreamWaitEvent(b->aux, b->fork_ev, 0));
+  }
+  CUIMG_TRY(launch(ctx, "fast_saliency", [&] { step_fast_kernel<<<b->total[K_FAST], 256, 0, fst>>>(J, pf[K_FAST], nj); }));
+  if (fork) CUIMG_HIP(ctx, hipEventRecord(b->join_ev, b->aux));
+  CUIMG_TRY(launch(ctx, "fills", [&] { step_fill_kernel<<<b->total[K_FILL], 256, 0, st>>>(J, pf[K_FILL], nj); }));
+  CUIMG_TRY(launch(ctx, "bc2s_match", [&] { step_bmatch_kernel<<<b->total[K_BMATCH], 256, 0, st>>>(J, pf[K_BMATCH], nj); }));
+  CUIMG_TRY(launch(ctx, "flow_stats", [&] { step_fstat_kernel<<<b->total[K_FSTAT], 256, 0, st>>>(J, pf[K_FSTAT], nj); }));
+  CUIMG_TRY(launch(ctx, "flow_fusion", [&] { step_fuse_kernel<<<b->total[K_FUSE], 256, 0, st>>>(J, pf[K_FUSE], nj); }));
+  if (fork) CUIMG_HIP(ctx, hipStreamWaitEvent(st, b->join_ev, 0));
+  CUIMG_TRY(launch(ctx, "particle_mask",
+                   [&] { step_pmask_kernel<<<b->total[K_PMASK], 256, 0, st>>>(J, pf[K_PMASK], nj); }));
+  CUIMG_TRY(launch(ctx, "detect_count", [&] { step_detc_kernel<<<b->total[K_DETC], 256, 0, st>>>(J, pf[K_DETC], nj); }));
+  CUIMG_TRY(launch(ctx, "scatter_insert",
+                   [&] { step_scati_kernel<<<b->total[K_SCATI], 256, 0, st>>>(J, pf[K_SCATI], nj); }));
+  CUIMG_TRY(launch(ctx, "filter", [&] { step_filter_kernel<<<b->total[K_FILT], 256, 0, st>>>(J, pf[K_FILT], nj); }));
+  CUIMG_TRY(launch(ctx, "merge_scan", [&] { step_mscan_kernel<<<b->total[K_MSCAN], 256, 0, st>>>(J, pf[K_MSCAN], nj); }));
+  CUIMG_TRY(launch(ctx, "merge_round1", [&] { step_mr1_kernel<<<b->total[K_MR1], 256, 0, st>>>(J, pf[K_MR1], nj); }));
+  return launch(ctx, "merge_resolve", [&] { step_mres_kernel<<<nj, 1024, 0, st>>>(J); });
+}
+
 static int enqueue_step(cuimg_hip_batch* b, bool capturing, bool with_pyramid = true) {
   cuimg_hip_ctx* ctx = b->ctx;
   hipStream_t st = ctx->stream;
@@ -886,6 +1072,7 @@ static int enqueue_step(cuimg_hip_batch* b, bool capturing, bool with_pyramid = 
   const level_job* J = b->dj;
   uint32_t* const* pf = b->prefix_d;
   if (with_pyramid) CUIMG_TRY(enqueue_pyramid(b));
+  if (b->P0.profile == 2) return enqueue_step_bc2s(b, capturing);
   // FAST needs only the pyramid: in a captured step it runs beside the matcher
   // on the second stream; plain launches (and per-launch timing on the context
   // stream) keep it in order, where two more stream calls per step would only
@@ -1039,6 +1226,15 @@ static int clear_stream(cuimg_hip_batch* b, int i) {
 }
 
 // ------------------------------------------- standalone detector entry points
+// Descriptors of n particles at their (truncated) positions (bc2s set_particles).
+__global__ __launch_bounds__(256) void features_at_kernel(const cuimg_particle_f* parts, uint32_t n, bc2s_view F,
+                                                          uint4* feat) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int r = (int)parts[i].pos_r, c = (int)parts[i].pos_c;
+  feat[i] = (r >= 0 && r < F.nr && c >= 0 && c < F.nc) ? bc2s_feature_at(F, r, c) : make_uint4(0, 0, 0, 0);
+}
+
 __global__ __launch_bounds__(256) void mask_fill_kernel(uint8_t* m, int pitch, int nr, int nc) {
   fill_u8_body(m, pitch, nr, nc, 0xFFFFFFFFu, blockIdx.x, gridDim.x);
 }
@@ -1364,7 +1560,60 @@ int cuimg_hip_tracker_set_particles(cuimg_hip_tracker* t, int scale, const cuimg
     CUIMG_HIP(b->ctx, hipMemcpyAsync(R.parts[c.cur & 1], in, sizeof(cuimg_particle_f) * n, hipMemcpyHostToDevice,
                                      b->ctx->stream));
   CUIMG_HIP(b->ctx, hipMemcpyAsync(&b->cnt[j].n, &n, sizeof(uint32_t), hipMemcpyHostToDevice, b->ctx->stream));
+  // bc2s strategy: each particle's descriptor at its position in the scale's
+  // current feature images (zero before the first frame)
+  if (b->P0.profile == 2 && n) {
+    level_counters c0;
+    CUIMG_TRY(read_counters(b, b->first[t->stream], &c0));
+    if (c0.steps == 0) {
+      CUIMG_HIP(b->ctx, hipMemsetAsync(R.feat[c.cur & 1], 0, sizeof(uint4) * n, b->ctx->stream));
+    } else {
+      const bc2s_view F = b->hj[j].bc.F;
+      CUIMG_TRY(launch(b->ctx, "features_at", [&] {
+        features_at_kernel<<<ceil_div((int)n, 256), 256, 0, b->ctx->stream>>>(R.parts[c.cur & 1], n, F,
+                                                                               R.feat[c.cur & 1]);
+      }));
+    }
+  }
   CUIMG_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+  return 0;
+}
+
+int cuimg_hip_tracker_get_features(cuimg_hip_tracker* t, int scale, uint8_t* out, uint32_t cap, uint32_t* n) {
+  int j;
+  CUIMG_TRY(job_of(t, scale, &j));
+  if (t->b->P0.profile != 2) return set_error(t->b->ctx, CUIMG_E_INVALID, "get_features: not a bc2s tracker");
+  level_counters c;
+  CUIMG_TRY(read_counters(t->b, j, &c));
+  if (n) *n = c.n;
+  if (!out) return 0;
+  if (c.n > cap)
+    return set_error(t->b->ctx, CUIMG_E_CAPACITY, "get_features: %u particles, buffer holds %u", c.n, cap);
+  CUIMG_HIP(t->b->ctx, hipMemcpyAsync(out, t->b->lv[j].feat[c.cur & 1], 16 * (size_t)c.n, hipMemcpyDeviceToHost,
+                                      t->b->ctx->stream));
+  CUIMG_HIP(t->b->ctx, hipStreamSynchronize(t->b->ctx->stream));
+  return 0;
+}
+
+int cuimg_hip_tracker_bc2s_images(cuimg_hip_tracker* t, int scale, cuimg_img* s1, cuimg_img* s2,
+                                  cuimg_img* contrast) {
+  int j;
+  CUIMG_TRY(job_of(t, scale, &j));
+  if (t->b->P0.profile != 2) return set_error(t->b->ctx, CUIMG_E_INVALID, "bc2s_images: not a bc2s tracker");
+  const level_res& R = t->b->lv[j];
+  if (s1) *s1 = R.s1;
+  if (s2) *s2 = R.s2;
+  if (contrast) *contrast = R.con;
+  return 0;
+}
+
+int cuimg_hip_tracker_flow(cuimg_hip_tracker* t, int scale, cuimg_img* flow, cuimg_img* msc) {
+  int j;
+  CUIMG_TRY(job_of(t, scale, &j));
+  if (t->b->P0.profile != 2) return set_error(t->b->ctx, CUIMG_E_INVALID, "flow: not a bc2s tracker");
+  const level_res& R = t->b->lv[j];
+  if (flow) *flow = cuimg_img{R.flow, R.cr, R.cc, R.cc * 8, 0, CUIMG_F32X2};
+  if (msc) *msc = cuimg_img{R.msc, R.cr, R.cc, R.cc * 4, 0, CUIMG_U32};
   return 0;
 }
 

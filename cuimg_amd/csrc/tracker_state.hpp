@@ -90,6 +90,14 @@ struct lk_args {
   lk_params_dev P;
 };
 
+// The bc2s feature images of one scale (u8, borders of 3 and 6, clamp-filled).
+struct bc2s_view {
+  const uint8_t* s1;
+  const uint8_t* s2;
+  int p1, p2;  // pitches
+  int nr, nc;
+};
+
 struct detect_args {
   // level geometry
   int nr, nc;
@@ -110,6 +118,11 @@ struct detect_args {
   level_counters* cnt;
   uint32_t cap;
   int nblocks;                  // compaction blocks covering cap
+  // bc2s strategy: 16-byte descriptors follow their particles through the
+  // compaction (feat -> feat_out); new particles take theirs from F
+  uint4* feat;                  // nullptr: no descriptors
+  uint4* feat_out;
+  bc2s_view F;
 };
 
 // A particle with at least one eligible merge target: b[k] is the c8

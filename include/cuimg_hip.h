@@ -177,7 +177,10 @@ int cuimg_hip_fast_select(cuimg_hip_ctx* ctx, const cuimg_img* sal, const cuimg_
 typedef struct {
   int32_t nscales;
   int32_t profile;              /* 0: reference matcher on u8 frames (ws 11, 30 it,
-                                   top-25); 1: blurred FP32 frames */
+                                   top-25); 1: blurred FP32 frames; 2: the bc2s
+                                   descriptor with the gradient-descent matcher
+                                   (generic_strategy, tracking_strategies.hpp:28-111;
+                                   the LK and blur settings are unused) */
   int32_t lk_ws;                /* window side (odd) */
   int32_t lk_max_iters;
   float lk_eps;                 /* stop when |delta| < eps */
@@ -279,6 +282,51 @@ int cuimg_hip_tracker_frame_count(cuimg_hip_tracker* t, int* out);
 /* Totals since creation (or clear) at scale s: particles with age > 0 run
  * through the matcher, and how many of them were accepted.  Synchronises. */
 int cuimg_hip_tracker_work(cuimg_hip_tracker* t, int scale, uint64_t* live, uint64_t* accepted);
+
+/* Profile 2 (bc2s strategy).  Particle positions and speeds are integers (the
+ * reference's particle::pos is i_short2) held in the float fields.
+ * The 16-byte descriptors of the dense vector of scale s
+ * (particle_container::features(), tracking2/particle_container.h:95-97):
+ * bytes 0..7 from s1, 8..15 from s2 (the reference's weights are 255 for
+ * every descriptor and are not stored).  Synchronises. */
+int cuimg_hip_tracker_get_features(cuimg_hip_tracker* t, int scale, uint8_t* out, uint32_t cap, uint32_t* n);
+/* Views of the scale's feature images s1 / s2 (bc2s_feature::s1(), s2(),
+ * tracking2/bc2s_feature.h:76-77; borders 3 / 6, clamp-filled) and of its
+ * contrast (contrast_ of generic_strategy, saturated at 255). */
+int cuimg_hip_tracker_bc2s_images(cuimg_hip_tracker* t, int scale, cuimg_img* s1, cuimg_img* s2,
+                                  cuimg_img* contrast);
+/* Views of the scale's flow (generic_strategy::flow(), tracking_strategies.h:82;
+ * F32X2 per 8x8 cell, (-500, -500) = no flow) and multiscale count (U32 per
+ * cell) of the last frame. */
+int cuimg_hip_tracker_flow(cuimg_hip_tracker* t, int scale, cuimg_img* flow, cuimg_img* msc);
+
+/* bc2s_feature<cpu>::update (tracking2/bc2s_feature.hpp:232-253, the build
+ * without OpenCV): s1 = blur121(in), s2 = blur121(s1), each border of s1 / s2
+ * (up to 3 / 6) clamp-filled; and contrast_kernel
+ * (tracking2/tracking_strategies.hpp:104-136) into `contrast` (U32, or U8
+ * saturated at 255, or NULL).  in: U8 of any border (reads clamp-to-edge). */
+int cuimg_hip_bc2s_images(cuimg_hip_ctx* ctx, const cuimg_img* in, const cuimg_img* s1, const cuimg_img* s2,
+                          const cuimg_img* contrast);
+/* gaussian_blur_sigma1 (cpu/gaussian_blur.h:9-47): (2 v + left + right) / 4 on
+ * the rows, then on the columns, in integers; the edge pixels the reference
+ * leaves unwritten use the edge pixel for the missing neighbour. */
+int cuimg_hip_blur121(cuimg_hip_ctx* ctx, const cuimg_img* in, const cuimg_img* out);
+/* gaussian_blur(src, dst, tmp, gaussian_kernel<cuda_gpu>(sigma, hw))
+ * (gpu/gaussian_blur.h:31-93): taps (int)(100 g(sigma, i)), i = -hw..hw; a
+ * column pass into a u8 temporary, then a row pass, each divided by the tap
+ * sum with truncation, centred, clamp-to-edge.  hw <= 15.  Synchronises. */
+int cuimg_hip_gaussian_int(cuimg_hip_ctx* ctx, const cuimg_img* in, const cuimg_img* out, float sigma, int hw);
+/* bc2s_feature::operator() (tracking2/bc2s_feature.hpp:119-143) at n points
+ * (device (row, col) int32 pairs) into out16 (device, 16 bytes each).  s1 / s2
+ * as the tracker holds them (borders >= 3 / >= 6, clamp-filled). */
+int cuimg_hip_bc2s_features(cuimg_hip_ctx* ctx, const cuimg_img* s1, const cuimg_img* s2, const int32_t* pts,
+                            uint32_t n, uint8_t* out16);
+/* two_step_gradient_descent_match (scale 0) or gradient_descent_match at
+ * scale 1 / 2 (tracking2/gradient_descent_matcher.hpp:9-125) of n descriptors
+ * (device, 16 bytes each) from n predictions (device (row, col) int32 pairs):
+ * displacements (device int32 pairs) and distances (device floats). */
+int cuimg_hip_bc2s_match(cuimg_hip_ctx* ctx, const cuimg_img* s1, const cuimg_img* s2, const uint8_t* feats16,
+                         const int32_t* preds, uint32_t n, int scale, int32_t* out_disp, float* out_dist);
 
 /* merge_trajectories_kernel over a dense vector (tracking2/merge_trajectories.h:10-35,
  * launched at pyrlk_tracker.hpp:865-870), with the result of the serial

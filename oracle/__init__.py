@@ -80,13 +80,17 @@ def gpu_sum_lanes(ws):
 def profile_params(profile, nscales, fast_th=20.0, **kw):
     """Profile 'R' = the reference's matcher defaults on u8 frames
     (tracking2/pyrlk_tracker.hpp:173,212,283); profile 'B' = 7x7 window,
-    <=5 iterations, on sigma=1 / 7-tap blurred FP32 frames."""
+    <=5 iterations, on sigma=1 / 7-tap blurred FP32 frames; profile 'C' = the
+    bc2s descriptor with the gradient-descent matcher (generic_strategy,
+    the reference sample's bc2s_fast_gradient_cpu; LK settings unused)."""
     p = TrackerParams()
     p.nscales = nscales
     if profile == "R":
         p.profile, p.lk_ws, p.lk_max_iters, p.lk_top_n = 0, 11, 30, 25
     elif profile == "B":
         p.profile, p.lk_ws, p.lk_max_iters, p.lk_top_n = 1, 7, 5, 49
+    elif profile == "C":
+        p.profile, p.lk_ws, p.lk_max_iters, p.lk_top_n = 2, 7, 5, 49
     else:
         raise ValueError(profile)
     p.lk_eps = 0.1
@@ -168,6 +172,20 @@ class Oracle:
         L.orc_particles_compact.argtypes = [vp, c_int, P(ctypes.c_int32), c_int, c_int, P(ctypes.c_uint32)]
         L.orc_fast_select.argtypes = [P(ctypes.c_uint8), P(ctypes.c_int32), c_int, c_int, c_int, c_float,
                                       P(ctypes.c_int32), c_int]
+        u8p, i32p, u32p = P(ctypes.c_uint8), P(ctypes.c_int32), P(ctypes.c_uint32)
+        L.orc_blur121_u8.argtypes = [u8p, c_int, c_int, u8p]
+        L.orc_gaussian_int_taps.argtypes = [c_float, c_int, i32p]
+        L.orc_gaussian_int_u8.argtypes = [u8p, c_int, c_int, c_float, c_int, u8p]
+        L.orc_contrast_u32.argtypes = [u8p, c_int, c_int, u32p]
+        L.orc_bc2s_feature.argtypes = [u8p, u8p, c_int, c_int, c_int, c_int, u8p]
+        L.orc_bc2s_distance.restype = c_int
+        L.orc_bc2s_distance.argtypes = [u8p, u8p, c_int, c_int, u8p, c_int, c_int, c_int]
+        L.orc_bc2s_match.argtypes = [u8p, u8p, c_int, c_int, u8p, c_int, c_int, c_int, i32p, P(c_float)]
+        L.orc_tracker_get_features.restype = c_int
+        L.orc_tracker_get_features.argtypes = [vp, c_int, u8p, c_int]
+        L.orc_tracker_get_bc2s_images.argtypes = [vp, c_int, vp, vp, vp]
+        L.orc_tracker_flow_dims.argtypes = [vp, c_int, P(c_int), P(c_int)]
+        L.orc_tracker_get_flow.argtypes = [vp, c_int, vp, vp]
         L.orc_lk_loss_grad_f64.restype = c_double
         L.orc_lk_loss_grad_f64.argtypes = [P(c_float), P(c_float), c_int, c_int, P(c_float), c_int,
                                            P(c_float), P(c_float), c_int, c_int, vp, vp]
@@ -326,6 +344,64 @@ class Oracle:
     def tracker(self, nr, nc, params):
         return OracleTracker(self, nr, nc, params)
 
+    # ---------------------------------------------------------------- bc2s
+    def blur121(self, img):
+        img = np.ascontiguousarray(img, np.uint8)
+        out = np.zeros_like(img)
+        self.L.orc_blur121_u8(_ptr(img, ctypes.c_uint8), img.shape[0], img.shape[1], _ptr(out, ctypes.c_uint8))
+        return out
+
+    def gaussian_int_taps(self, sigma, hw):
+        out = np.zeros(2 * hw + 1, np.int32)
+        self.L.orc_gaussian_int_taps(ctypes.c_float(sigma), int(hw), _ptr(out, ctypes.c_int32))
+        return out
+
+    def gaussian_int(self, img, sigma, hw):
+        img = np.ascontiguousarray(img, np.uint8)
+        out = np.zeros_like(img)
+        self.L.orc_gaussian_int_u8(_ptr(img, ctypes.c_uint8), img.shape[0], img.shape[1], ctypes.c_float(sigma),
+                                   int(hw), _ptr(out, ctypes.c_uint8))
+        return out
+
+    def contrast(self, img):
+        img = np.ascontiguousarray(img, np.uint8)
+        out = np.zeros(img.shape, np.uint32)
+        self.L.orc_contrast_u32(_ptr(img, ctypes.c_uint8), img.shape[0], img.shape[1], _ptr(out, ctypes.c_uint32))
+        return out
+
+    def bc2s_images(self, img):
+        """(s1, s2): the bc2s feature images of a u8 frame."""
+        s1 = self.blur121(img)
+        return s1, self.blur121(s1)
+
+    def bc2s_feature(self, s1, s2, r, c):
+        s1 = np.ascontiguousarray(s1, np.uint8)
+        s2 = np.ascontiguousarray(s2, np.uint8)
+        out = np.zeros(16, np.uint8)
+        self.L.orc_bc2s_feature(_ptr(s1, ctypes.c_uint8), _ptr(s2, ctypes.c_uint8), s1.shape[0], s1.shape[1],
+                                int(r), int(c), _ptr(out, ctypes.c_uint8))
+        return out
+
+    def bc2s_distance(self, s1, s2, f, r, c, scale=1):
+        s1 = np.ascontiguousarray(s1, np.uint8)
+        s2 = np.ascontiguousarray(s2, np.uint8)
+        f = np.ascontiguousarray(f, np.uint8)
+        return self.L.orc_bc2s_distance(_ptr(s1, ctypes.c_uint8), _ptr(s2, ctypes.c_uint8), s1.shape[0],
+                                        s1.shape[1], _ptr(f, ctypes.c_uint8), int(r), int(c), int(scale))
+
+    def bc2s_match(self, s1, s2, f, pred, scale=0):
+        """scale 0: two_step_gradient_descent_match; 1 / 2: one descent.
+        Returns ((dr, dc), distance)."""
+        s1 = np.ascontiguousarray(s1, np.uint8)
+        s2 = np.ascontiguousarray(s2, np.uint8)
+        f = np.ascontiguousarray(f, np.uint8)
+        d = np.zeros(2, np.int32)
+        dist = ctypes.c_float()
+        self.L.orc_bc2s_match(_ptr(s1, ctypes.c_uint8), _ptr(s2, ctypes.c_uint8), s1.shape[0], s1.shape[1],
+                              _ptr(f, ctypes.c_uint8), int(pred[0]), int(pred[1]), int(scale),
+                              _ptr(d, ctypes.c_int32), ctypes.byref(dist))
+        return (int(d[0]), int(d[1])), float(dist.value)
+
     # ---------------------------------------------------------------- adjoint
     def loss_grad(self, I0, I1, coef, K, pos, d, ws=7, want_dI=True):
         I0 = np.ascontiguousarray(I0, np.float32)
@@ -444,6 +520,32 @@ class OracleTracker:
         G = np.zeros((nr, nc, 2), np.float32)
         self.L.orc_tracker_get_matcher_images(self.h, s, A.ctypes.data, B.ctypes.data, G.ctypes.data)
         return A, B, G
+
+    def features(self, s):
+        """bc2s descriptors of the dense vector (n x 16 bytes)."""
+        n = self.L.orc_tracker_nparticles(self.h, s)
+        out = np.zeros((max(n, 1), 16), np.uint8)
+        k = self.L.orc_tracker_get_features(self.h, s, _ptr(out, ctypes.c_uint8), n)
+        assert k == n, (k, n)
+        return out[:n]
+
+    def bc2s_images(self, s):
+        """(s1, s2, contrast) of the scale's last frame."""
+        nr, nc = self.dims(s)
+        s1 = np.zeros((nr, nc), np.uint8)
+        s2 = np.zeros((nr, nc), np.uint8)
+        con = np.zeros((nr, nc), np.uint32)
+        self.L.orc_tracker_get_bc2s_images(self.h, s, s1.ctypes.data, s2.ctypes.data, con.ctypes.data)
+        return s1, s2, con
+
+    def flow(self, s):
+        """(flow (cr, cc, 2) float32, multiscale count (cr, cc) uint32)."""
+        cr, cc = ctypes.c_int(), ctypes.c_int()
+        self.L.orc_tracker_flow_dims(self.h, s, ctypes.byref(cr), ctypes.byref(cc))
+        f = np.zeros((cr.value, cc.value, 2), np.float32)
+        m = np.zeros((cr.value, cc.value), np.uint32)
+        self.L.orc_tracker_get_flow(self.h, s, f.ctypes.data, m.ctypes.data)
+        return f, m
 
     def _decisions(self, fn, s):
         n = self.L.orc_tracker_nparticles(self.h, s) + 1

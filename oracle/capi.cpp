@@ -164,10 +164,19 @@ int orc_tracker_get_particles(void* t, int s, orc_particle* out, int cap) {
   return n;
 }
 
+// Profile 2: each particle's descriptor is taken at its position in the
+// scale's current feature images (zero before the first frame).
 int orc_tracker_set_particles(void* t, int s, const orc_particle* in, int n) {
-  level_state& L = static_cast<tracker*>(t)->levels[s];
+  tracker& T = *static_cast<tracker*>(t);
+  level_state& L = T.levels[s];
   L.parts.resize(n);
   copy_bytes(L.parts.data(), in, sizeof(particle) * n);
+  if (T.P.profile == 2) {
+    L.feats.assign(n, feat16{});
+    if (!L.s1.px.empty())
+      for (int i = 0; i < n; i++)
+        L.feats[i] = bc2s_feature(bc2s_images{&L.s1, &L.s2}, tz(L.parts[i].pos_r), tz(L.parts[i].pos_c));
+  }
   return n;
 }
 
@@ -244,6 +253,97 @@ int orc_fast_select(const uint8_t* sal, const int32_t* sparse, int nr, int nc, i
     pts[2 * k + 1] = v[k].second;
   }
   return (int)v.size();
+}
+
+void orc_blur121_u8(const uint8_t* in, int nr, int nc, uint8_t* out) {
+  image<uint8_t> o;
+  blur121_u8(wrap(in, nr, nc), o);
+  copy_bytes(out, o.px.data(), o.px.size());
+}
+
+void orc_gaussian_int_taps(float sigma, int hw, int32_t* out) {
+  const std::vector<int> t = int_gauss_taps(sigma, hw);
+  for (size_t i = 0; i < t.size(); i++) out[i] = t[i];
+}
+
+void orc_gaussian_int_u8(const uint8_t* in, int nr, int nc, float sigma, int hw, uint8_t* out) {
+  image<uint8_t> o;
+  gaussian_int_u8(wrap(in, nr, nc), sigma, hw, o);
+  copy_bytes(out, o.px.data(), o.px.size());
+}
+
+void orc_contrast_u32(const uint8_t* in, int nr, int nc, uint32_t* out) {
+  image<uint32_t> o;
+  contrast_u32(wrap(in, nr, nc), o);
+  copy_bytes(out, o.px.data(), sizeof(uint32_t) * o.px.size());
+}
+
+void orc_bc2s_feature(const uint8_t* s1, const uint8_t* s2, int nr, int nc, int r, int c, uint8_t* out16) {
+  const image<uint8_t> a = wrap(s1, nr, nc), b = wrap(s2, nr, nc);
+  const feat16 f = bc2s_feature(bc2s_images{&a, &b}, r, c);
+  std::memcpy(out16, f.d, 16);
+}
+
+int orc_bc2s_distance(const uint8_t* s1, const uint8_t* s2, int nr, int nc, const uint8_t* f16, int r, int c,
+                      int scale) {
+  const image<uint8_t> a = wrap(s1, nr, nc), b = wrap(s2, nr, nc);
+  feat16 f;
+  std::memcpy(f.d, f16, 16);
+  return bc2s_distance(bc2s_images{&a, &b}, f, r, c, scale);
+}
+
+void orc_bc2s_match(const uint8_t* s1, const uint8_t* s2, int nr, int nc, const uint8_t* f16, int pred_r,
+                    int pred_c, int scale, int32_t* out_d2, float* out_dist) {
+  const image<uint8_t> a = wrap(s1, nr, nc), b = wrap(s2, nr, nc);
+  feat16 f;
+  std::memcpy(f.d, f16, 16);
+  const bc2s_images F{&a, &b};
+  const descent_result m = scale == 0 ? two_step_gradient_descent_match(F, pred_r, pred_c, f)
+                                      : gradient_descent_match(F, pred_r, pred_c, f, scale);
+  out_d2[0] = m.dr;
+  out_d2[1] = m.dc;
+  *out_dist = m.dist;
+}
+
+int orc_tracker_get_features(void* t, int s, uint8_t* out16, int cap) {
+  const level_state& L = static_cast<tracker*>(t)->levels[s];
+  const int n = (int)L.feats.size();
+  if (n > cap) return -n;
+  copy_bytes(out16, L.feats.data(), 16 * (size_t)n);
+  return n;
+}
+
+void orc_tracker_get_bc2s_images(void* t, int s, uint8_t* s1, uint8_t* s2, uint32_t* contrast) {
+  const level_state& L = static_cast<tracker*>(t)->levels[s];
+  const size_t N = (size_t)L.nr * L.nc;
+  auto put = [&](void* dst, const void* src, size_t bytes, size_t have) {
+    if (!dst) return;
+    if (have) copy_bytes(dst, src, bytes);
+    else std::memset(dst, 0, bytes);
+  };
+  put(s1, L.s1.px.data(), N, L.s1.px.size());
+  put(s2, L.s2.px.data(), N, L.s2.px.size());
+  put(contrast, L.contrast.px.data(), 4 * N, L.contrast.px.size());
+}
+
+void orc_tracker_flow_dims(void* t, int s, int* cr, int* cc) {
+  const level_state& L = static_cast<tracker*>(t)->levels[s];
+  *cr = L.cr;
+  *cc = L.cc;
+}
+
+void orc_tracker_get_flow(void* t, int s, float* flow2, uint32_t* msc) {
+  const level_state& L = static_cast<tracker*>(t)->levels[s];
+  const size_t N = (size_t)L.cr * L.cc;
+  if (flow2) {
+    if (L.flow.px.size() == N) copy_bytes(flow2, L.flow.px.data(), sizeof(f2) * N);
+    else
+      for (size_t i = 0; i < 2 * N; i++) flow2[i] = kNoFlow;
+  }
+  if (msc) {
+    if (L.msc.px.size() == N) copy_bytes(msc, L.msc.px.data(), 4 * N);
+    else std::memset(msc, 0, 4 * N);
+  }
 }
 
 void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }

@@ -19,7 +19,7 @@ typedef struct {
 
 typedef struct {
   int32_t nscales;
-  int32_t profile;        /* 0: reference defaults (u8), 1: blurred FP32 */
+  int32_t profile;        /* 0: reference defaults (u8), 1: blurred FP32, 2: bc2s */
   int32_t lk_ws;
   int32_t lk_max_iters;
   float lk_eps;
@@ -100,6 +100,31 @@ void orc_particle_mask(const orc_particle* parts, int n, int nr, int nc, uint8_t
  * raster order; returns the number of points (at most cap written). */
 int orc_fast_select(const uint8_t* sal, const int32_t* sparse, int nr, int nc, int box, float th, int32_t* pts,
                     int cap);
+
+/* ------------------------------------------------------------ bc2s path */
+/* Integer 1-2-1 blur, rows then columns, each / 4 truncated, clamp-to-edge
+ * (cpu/gaussian_blur.h:9-47). */
+void orc_blur121_u8(const uint8_t* in, int nr, int nc, uint8_t* out);
+/* Taps (int)(100 g(sigma, i)), i = -hw..hw (gpu/gaussian_blur.h:31-53). */
+void orc_gaussian_int_taps(float sigma, int hw, int32_t* out);
+/* Separable integer Gaussian with those taps (columns then rows, each pass
+ * divided by the tap sum with truncation into u8, clamp-to-edge). */
+void orc_gaussian_int_u8(const uint8_t* in, int nr, int nc, float sigma, int hw, uint8_t* out);
+/* contrast_kernel (tracking_strategies.hpp:104-136), clamp-to-edge reads. */
+void orc_contrast_u32(const uint8_t* in, int nr, int nc, uint32_t* out);
+/* bc2s descriptor at (r, c) of the feature images s1, s2 (16 bytes). */
+void orc_bc2s_feature(const uint8_t* s1, const uint8_t* s2, int nr, int nc, int r, int c, uint8_t* out16);
+int orc_bc2s_distance(const uint8_t* s1, const uint8_t* s2, int nr, int nc, const uint8_t* f16, int r, int c,
+                      int scale);
+/* scale 1 or 2: gradient_descent_match at that scale; 0: the two-step match.
+ * out_d2: (row, col) displacement from the prediction. */
+void orc_bc2s_match(const uint8_t* s1, const uint8_t* s2, int nr, int nc, const uint8_t* f16, int pred_r,
+                    int pred_c, int scale, int32_t* out_d2, float* out_dist);
+/* Tracker state of the bc2s strategy (profile 2). */
+int orc_tracker_get_features(void* t, int s, uint8_t* out16, int cap);
+void orc_tracker_get_bc2s_images(void* t, int s, uint8_t* s1, uint8_t* s2, uint32_t* contrast);
+void orc_tracker_flow_dims(void* t, int s, int* cr, int* cc);
+void orc_tracker_get_flow(void* t, int s, float* flow2, uint32_t* msc);
 
 /* OpenMP threads of the oracle's parallel loops. */
 void orc_set_threads(int n);

@@ -119,4 +119,34 @@ struct match_result { f2 tr; float dist; };
 match_result lk_match(const lk_params& P, f2 p, f2 pred_tr, const lk_level& L);
 void sobel_gradient(const image<float>& prev, bool integer_source, image<f2>& g);
 
+// ---------------------------------------------------------------- bc2s
+// Integer 1-2-1 blur (rows then columns, each truncated by / 4), clamp-to-edge.
+void blur121_u8(const image<uint8_t>& in, image<uint8_t>& out);
+// Integer taps (int)(100 * g(sigma, i)) of gaussian_kernel<cuda_gpu>.
+std::vector<int> int_gauss_taps(float sigma, int hw);
+// Separable integer Gaussian with those taps (columns then rows), each pass
+// truncated by the tap sum into u8, clamp-to-edge.
+void gaussian_int_u8(const image<uint8_t>& in, float sigma, int hw, image<uint8_t>& out);
+// contrast_kernel: |differences| of the 4 opposite pairs at distance 2.
+void contrast_u32(const image<uint8_t>& in, image<uint32_t>& out);
+
+// bc2s descriptor: 8 samples of s1 on the radius-3 ring (every other point)
+// and 8 of s2 at twice the offsets.  The weights of the reference's
+// descriptor are 255 for every descriptor it builds, so they are not stored.
+struct feat16 {
+  uint8_t d[16];
+};
+struct bc2s_images {
+  const image<uint8_t>* s1;
+  const image<uint8_t>* s2;
+};
+feat16 bc2s_feature(const bc2s_images& F, int r, int c);
+int bc2s_distance(const bc2s_images& F, const feat16& a, int r, int c, int scale);
+struct descent_result {
+  int dr, dc;
+  float dist;
+};
+descent_result gradient_descent_match(const bc2s_images& F, int pr, int pc, const feat16& f, int scale);
+descent_result two_step_gradient_descent_match(const bc2s_images& F, int pr, int pc, const feat16& f);
+
 }  // namespace orc

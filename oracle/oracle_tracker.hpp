@@ -8,7 +8,8 @@ namespace orc {
 
 struct tracker_params {
   int nscales = 3;
-  int profile = 1;  // 0: reference defaults on u8 frames, 1: blurred FP32 frames
+  int profile = 1;  // 0: reference defaults on u8 frames, 1: blurred FP32 frames,
+                    // 2: bc2s descriptor + gradient-descent matcher (generic_strategy)
   lk_params lk;
   int fast_n = 9;
   float fast_th = 20.f;
@@ -43,7 +44,20 @@ struct level_state {
   bool seen = false;  // a frame went through this scale since the last tracker clear
   box pdomain{0, 0, -1, -1};  // pyrlk_cpu::particle_domain_
   int box_size = 3;             // fast_detector::set_box_size of this scale
+  // bc2s strategy (generic_strategy, profile 2)
+  image<uint32_t> contrast;     // contrast_
+  image<uint8_t> s1, s2;        // feature_ (bc2s_feature)
+  std::vector<feat16> feats;    // pset.features(), parallel to parts
+  int cr = 0, cc = 0;           // flow cells: domain_div_up(domain, 8)
+  std::vector<int> st_count;    // flow_stats_ .first
+  std::vector<f2> st_speed;     // flow_stats_ .second
+  image<f2> flow;               // flow_
+  image<uint32_t> msc;          // multiscale_count_
 };
+
+// NO_FLOW (tracking2/tracker_defines.h:4)
+constexpr float kNoFlow = -500.f;
+constexpr int kFlowRatio = 8;  // generic_strategy::flow_ratio
 
 // Detector mask of a dense vector (255, 0 on the 3x3 block of every particle).
 image<uint8_t> particle_mask(const std::vector<particle>& parts, int nr, int nc);
@@ -76,6 +90,11 @@ struct tracker {
   void new_particles(level_state& L);
   void compact(level_state& L);
   static void merge(level_state& L);
+  // generic_strategy<bc2s_feature, fast_detector> (profile 2)
+  void update_level_bc2s(int s);
+  void match_bc2s(int s);
+  void flow_stats_bc2s(int s);
+  void filter_bc2s(level_state& L);
 };
 
 void merge_state(std::vector<particle>& parts, const image<int32_t>& sparse);

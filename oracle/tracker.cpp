@@ -21,6 +21,8 @@ tracker::tracker(int nr, int nc, const tracker_params& p) : P(p) {
     L.sparse = image<int32_t>(r, c, -1);  // particle_container ctor memset -1
     L.pdomain = inner(r, c, 3);           // pyrlk_cpu ctor: d - border(3)
     L.box_size = P.box_size;
+    L.cr = (r + kFlowRatio - 1) / kFlowRatio;  // domain_div_up (generic_strategy ctor)
+    L.cc = (c + kFlowRatio - 1) / kFlowRatio;
     levels.push_back(std::move(L));
     r = r / 2;  // obox2d / 2 (tracking2/tracker.hpp:18-26)
     c = c / 2;
@@ -37,7 +39,10 @@ void tracker::clear() {
     L.frame_cpt = 0;
     L.seen = false;
     L.parts.clear();
+    L.feats.clear();
     L.matches.clear();
+    L.s1 = image<uint8_t>();  // descriptors of set particles are zero until the next frame
+    L.s2 = image<uint8_t>();
     std::fill(L.sparse.px.begin(), L.sparse.px.end(), -1);
   }
 }
@@ -311,6 +316,7 @@ void merge_state(std::vector<particle>& parts, const image<int32_t>& sparse) {
 // of the previous and the current frame), which is what its frame counter
 // restarting changes nothing about.
 void tracker::update_level(int s) {
+  if (P.profile == 2) return update_level_bc2s(s);
   level_state& L = levels[s];
   if (L.frame_cpt > 0 || L.seen) L.prev_u8 = L.cur_u8;
   else L.prev_u8 = L.in_u8;
@@ -326,6 +332,168 @@ void tracker::update_level(int s) {
     compact(L);
   }
   if (!(L.frame_cpt % P.filtering_frequency)) {
+    if (P.with_merge) merge(L);
+  }
+  L.frame_cpt++;
+}
+
+// ------------------------------------------------------------- bc2s strategy
+// match_particles_kernel (tracking2/tracking_strategies.hpp:274-402) with
+// particle_container::before_matching and move / remove applied in ascending
+// index order (the sparse buffer keeps the last writer).  Positions and speeds
+// are integers (particle::pos is i_short2, the moves are integer); they are
+// kept in the float fields.  multiscale_prediction returns
+// motion_based_prediction (predictions.h:20-32): no motion for age 1, else the
+// speed.  With every descriptor weight 255, cpt = 8 and the test is
+// n_diff < 4 over the s2 samples.
+void tracker::match_bc2s(int s) {
+  level_state& L = levels[s];
+  std::fill(L.sparse.px.begin(), L.sparse.px.end(), -1);
+  L.compact_has_run = false;
+  const bc2s_images F{&L.s1, &L.s2};
+  auto has = [&](int r, int c) { return r >= 0 && r < L.nr && c >= 0 && c < L.nc; };
+  L.last_decisions.assign(L.parts.size(), lk_decision{0, f2{0.f, 0.f}, 0.f});
+  for (size_t i = 0; i < L.parts.size(); i++) {
+    particle& p = L.parts[i];
+    if (p.age == 0) continue;
+    const int r = tz(p.pos_r), c = tz(p.pos_c);
+    const int pr = r + (p.age == 1 ? 0 : tz(p.speed_r)), pc = c + (p.age == 1 ? 0 : tz(p.speed_c));
+    bool keep = false;
+    lk_decision& d = L.last_decisions[i];
+    if (has(pr, pc)) {
+      const descent_result m = two_step_gradient_descent_match(F, pr, pc, L.feats[i]);
+      const int mr = pr + m.dr, mc = pc + m.dc;
+      d.dist = m.dist;
+      d.tr = f2{(float)(pr - r + m.dr), (float)(pc - c + m.dc)};
+      if (has(mr, mc)) {
+        const feat16 mf = bc2s_feature(F, mr, mc);
+        int n_diff = 0;
+        for (int k = 8; k < 16; k++)
+          if (std::abs((int)L.feats[i].d[k] - (int)mf.d[k]) > P.k) n_diff++;
+        if (n_diff < 4 && !((float)L.contrast(mr, mc) < 3.f)) {
+          keep = true;
+          // kernel_particle_container::move (particle_container.hpp:422-466)
+          p.age++;
+          p.speed_r = d.tr.r;
+          p.speed_c = d.tr.c;
+          p.pos_r = (float)mr;
+          p.pos_c = (float)mc;
+          if (norml2(p.speed_r, p.speed_c) > 0.f) L.feats[i] = mf;
+          L.sparse(mr, mc) = (int32_t)i;
+        }
+      }
+    }
+    d.accept = keep ? 1 : 0;
+    if (!keep) p.age = 0;
+  }
+}
+
+// compute_flow_stats_kernel + flow_fusion_kernel / flow_fusion_kernel_root
+// (tracking_strategies.hpp:537-634, 688-701): per 8x8 cell, the number of
+// pixels holding a particle of age >= 1 and the speed of the last one in
+// raster order (the reference assigns, it does not sum); then the fused count
+// and flow.  An upper cell bin / 2 outside the coarser scale's cell grid (a
+// side of 16k + 1 pixels) is read as empty (count 0, no flow): the reference
+// reads past its row there.
+void tracker::flow_stats_bc2s(int s) {
+  level_state& L = levels[s];
+  L.st_count.assign((size_t)L.cr * L.cc, 0);
+  L.st_speed.assign((size_t)L.cr * L.cc, f2{0.f, 0.f});
+  for (int R = 0; R < L.cr; R++)
+    for (int C = 0; C < L.cc; C++) {
+      int count = 0;
+      f2 last{0.f, 0.f};
+      for (int r = 0; r < kFlowRatio; r++)
+        for (int c = 0; c < kFlowRatio; c++) {
+          const int y = R * kFlowRatio + r, x = C * kFlowRatio + c;
+          if (!L.sparse.has(y, x) || L.sparse(y, x) == -1) continue;
+          const particle& q = L.parts[L.sparse(y, x)];
+          if (q.age >= 1) {
+            count++;
+            last = f2{q.speed_r, q.speed_c};
+          }
+        }
+      L.st_count[(size_t)R * L.cc + C] = count;
+      L.st_speed[(size_t)R * L.cc + C] = last;
+    }
+  L.flow = image<f2>(L.cr, L.cc, f2{kNoFlow, kNoFlow});
+  L.msc = image<uint32_t>(L.cr, L.cc, 0u);
+  const bool root = s + 1 >= (int)levels.size();
+  for (int R = 0; R < L.cr; R++)
+    for (int C = 0; C < L.cc; C++) {
+      const int n = L.st_count[(size_t)R * L.cc + C];
+      const f2 v = L.st_speed[(size_t)R * L.cc + C];
+      if (root) {
+        L.msc(R, C) = (uint32_t)n;
+        if (n > 1) L.flow(R, C) = f2{v.r / (float)n, v.c / (float)n};
+      } else {
+        const level_state& U = levels[s + 1];
+        const bool in = R / 2 < U.cr && C / 2 < U.cc;
+        const int un = in ? U.st_count[(size_t)(R / 2) * U.cc + C / 2] : 0;
+        const f2 uf = in ? U.flow(R / 2, C / 2) : f2{kNoFlow, kNoFlow};
+        L.msc(R, C) = (uint32_t)(n + un);
+        if (n > 1) L.flow(R, C) = v;
+        else if (uf.r != kNoFlow || uf.c != kNoFlow) L.flow(R, C) = f2{uf.r * 2.f, uf.c * 2.f};
+      }
+    }
+}
+
+// filter_bad_particles_kernel (tracking_strategies.hpp:491-525) on every
+// particle of the dense vector.
+void tracker::filter_bc2s(level_state& L) {
+  for (particle& p : L.parts) {
+    const int R = tz(p.pos_r) / kFlowRatio, C = tz(p.pos_c) / kFlowRatio;
+    const uint32_t m = L.msc(R, C);
+    if (p.age > 2 && m > 0) {
+      const f2 f = L.flow(R, C);
+      if (norml2(p.speed_r - f.r, p.speed_c - f.c) > 10.f) {
+        p.age = 0;
+        continue;
+      }
+    }
+    bool alone = true;
+    for (int k = 0; k < 8; k++) {
+      const int y = R + kC8[k][0], x = C + kC8[k][1];
+      if (L.msc.has(y, x) && L.msc(y, x) != 0) alone = false;
+    }
+    if (alone && m == 1) p.age = 0;
+  }
+}
+
+// generic_strategy::update (tracking_strategies.hpp:155-198) with
+// bc2s_feature<cpu>::update built without OpenCV (bc2s_feature.hpp:232-253:
+// two 1-2-1 blurs, then the borders clamp-filled) and
+// generic_strategy::create_detector_mask (:228-272: the particle mask, then 0
+// where the contrast is below 5).  estimate_camera_motion sets zero motions
+// that nothing reads.
+void tracker::update_level_bc2s(int s) {
+  level_state& L = levels[s];
+  L.cur_u8 = L.in_u8;
+  contrast_u32(L.cur_u8, L.contrast);
+  blur121_u8(L.cur_u8, L.s1);
+  blur121_u8(L.s1, L.s2);
+  L.seen = true;
+  match_bc2s(s);
+  flow_stats_bc2s(s);
+  if (!(L.frame_cpt % P.detector_frequency)) {
+    detector_mask(L);
+    for (size_t i = 0; i < L.mask.px.size(); i++)
+      if (L.contrast.px[i] < 5) L.mask.px[i] = 0;
+    fast_saliency(L.cur_u8, L.mask, P.fast_n, (int)P.fast_th, L.sal);
+    const std::vector<std::pair<int, int>> pts = fast_select(L.sal, L.sparse, L.box_size, P.fast_th);
+    const bc2s_images F{&L.s1, &L.s2};
+    for (const auto& rc : pts) L.feats.push_back(bc2s_feature(F, rc.first, rc.second));
+    container_add(L.parts, L.sparse, pts);
+    // compact(): the features follow their particles
+    std::vector<feat16> kept;
+    kept.reserve(L.feats.size());
+    for (size_t i = 0; i < L.parts.size(); i++)
+      if (L.parts[i].age != 0) kept.push_back(L.feats[i]);
+    L.feats.swap(kept);
+    compact(L);
+  }
+  if (!(L.frame_cpt % P.filtering_frequency)) {
+    filter_bc2s(L);
     if (P.with_merge) merge(L);
   }
   L.frame_cpt++;
