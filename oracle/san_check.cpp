@@ -7,6 +7,7 @@
 // sparse image consistent), so a silent corruption also fails.
 //
 //   san_check [frames]     exit status 0 = clean; sanitizer reports abort
+// (profiles 0 and 1: the pyramid-LK strategy; 2: the bc2s strategy)
 #include "oracle.h"
 
 #include <cmath>
@@ -127,6 +128,25 @@ void images(const std::vector<uint8_t>& v, int nr, int nc) {
   const double L = orc_lk_loss_grad_f64(f.data(), f.data(), nr, nc, coef.data(), 3, pos2.data(), d2.data(), 2, 7,
                                         gd.data(), gI.data());
   check(std::isfinite(L), "adjoint loss finite");
+  // bc2s path
+  std::vector<uint8_t> b1(N), b2(N), gi(N);
+  std::vector<uint32_t> con(N);
+  orc_blur121_u8(v.data(), nr, nc, b1.data());
+  orc_blur121_u8(b1.data(), nr, nc, b2.data());
+  orc_contrast_u32(v.data(), nr, nc, con.data());
+  std::vector<int32_t> taps(7);
+  orc_gaussian_int_taps(2.f, 3, taps.data());
+  orc_gaussian_int_u8(v.data(), nr, nc, 2.f, 3, gi.data());
+  for (int k = 0; k < 40; k++) {
+    const int r = (k * 37) % nr, c = (k * 53) % nc;
+    uint8_t f16[16];
+    orc_bc2s_feature(b1.data(), b2.data(), nr, nc, r, c, f16);
+    check(orc_bc2s_distance(b1.data(), b2.data(), nr, nc, f16, r, c, 1) == 0, "bc2s distance at its pixel");
+    int32_t d2[2];
+    float dist;
+    for (int scale = 0; scale < 3; scale++)
+      orc_bc2s_match(b1.data(), b2.data(), nr, nc, f16, r + (k % 5) - 2, c - (k % 3) + 1, scale, d2, &dist);
+  }
 }
 
 // Standalone container routines on a state taken from a tracker.
@@ -163,9 +183,11 @@ void container(void* t, int s) {
 
 void tracker_runs(const std::vector<uint8_t>& v, int nr, int nc, int nframes) {
   const size_t N = (size_t)nr * nc;
-  for (int profile = 0; profile < 2; profile++)
+  for (int profile = 0; profile < 3; profile++)
     for (int lanes : {0, 16}) {
-      const orc_tracker_params P = params(profile, 3, 12.f, profile == 0 && lanes ? 64 : lanes);
+      if (profile == 2 && lanes) continue;  // integer strategy: one arithmetic
+      orc_tracker_params P = params(profile, 3, 12.f, profile == 0 && lanes ? 64 : lanes);
+      if (profile == 2) P.detector_frequency = 3;
       void* a = orc_tracker_create(nr, nc, &P);
       void* b = orc_tracker_create(nr, nc, &P);
       check(orc_tracker_nscales(a) == 3, "nscales");
@@ -201,7 +223,18 @@ void tracker_runs(const std::vector<uint8_t>& v, int nr, int nc, int nframes) {
         std::vector<int32_t> acc(n + 1);
         std::vector<float> tr(2 * n + 2), dist(n + 1);
         orc_tracker_get_decisions(a, s, acc.data(), tr.data(), dist.data(), n);
-        orc_tracker_lk_decide(a, s, acc.data(), tr.data(), dist.data(), n);
+        if (profile != 2) orc_tracker_lk_decide(a, s, acc.data(), tr.data(), dist.data(), n);
+        if (profile == 2) {
+          std::vector<uint8_t> f16(16 * (size_t)(n + 1)), i1(Ns), i2(Ns);
+          std::vector<uint32_t> cst(Ns);
+          check(orc_tracker_get_features(a, s, f16.data(), n) == n, "features");
+          orc_tracker_get_bc2s_images(a, s, i1.data(), i2.data(), cst.data());
+          int cr, cc;
+          orc_tracker_flow_dims(a, s, &cr, &cc);
+          std::vector<float> fl(2 * (size_t)cr * cc);
+          std::vector<uint32_t> msc((size_t)cr * cc);
+          orc_tracker_get_flow(a, s, fl.data(), msc.data());
+        }
         container(a, s);
       }
       // settings of a scale, particles set from outside, clear

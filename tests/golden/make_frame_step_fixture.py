@@ -3,7 +3,9 @@ each of 10 frames of the seeded 320x240 translation video (3 scales, FAST
 threshold 20), for the two matcher profiles (R: the reference's matcher on u8
 frames; B: 7x7 / 5 steps on blurred FP32 frames) and the oracle's two
 arithmetic readings ("raster": every product rounded, sums in loop order;
-"gpu": fused multiply-adds and lane-group sums).
+"gpu": fused multiply-adds and lane-group sums), and for the bc2s strategy
+(C: integer arithmetic, one reading, the reference sample's detector
+frequency of 10; its descriptors, flow and multiscale counts are stored too).
 
 Stored per frame and scale: the dense vector's length, its live count, the
 sum of ages, and SHA-256 digests of the dense vector (speed, position, age),
@@ -50,6 +52,19 @@ def state(t, nscales):
     return out
 
 
+def state_bc2s(t, nscales):
+    """state() plus the bc2s descriptors, flow and multiscale counts."""
+    out = state(t, nscales)
+    for s in range(nscales):
+        flow, msc = t.flow(s)
+        out[s].update({"features": digest(t.features(s)), "flow": digest(flow.astype(np.float32)),
+                       "msc": digest(msc.astype(np.uint32))})
+    return out
+
+
+C_DET_FREQ = 10
+
+
 def video():
     import cuimg_amd as cu
     return cu.synth_video(SEED, NR, NC, NFRAMES, 0)
@@ -74,6 +89,13 @@ def main():
                 frames.append(state(t, NSCALES))
             fixture["runs"][f"{profile}/{arith}"] = frames
             print(profile, arith, [sc["live"] for sc in frames[-1]])
+    t = orc.tracker(NR, NC, oracle.profile_params("C", NSCALES, TH, detector_frequency=C_DET_FREQ))
+    frames = []
+    for f in range(NFRAMES):
+        t.run(v[f])
+        frames.append(state_bc2s(t, NSCALES))
+    fixture["runs"]["C/exact"] = frames
+    print("C exact", [sc["live"] for sc in frames[-1]])
     with open(OUT, "w") as fh:
         json.dump(fixture, fh, indent=1)
     print("wrote", OUT)

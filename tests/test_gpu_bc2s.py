@@ -235,3 +235,20 @@ def test_bc2s_batch_streams_equal_own_trackers(ctx):
     for t in solo:
         t.close()
     batch.close()
+
+
+def test_bc2s_free_run_matches_committed_fixture(ctx):
+    """Ten free-running frames of the 320x240 video with the reference
+    sample's detector frequency: every frame equals the committed fixture."""
+    import json
+    import os
+
+    import golden.make_frame_step_fixture as m
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "frame_step_320x240.json")) as f:
+        want = json.load(f)["runs"]["C/exact"]
+    t = cu.Tracker(ctx, m.NR, m.NC, cu.profile_params("C", m.NSCALES, m.TH, detector_frequency=m.C_DET_FREQ))
+    v = m.video()
+    for f in range(m.NFRAMES):
+        t.run(v[f])
+        assert m.state_bc2s(t, m.NSCALES) == want[f], f
+    t.close()

@@ -2,7 +2,8 @@
 (tests/golden/frame_step_320x240.json, written by
 tests/golden/make_frame_step_fixture.py): a regression pin of every frame's
 dense vector, sparse image, saliency and matches, for both matcher profiles
-and both arithmetic readings, and of the synthetic video itself."""
+and both arithmetic readings and for the bc2s strategy (with its descriptors,
+flow and multiscale counts), and of the synthetic video itself."""
 import json
 import os
 
@@ -23,6 +24,15 @@ def fixture():
 def test_video_is_the_fixture_video(fixture):
     import golden.make_frame_step_fixture as m
     assert m.digest(m.video()) == fixture["video"]["digest"]
+
+
+def test_oracle_bc2s_frame_steps_match_fixture(orc, fixture):
+    import golden.make_frame_step_fixture as m
+    t = orc.tracker(m.NR, m.NC, oracle.profile_params("C", m.NSCALES, m.TH, detector_frequency=m.C_DET_FREQ))
+    v = m.video()
+    for f, want in enumerate(fixture["runs"]["C/exact"]):
+        t.run(v[f])
+        assert m.state_bc2s(t, m.NSCALES) == want, f
 
 
 @pytest.mark.parametrize("run", ["B/raster", "B/gpu", "R/raster", "R/gpu"])
