@@ -1,10 +1,12 @@
 // Particle container of one tracker scale, read from the device
 // (tracking2/particle_container.h:13-225, .hpp:22-578).  The dense AoS layout
 // particle_f<translation_transform> is the reference's: 20 bytes,
-// {speed(row,col), pos(row,col), unsigned short age}.
+// {speed(row,col), pos(row,col), unsigned short age}; the bc2s strategy's
+// particle (integer position) is converted from it.
 #ifndef CUIMG_TRACKING2_PARTICLE_CONTAINER_H_
 # define CUIMG_TRACKING2_PARTICLE_CONTAINER_H_
 
+# include <cstring>
 # include <functional>
 # include <vector>
 
@@ -38,16 +40,61 @@ namespace cuimg
   static_assert(sizeof(particle_f<translation_transform>) == sizeof(cuimg_particle_f),
                 "particle_f<translation_transform> is the 20-byte cuimg_particle_f");
 
-  // Read side of particle_container<F, particle_f<T>, A> for one scale of a
-  // device tracker.  Accessors copy the device state on demand.
-  class particle_container_view
+  // particle (tracking2/particle_container.h:13-45): the bc2s strategy's
+  // particle, integer position.  The match times are not used by the
+  // strategy and read as 0.
+  struct particle
+  {
+    typedef translation_transform transform;
+    typedef i_short2 coords_type;
+    particle() : speed(0.f, 0.f), age(0), fault(0), prev_match_time(0), next_match_time(0) {}
+    translation_transform speed;
+    i_short2 pos;
+    unsigned short age;
+    unsigned short fault;
+    int prev_match_time;
+    int next_match_time;
+  };
+
+  // bc2s (tracking2/bc2s_feature.h:16-43): 16 descriptor bytes and their
+  // weights (255 for every descriptor the strategy builds).
+  struct bc2s
+  {
+    bc2s() { for (int i = 0; i < 16; i++) distances[i] = 0, weights[i] = 255; }
+    const unsigned char& operator[](unsigned i) const { return distances[i]; }
+    unsigned char& operator[](unsigned i) { return distances[i]; }
+    unsigned char distances[16];
+    unsigned char weights[16];
+  };
+
+  namespace internal
+  {
+    inline void from_device(const cuimg_particle_f& d, particle_f<translation_transform>& p)
+    {
+      static_assert(sizeof(p) == sizeof(d), "same layout");
+      std::memcpy(static_cast<void*>(&p), &d, sizeof d);
+    }
+    inline void from_device(const cuimg_particle_f& d, particle& p)
+    {
+      p.speed = translation_transform(d.speed_r, d.speed_c);
+      p.pos = i_short2((short)d.pos_r, (short)d.pos_c);
+      p.age = d.age;
+      p.fault = 0;
+      p.prev_match_time = p.next_match_time = 0;
+    }
+  }
+
+  // Read side of particle_container<F, P, A> for one scale of a device
+  // tracker.  Accessors copy the device state on demand.
+  template <typename P>
+  class particle_container_view_t
   {
   public:
-    typedef particle_f<translation_transform> particle_type;
+    typedef P particle_type;
     typedef std::vector<particle_type> particle_vector;
     typedef std::vector<unsigned int> uint_vector;
 
-    particle_container_view(std::function<cuimg_hip_tracker*()> t, std::function<hip_context*()> ctx, int scale)
+    particle_container_view_t(std::function<cuimg_hip_tracker*()> t, std::function<hip_context*()> ctx, int scale)
       : t_(t), ctx_(ctx), scale_(scale) {}
 
     const particle_vector& dense_particles() const
@@ -57,11 +104,26 @@ namespace cuimg
       dense_.clear();
       if (!t) return dense_;
       ctx_()->check(cuimg_hip_tracker_get_particles(t, scale_, nullptr, 0, &n), "dense_particles");
+      raw_.resize(n);
+      if (n) ctx_()->check(cuimg_hip_tracker_get_particles(t, scale_, raw_.data(), n, &n), "dense_particles");
       dense_.resize(n);
-      if (n)
-        ctx_()->check(cuimg_hip_tracker_get_particles(t, scale_, reinterpret_cast<cuimg_particle_f*>(dense_.data()),
-                                                    n, &n), "dense_particles");
+      for (uint32_t i = 0; i < n; i++) internal::from_device(raw_[i], dense_[i]);
       return dense_;
+    }
+
+    // particle_container::features() of the bc2s strategy.
+    const std::vector<bc2s>& features() const
+    {
+      cuimg_hip_tracker* t = t_();
+      uint32_t n = 0;
+      feats_.clear();
+      if (!t) return feats_;
+      ctx_()->check(cuimg_hip_tracker_get_features(t, scale_, nullptr, 0, &n), "features");
+      std::vector<uint8_t> bytes(16 * (size_t)n);
+      if (n) ctx_()->check(cuimg_hip_tracker_get_features(t, scale_, bytes.data(), n, &n), "features");
+      feats_.resize(n);
+      for (uint32_t i = 0; i < n; i++) std::memcpy(feats_[i].distances, &bytes[16 * (size_t)i], 16);
+      return feats_;
     }
 
     int size() const
@@ -139,9 +201,13 @@ namespace cuimg
     std::function<cuimg_hip_tracker*()> t_;
     std::function<hip_context*()> ctx_;
     int scale_;
+    mutable std::vector<cuimg_particle_f> raw_;
     mutable particle_vector dense_;
     mutable uint_vector matches_;
+    mutable std::vector<bc2s> feats_;
   };
+
+  typedef particle_container_view_t<particle_f<translation_transform> > particle_container_view;
 }
 
 #endif

@@ -14,6 +14,7 @@
 # include <cuimg/gl.h>
 # include <cuimg/gpu/device_image2d.h>
 # include <cuimg/tracking2/pyrlk_hip.h>
+# include <cuimg/tracking2/tracking_strategies.h>
 
 namespace cuimg
 {

@@ -252,3 +252,21 @@ def test_bc2s_free_run_matches_committed_fixture(ctx):
         t.run(v[f])
         assert m.state_bc2s(t, m.NSCALES) == want[f], f
     t.close()
+
+
+def test_cpp_bc2s_sample_facade_matches_c_abi():
+    """The reference sample's own strategy and settings through the C++ facade
+    (tracker<tracking_strategies::bc2s_fast_gradient_cpu>): trajectories kept
+    with sync_attributes, the facade's particles and descriptors equal a raw C
+    ABI tracker's; the sample checks it and prints OK."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cuimg_amd", "bin",
+                       "bc2s_sample")
+    if not os.path.exists(exe):
+        pytest.skip("the C++ sample is not built")
+    print("bc2s_sample 3 20 14 240 320", flush=True)
+    r = subprocess.run([exe, "3", "20", "14", "240", "320"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert r.stdout.strip().endswith("OK")
+
