@@ -16,7 +16,8 @@ over scales and timed frames, divided by the timed wall time.
 
 Printed: one JSON line (rank 0).  Extra keys carry frames/s, per-kernel
 timings, the threshold-20 run, the Gaussian / pyramid roofline microbench and
-the CPU baseline.
+the CPU baseline, and the bc2s strategy (the reference sample's default
+tracker) on the same video with the sample's settings.
 """
 import argparse
 import json
@@ -51,6 +52,10 @@ def lk_algorithmic_bytes(level_shapes, nparts):
 
 KERNELS = ("pyramid_build", "gaussian_sep", "gradient", "fills", "lk_track", "particle_mask", "fast_saliency",
            "detect_count", "scatter_insert", "merge_scan", "merge_round1", "merge_resolve")
+BC2S_KERNELS = ("pyramid_build", "bc2s_images", "fast_saliency", "fills", "bc2s_match", "flow_stats", "flow_fusion",
+                "particle_mask", "detect_count", "scatter_insert", "filter", "merge_scan", "merge_round1",
+                "merge_resolve")
+BC2S_DET_FREQ = 10  # the reference sample's detector frequency
 
 
 def upload_video(cu, ctx, nframes):
@@ -65,13 +70,14 @@ def upload_video(cu, ctx, nframes):
     return video, frames
 
 
-def tracker_bench(cu, ctx, frames, fast_th, warmup, steps, prof_steps, pct_steps=0):
+def tracker_bench(cu, ctx, frames, fast_th, warmup, steps, prof_steps, pct_steps=0, profile="B", det_freq=1,
+                  kernel_names=KERNELS):
     """warmup untimed frames, `steps` timed frame steps replayed from the
     captured graph, `pct_steps` more graph steps each between two HIP events
     (median and p10 / p90 of the step time), then `prof_steps` more steps with
     plain launches and HIP events around every kernel (the per-kernel
     breakdown)."""
-    params = cu.profile_params("B", NSCALES, fast_th)
+    params = cu.profile_params(profile, NSCALES, fast_th, detector_frequency=det_freq)
     trk = cu.Tracker(ctx, NR, NC, params)
     shapes = [(NR >> s, NC >> s) for s in range(NSCALES)]
     for f in range(warmup):
@@ -103,7 +109,7 @@ def tracker_bench(cu, ctx, frames, fast_th, warmup, steps, prof_steps, pct_steps
         ctx.sync()
         pdt = time.perf_counter() - t1
         live2 = [trk.work(s)[0] for s in range(NSCALES)]
-        for name in KERNELS:
+        for name in kernel_names:
             n, ms = ctx.profile_query(name)
             if n:
                 kernels[name] = {"launches": n, "total_ms": round(ms, 4), "avg_us": round(ms / n * 1000, 3)}
@@ -442,6 +448,28 @@ def cpu_baseline(video, budget_s, fast_th):
             "one_thread": {"value": round(u1 / s1, 1), "sample": f"frames 1..{f1}, {s1:.1f} s"}}
 
 
+def bc2s_cpu_baseline(video, budget_s, fast_th):
+    """The CPU oracle's bc2s strategy (serial loops) on the first frames of the
+    same video with the reference sample's settings."""
+    import oracle
+    orc = oracle.Oracle()
+    nr, nc = video.shape[1:]
+    t = orc.tracker(nr, nc, oracle.profile_params("C", NSCALES, fast_th, detector_frequency=BC2S_DET_FREQ))
+    t.run(video[0])
+    frames, updates, spent = 0, 0, 0.0
+    while spent < budget_s and frames + 1 < len(video):
+        live = sum(int((t.particles(s)["age"] > 0).sum()) for s in range(NSCALES))
+        t0 = time.perf_counter()
+        t.run(video[frames + 1])
+        spent += time.perf_counter() - t0
+        frames += 1
+        updates += live
+    return {"value": round(updates / spent, 1), "unit": "particle-updates/s", "cores": 1, "kind": "port",
+            "frames_per_s": round(frames / spent, 2),
+            "sample": f"CPU oracle bc2s strategy, frames 1..{frames} of the same translation video at FAST th {fast_th:g}, "
+                      f"detector every {BC2S_DET_FREQ} frames ({spent:.1f} s, {updates} particle updates)"}
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -524,6 +552,7 @@ def main():
     ap.add_argument("--no-micro", action="store_true")
     ap.add_argument("--no-sparse", action="store_true", help="skip the FAST th 20 run")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-stream batched step")
+    ap.add_argument("--no-bc2s", action="store_true", help="skip the bc2s strategy run")
     ap.add_argument("--fast-th", type=float, default=FAST_TH)
     ap.add_argument("--prof-steps", type=int, default=30,
                     help="frame steps after the timed region run with per-kernel HIP events")
@@ -549,6 +578,28 @@ def main():
         log(f"  {sparse['particle_updates_per_s']:.4g} particle-updates/s, {sparse['frames_per_s']} frames/s")
     for im in frames:
         im.free()
+    bc2s, bc2s_video = None, None
+    if not args.no_bc2s:
+        # the integer descent follows at most 2 x 10 px per frame and the
+        # reference's bc2s strategy has no coarse-to-fine prediction, so the
+        # affine video (up to ~35 px at the frame corners) loses its finest
+        # scale: the bc2s run uses the +-3 px subpixel translation generator
+        steps = min(args.steps, 120)
+        warm = min(args.warmup, 30)
+        prof = min(args.prof_steps, 20)
+        bc2s_video = cu.synth_video(SEED, NR, NC, warm + steps + prof, 0)
+        bframes = [ctx.upload(bc2s_video[f]) for f in range(len(bc2s_video))]
+        log(f"bc2s strategy, FAST th {args.fast_th:g}, detector every {BC2S_DET_FREQ} frames")
+        bc = tracker_bench(cu, ctx, bframes, args.fast_th, warm, steps, prof, 0, profile="C", det_freq=BC2S_DET_FREQ,
+                           kernel_names=BC2S_KERNELS)
+        bc2s = summary(bc)
+        bc2s["workload"] = (f"tracker<bc2s_fast_gradient_cpu> (the reference sample's strategy and settings): "
+                            f"1920x1080 u8 video translated by a subpixel shift in [-3, 3] px per frame, {NSCALES} "
+                            f"scales, FAST th {args.fast_th:g}, detector every {BC2S_DET_FREQ} frames, filtering "
+                            f"every frame, {steps} steps")
+        log(f"  {bc2s['particle_updates_per_s']:.4g} particle-updates/s, {bc2s['frames_per_s']} frames/s")
+        for im in bframes:
+            im.free()
     log("tracker from host frames")
     host_in = host_frames_bench(cu, ctx, video, args.fast_th, 20, min(args.steps, 100))
     log(f"  {host_in['frames_per_s']} frames/s from host memory")
@@ -568,6 +619,8 @@ def main():
     if not args.no_cpu:
         log("cpu baseline")
         cpu = cpu_baseline(video, args.cpu_budget, args.fast_th)
+        if bc2s is not None:
+            bc2s["cpu_baseline"] = bc2s_cpu_baseline(bc2s_video, min(args.cpu_budget, 8.0), args.fast_th)
 
     s = summary(tr)
     line = {
@@ -597,6 +650,7 @@ def main():
         "fast_th_20": sparse,
         "host_frames": host_in,
         "batch_64_streams": batch,
+        "bc2s": bc2s,
         "gaussian_pyramid_roofline": micro,
         "adjoint": adjoint,
         "cpu_baseline": cpu,

@@ -4,8 +4,7 @@
 
 namespace cuimg {
 
-template <bool CLAMP>
-__global__ __launch_bounds__(256) void bc2s_prep_kernel(bc2s_prep_args a) { bc2s_prep_tile<CLAMP>(a, blockIdx.x); }
+__global__ __launch_bounds__(256) void bc2s_prep_kernel(bc2s_prep_args a) { bc2s_prep_tile(a, blockIdx.x); }
 
 __global__ __launch_bounds__(256) void gint_kernel(gint_args a, int columns) {
   const int c = blockIdx.x * 256 + threadIdx.x, r = blockIdx.y;
@@ -74,7 +73,7 @@ static int prep(cuimg_hip_ctx* ctx, const cuimg_img* in, const cuimg_img* s1, co
   a.nc = in->ncols;
   a.tiles_x = ceil_div(a.nc, kB2W);
   const int tiles = a.tiles_x * ceil_div(a.nr, kB2H);
-  return launch(ctx, "bc2s_images", [&] { bc2s_prep_kernel<true><<<tiles, 256, 0, ctx->stream>>>(a); });
+  return launch(ctx, "bc2s_images", [&] { bc2s_prep_kernel<<<tiles, 256, 0, ctx->stream>>>(a); });
 }
 
 }  // namespace cuimg

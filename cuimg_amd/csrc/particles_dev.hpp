@@ -42,26 +42,6 @@ static __constant__ int8_t kC8C[8] = {1, 1, 1, 0, 0, -1, -1, -1};
 
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 
-// bc2s descriptor offsets: every other point of the radius-3 ring
-// (circle_r3_h[0, 2, .., 14], cuimg/neighb2d_data.h:184-189), (row, col); the
-// s2 samples are taken at twice the offsets.
-static __constant__ int8_t kB2R[8] = {-3, -2, 0, 2, 3, 2, 0, -2};
-static __constant__ int8_t kB2C[8] = {0, 2, 3, 2, 0, -2, -3, -2};
-
-// bc2s_internals::compute_feature (tracking2/bc2s_feature.hpp:119-143): bytes
-// 0..7 from s1, 8..15 from s2, packed little-endian into 4 words.
-__device__ __forceinline__ uint4 bc2s_feature_at(const bc2s_view& F, int r, int c) {
-  uint32_t w[4] = {0, 0, 0, 0};
-#pragma unroll
-  for (int i = 0; i < 8; i++) {
-    const uint32_t a = *px<uint8_t>(F.s1, F.p1, r + kB2R[i], c + kB2C[i]);
-    const uint32_t b = *px<uint8_t>(F.s2, F.p2, r + 2 * kB2R[i], c + 2 * kB2C[i]);
-    w[i >> 2] |= a << (8 * (i & 3));
-    w[2 + (i >> 2)] |= b << (8 * (i & 3));
-  }
-  return make_uint4(w[0], w[1], w[2], w[3]);
-}
-
 __device__ __forceinline__ int tzf(float v) { return (int)v; }
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
@@ -429,7 +409,7 @@ __device__ inline void insert_new_step(const detect_args& a, int bid) {
       p.age = 1;
       p.pad = 0;
       a.parts_out[j] = p;
-      if (a.feat) a.feat_out[j] = bc2s_feature_at(a.F, r, c);
+      if (a.feat) a.feat_out[j] = a.desc[(size_t)r * a.desc_pitch + c];
       *px<uint32_t>(a.sparse, a.sparse_pitch, r, c) = j;
       a.matches[n_old + kk] = j;
     } else {

@@ -62,6 +62,7 @@ struct level_res {
   // bc2s strategy: contrast (saturated u8), feature images, descriptors of
   // both particle vectors, per-cell statistics / flow / multiscale count
   cuimg_img con{}, s1{}, s2{};
+  uint4* desc = nullptr;  // per-pixel descriptors (nc per row)
   uint4* feat[2] = {nullptr, nullptr};
   int cr = 0, cc = 0;
   uint4* stats = nullptr;
@@ -320,7 +321,7 @@ __global__ __launch_bounds__(256) void step_fast_kernel(const level_job* jobs, c
 // bc2s strategy: contrast and feature images of the new frame.
 __global__ __launch_bounds__(256) void step_bprep_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
   CUIMG_JOB(prefix, njobs);
-  bc2s_prep_tile<false>(J.bc.prep[J.cnt0->steps & 1u], lb);
+  bc2s_prep_tile(J.bc.prep[J.cnt0->steps & 1u], lb);
 }
 
 // bc2s strategy: match_particles_kernel over the current vector.
@@ -337,7 +338,8 @@ __global__ __launch_bounds__(256) void step_bmatch_kernel(const level_job* jobs,
   const detect_args& d = J.det[f.cur];
   const uint32_t n = J.cnt->n;
   if (lb == 0 && threadIdx.x == 0) J.cnt->n_matched = n;
-  const uint32_t first = (uint32_t)lb * blockDim.x + threadIdx.x, stride = (uint32_t)nb * blockDim.x;
+  // 8 lanes per particle
+  const uint32_t first = ((uint32_t)lb * blockDim.x + threadIdx.x) >> 3, stride = ((uint32_t)nb * blockDim.x) >> 3;
   uint32_t live = 0, acc = 0;
   bc2s_match_body(J.bc, d.parts, J.bc.feat[f.cur], n, J.sparse, J.sparse_pitch, J.bc.dec_accept, J.bc.dec_tr,
                   J.bc.dec_dist, first, stride, &live, &acc);
@@ -354,8 +356,7 @@ __global__ __launch_bounds__(256) void step_bmatch_kernel(const level_job* jobs,
 __global__ __launch_bounds__(256) void step_fstat_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
   CUIMG_JOB(prefix, njobs);
   const step_flags f = flags_of(J);
-  bc2s_stats_body(J.bc, J.det[f.cur].parts, J.sparse, J.sparse_pitch, (uint32_t)lb * blockDim.x + threadIdx.x,
-                  (uint32_t)nb * blockDim.x);
+  bc2s_stats_strip(J.bc, J.det[f.cur].parts, J.sparse, J.sparse_pitch, lb);
 }
 
 // bc2s strategy: the statistics fused over the scales into flow and
@@ -648,6 +649,7 @@ static int alloc_level(cuimg_hip_batch* b, const cuimg_tracker_params* P, int r,
     CUIMG_TRY(image_alloc(ctx, r, c, 0, CUIMG_U8, &R.con));
     CUIMG_TRY(image_alloc(ctx, r, c, 3, CUIMG_U8, &R.s1));
     CUIMG_TRY(image_alloc(ctx, r, c, 6, CUIMG_U8, &R.s2));
+    CUIMG_TRY(dev_alloc_n(b, (size_t)r * c, &R.desc));
     for (int k = 0; k < 2; k++) CUIMG_TRY(dev_alloc_n(b, R.cap, &R.feat[k]));
     R.cr = ceil_div(r, kCell);
     R.cc = ceil_div(c, kCell);
@@ -754,8 +756,8 @@ static void build_job(cuimg_hip_batch* b, int i, int s) {
     d.nblocks = R.nblocks;
     d.feat = R.feat[cur];
     d.feat_out = R.feat[cur ^ 1];
-    d.F = bc2s_view{static_cast<const uint8_t*>(R.s1.begin), static_cast<const uint8_t*>(R.s2.begin), R.s1.pitch,
-                    R.s2.pitch, R.nr, R.nc};
+    d.desc = R.desc;
+    d.desc_pitch = R.nc;
     merge_args& m = J.mg[cur];
     m = R.merge;
     m.nr = R.nr;
@@ -811,11 +813,16 @@ static void build_bc2s(cuimg_hip_batch* b, int i, int s, level_job& J) {
     a.s2 = static_cast<uint8_t*>(R.s2.begin);
     a.s2_pitch = R.s2.pitch;
     a.s2_border = R.s2.border;
+    a.desc = R.desc;
+    a.desc_pitch = R.nc;
     a.nr = R.nr;
     a.nc = R.nc;
     a.tiles_x = ceil_div(R.nc, kB2W);
   }
-  B.F = J.det[0].F;
+  B.F = bc2s_view{static_cast<const uint8_t*>(R.s1.begin), static_cast<const uint8_t*>(R.s2.begin), R.s1.pitch,
+                  R.s2.pitch, R.nr, R.nc};
+  B.desc = R.desc;
+  B.desc_pitch = R.nc;
   B.con = static_cast<const uint8_t*>(R.con.begin);
   B.con_pitch = R.con.pitch;
   B.feat[0] = R.feat[0];
@@ -876,9 +883,9 @@ static int build_maps(cuimg_hip_batch* b) {
     const bool bc = b->P0.profile == 2;
     const long cells = (long)R.cr * R.cc;
     cnt[K_BPREP].push_back(bc ? (uint32_t)(ceil_div(R.nc, kB2W) * ceil_div(R.nr, kB2H)) : 0u);
-    cnt[K_BMATCH].push_back(bc ? (uint32_t)std::max<long>(1, std::min<long>(1024, (cap + 255) / 256)) : 0u);
-    cnt[K_FSTAT].push_back(bc ? (uint32_t)std::max<long>(1, (cells + 255) / 256) : 0u);
-    cnt[K_FUSE].push_back(cnt[K_FSTAT].back());
+    cnt[K_BMATCH].push_back(bc ? (uint32_t)std::max<long>(1, std::min<long>(2048, (8 * cap + 255) / 256)) : 0u);
+    cnt[K_FSTAT].push_back(bc ? (uint32_t)(R.cr * ceil_div(R.nc, 256)) : 0u);
+    cnt[K_FUSE].push_back(bc ? (uint32_t)std::max<long>(1, (cells + 255) / 256) : 0u);
     cnt[K_FILT].push_back(bc ? (uint32_t)std::max<long>(1, std::min<long>(512, (cap + 255) / 256)) : 0u);
   }
   // Matcher: one workgroup per kLkWaves * ppw particles of capacity, at most
@@ -1227,12 +1234,12 @@ static int clear_stream(cuimg_hip_batch* b, int i) {
 
 // ------------------------------------------- standalone detector entry points
 // Descriptors of n particles at their (truncated) positions (bc2s set_particles).
-__global__ __launch_bounds__(256) void features_at_kernel(const cuimg_particle_f* parts, uint32_t n, bc2s_view F,
-                                                          uint4* feat) {
+__global__ __launch_bounds__(256) void features_at_kernel(const cuimg_particle_f* parts, uint32_t n, const uint4* D,
+                                                          int nr, int nc, uint4* feat) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const int r = (int)parts[i].pos_r, c = (int)parts[i].pos_c;
-  feat[i] = (r >= 0 && r < F.nr && c >= 0 && c < F.nc) ? bc2s_feature_at(F, r, c) : make_uint4(0, 0, 0, 0);
+  feat[i] = (r >= 0 && r < nr && c >= 0 && c < nc) ? D[(size_t)r * nc + c] : make_uint4(0, 0, 0, 0);
 }
 
 __global__ __launch_bounds__(256) void mask_fill_kernel(uint8_t* m, int pitch, int nr, int nc) {
@@ -1568,10 +1575,9 @@ int cuimg_hip_tracker_set_particles(cuimg_hip_tracker* t, int scale, const cuimg
     if (c0.steps == 0) {
       CUIMG_HIP(b->ctx, hipMemsetAsync(R.feat[c.cur & 1], 0, sizeof(uint4) * n, b->ctx->stream));
     } else {
-      const bc2s_view F = b->hj[j].bc.F;
       CUIMG_TRY(launch(b->ctx, "features_at", [&] {
-        features_at_kernel<<<ceil_div((int)n, 256), 256, 0, b->ctx->stream>>>(R.parts[c.cur & 1], n, F,
-                                                                               R.feat[c.cur & 1]);
+        features_at_kernel<<<ceil_div((int)n, 256), 256, 0, b->ctx->stream>>>(R.parts[c.cur & 1], n, R.desc, R.nr,
+                                                                               R.nc, R.feat[c.cur & 1]);
       }));
     }
   }

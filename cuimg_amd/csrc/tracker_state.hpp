@@ -119,10 +119,12 @@ struct detect_args {
   uint32_t cap;
   int nblocks;                  // compaction blocks covering cap
   // bc2s strategy: 16-byte descriptors follow their particles through the
-  // compaction (feat -> feat_out); new particles take theirs from F
+  // compaction (feat -> feat_out); new particles take theirs from the
+  // scale's per-pixel descriptor image
   uint4* feat;                  // nullptr: no descriptors
   uint4* feat_out;
-  bc2s_view F;
+  const uint4* desc;
+  int desc_pitch;               // in uint4
 };
 
 // A particle with at least one eligible merge target: b[k] is the c8
