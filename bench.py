@@ -193,10 +193,12 @@ def batch_workload(cu):
     return shapes, params, parts
 
 
-def batch_bench(cu, ctx):
+def batch_bench(cu, ctx, tuning=None, sequential=True):
     """One frame step of 64 streams as one graph replay of the batched kernels,
-    against the same 64 streams stepped one tracker after the other with plain
-    launches; 7 timed steps (frames 1..7) after the seeded placement."""
+    against the same 64 streams stepped one tracker after the other, with
+    plain launches (the baseline of the speedup) and with each tracker
+    replaying its own captured graph; 7 timed steps (frames 1..7) after the
+    seeded placement.  `tuning` = (lk_ppw, blur_th) skips the autotuner."""
     shapes, params, parts = batch_workload(cu)
     t0 = time.time()
     frames = [[None] * BATCH_STREAMS for _ in range(BATCH_FRAMES)]
@@ -223,18 +225,24 @@ def batch_bench(cu, ctx):
         return time.perf_counter() - t1
 
     batch = cu.Batch(ctx, shapes, params)
-    best, tune = batch.autotune(run_batch)
+    if tuning is None:
+        log("batch: autotune")
+        best, tune = batch.autotune(run_batch)
+    else:
+        batch.set_tuning(*tuning)
+        tune = {}
     tb = []
-    for _ in range(3):
+    for k in range(3):
+        log(f"batch: batched run {k + 1} of 3")
         batch.clear()
         tb.append(run_batch(batch))
     live = sum(batch.tracker(i).work(s)[0] for i in range(BATCH_STREAMS) for s in range(params[i].nscales))
 
     solo = [cu.Tracker(ctx, h, w, params[i]) for i, (h, w) in enumerate(shapes)]
 
-    def run_solo():
+    def run_solo(graph):
         for i, t in enumerate(solo):
-            t.set_graph(False)
+            t.set_graph(graph)
             t.clear()
             t.run(frames[0][i])
         place(solo)
@@ -245,10 +253,16 @@ def batch_bench(cu, ctx):
         ctx.sync()
         return time.perf_counter() - t1
 
-    run_solo()  # warm-up
-    ts = min(run_solo() for _ in range(2))
+    tg = ts = float("nan")
+    if sequential:
+        log("batch: sequential trackers, graph replay")
+        run_solo(True)  # warm-up and capture
+        tg = min(run_solo(True) for _ in range(2))
+        log("batch: sequential trackers, plain launches")
+        run_solo(False)  # warm-up
+        ts = min(run_solo(False) for _ in range(2))
     same = 0
-    for i in range(BATCH_STREAMS):
+    for i in range(BATCH_STREAMS if sequential else 0):
         ok = True
         for s in range(params[i].nscales):
             a, b = batch.tracker(i).particles(s), solo[i].particles(s)
@@ -261,8 +275,10 @@ def batch_bench(cu, ctx):
            "pixels_per_frame": sum(h * w for h, w in shapes),
            "particles_scale0": sum(len(p[0]) for p in parts),
            "batch_ms_per_step": round(t_b / steps * 1000, 3),
-           "sequential_ms_per_step": round(ts / steps * 1000, 3),
-           "speedup": round(ts / t_b, 2),
+           "sequential_ms_per_step": round(ts / steps * 1000, 3) if sequential else None,
+           "speedup": round(ts / t_b, 2) if sequential else None,
+           "sequential_graph_ms_per_step": round(tg / steps * 1000, 3) if sequential else None,
+           "speedup_vs_graph_replay": round(tg / t_b, 2) if sequential else None,
            "stream_frames_per_s": round(BATCH_STREAMS * steps / t_b, 1),
            "particle_updates_per_s": round(live / tb[-1], 1),
            "streams_identical_to_own_tracker": f"{same}/{BATCH_STREAMS}",
@@ -366,6 +382,7 @@ def gaussian_microbench(cu, ctx, reps=40):
         outs = [ctx.image(shape[0], shape[1], dt) for _ in range(4)]
         nbytes = shape[0] * shape[1] * es * 2
         for sigma in (1, 2, 4):
+            log(f"gaussian {kind} sigma {sigma}")
             K = 3 * sigma
             run = lambda i: ctx.gaussian(ins[i % 4], outs[i % 4], sigma, K)  # noqa: E731
             us = span_us(ctx, run, reps)
@@ -507,12 +524,22 @@ VALU_PROFILE = os.path.join(ROOT, "profiles", "lk_track_1080p_th8_valu.txt")
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 
 
+# FP32 peak of MI355X (dense, non-packed), for the matcher's flop fraction
+FP32_PEAK_TFLOPS = 157.3
+# flops of one 7x7 level solve with at most 5 Gauss-Newton steps, counted as
+# 49 x (10 Sobel + 8 bilinear + 6 outer product) + 5 x 49 x (8 + 1 + 4) + 10
+LK_FLOPS_PER_SOLVE = 49 * (10 + 8 + 6) + 5 * 49 * (8 + 1 + 4) + 10
+
+
 def measured_valu(path=VALU_PROFILE):
+    """VALU wave instructions per level solve from the `valu_instr_per_level_solve
+    = X` line of the counter summary (a separate rocprofv3 run of the same
+    workload), None when there is none."""
     try:
         with open(path) as f:
             for line in f:
-                if line.strip().startswith("valu_instr_per_launch"):
-                    return int(line.split("=")[1].strip())
+                if line.strip().startswith("valu_instr_per_level_solve"):
+                    return float(line.split("=")[1].strip())
     except OSError:
         pass
     return None
@@ -527,19 +554,31 @@ def lk_roofline(tr):
     avg_parts = [u / prof["frames"] for u in prof["per_scale"]]
     bytes_per_launch = lk_algorithmic_bytes(tr["shapes"], avg_parts)
     achieved = bytes_per_launch / (k["avg_us"] * 1e-6) / 1e9
-    valu = measured_valu()
+    nscales = len(tr["shapes"])
+    solves = sum(n * (nscales - s) for s, n in enumerate(avg_parts))
+    flop_rate = solves * LK_FLOPS_PER_SOLVE / (k["avg_us"] * 1e-6) / 1e12
+    fp32 = {"achieved_tflops": round(flop_rate, 3), "peak_tflops": FP32_PEAK_TFLOPS,
+            "frac": round(flop_rate / FP32_PEAK_TFLOPS, 5), "level_solves_per_launch": round(solves, 1),
+            "flops_per_level_solve": LK_FLOPS_PER_SOLVE,
+            "model": "upper bound: every solve counted with 5 Gauss-Newton steps"}
+    per_solve = measured_valu()
     valu_entry = None
-    if valu:
-        rate = valu / (k["avg_us"] * 1e-6)
-        valu_entry = {"instr_per_launch": valu, "achieved_ginstr_s": round(rate / 1e9, 1),
+    if per_solve:
+        instr = per_solve * solves
+        rate = instr / (k["avg_us"] * 1e-6)
+        valu_entry = {"instr_per_level_solve": per_solve, "instr_per_launch_est": int(instr),
+                      "achieved_ginstr_s": round(rate / 1e9, 1),
                       "peak_ginstr_s": round(VALU_ISSUE_PEAK / 1e9, 1), "frac": round(rate / VALU_ISSUE_PEAK, 4),
-                      "model": "one wave64 VALU instruction per SIMD every 4 cycles, 256 CUs x 4 SIMDs at 2.4 GHz; "
-                               "instructions per launch from rocprofv3 SQ_INSTS_VALU"}
+                      "source": "instructions per level solve from a separate rocprofv3 SQ_INSTS_VALU run of the same "
+                                "workload (" + os.path.relpath(VALU_PROFILE, ROOT) + "), times this run's level "
+                                "solves per launch, over this run's live launch time",
+                      "model": "one wave64 VALU instruction per SIMD every 4 cycles, 256 CUs x 4 SIMDs at 2.4 GHz"}
     return {"kernel": "lk_track", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(),
-            "bytes_per_launch": int(bytes_per_launch), "avg_us": k["avg_us"], "valu": valu_entry,
-            "note": "the matcher is bound by VALU instruction issue, not HBM (see valu): bytes = both FP32 frames "
-                    "of every scale once + 40 B per particle matched; traffic from rocprofv3 FETCH_SIZE + WRITE_SIZE"}
+            "bytes_per_launch": int(bytes_per_launch), "avg_us": k["avg_us"], "fp32": fp32, "valu": valu_entry,
+            "note": "the matcher is bound by VALU instruction issue, not HBM or FP32 throughput (see valu, fp32): "
+                    "bytes = both FP32 frames of every scale once + 40 B per particle matched; traffic from "
+                    "rocprofv3 FETCH_SIZE + WRITE_SIZE"}
 
 
 def main():
@@ -556,14 +595,31 @@ def main():
     ap.add_argument("--fast-th", type=float, default=FAST_TH)
     ap.add_argument("--prof-steps", type=int, default=30,
                     help="frame steps after the timed region run with per-kernel HIP events")
+    ap.add_argument("--batch-tuning", type=str, default=None,
+                    help="lk_ppw,blur_th for the 64-stream batch instead of the autotuner (counter runs)")
+    ap.add_argument("--batch-no-sequential", action="store_true",
+                    help="with --only batch: skip the sequential per-stream trackers (counter runs)")
+    ap.add_argument("--only", choices=("micro", "batch"), default=None,
+                    help="run only the Gaussian / pyramid microbenchmark or only the 64-stream batch and print "
+                         "its JSON (profiling runs; not the benchmark line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
         sys.exit(2)
+    tuning = tuple(int(x) for x in args.batch_tuning.split(",")) if args.batch_tuning else None
 
     import cuimg_amd as cu
     ctx = cu.Context(0)
+    if args.only == "micro":
+        print(json.dumps({"gaussian_pyramid_roofline": gaussian_microbench(cu, ctx)}), flush=True)
+        ctx.close()
+        return
+    if args.only == "batch":
+        print(json.dumps({"batch_64_streams": batch_bench(cu, ctx, tuning, not args.batch_no_sequential)}),
+              flush=True)
+        ctx.close()
+        return
     pct_steps = 50 if args.steps >= 50 else 0
     video, frames = upload_video(cu, ctx, args.warmup + args.steps + pct_steps + args.prof_steps)
     log(f"tracker benchmark, FAST th {args.fast_th:g}")
@@ -606,7 +662,7 @@ def main():
     batch = None
     if not args.no_batch:
         log("batched 64-stream step")
-        batch = batch_bench(cu, ctx)
+        batch = batch_bench(cu, ctx, tuning)
         log(f"  batch {batch['batch_ms_per_step']} ms/step vs sequential {batch['sequential_ms_per_step']} ms/step "
             f"({batch['speedup']}x), identical {batch['streams_identical_to_own_tracker']}")
     micro = [] if args.no_micro else gaussian_microbench(cu, ctx)
