@@ -50,7 +50,7 @@ def lk_algorithmic_bytes(level_shapes, nparts):
     return img + 40 * sum(nparts)
 
 
-KERNELS = ("pyramid_build", "gaussian_sep", "gradient", "fills", "lk_track", "particle_mask", "fast_saliency",
+KERNELS = ("pyramid_build", "blur_gradient", "fills", "lk_track", "particle_mask", "fast_saliency",
            "detect_count", "scatter_insert", "merge_scan", "merge_round1", "merge_resolve")
 BC2S_KERNELS = ("pyramid_build", "bc2s_images", "fast_saliency", "fills", "bc2s_match", "flow_stats", "flow_fusion",
                 "particle_mask", "detect_count", "scatter_insert", "filter", "merge_scan", "merge_round1",

@@ -546,71 +546,6 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
 
 
 
-// Sobel/8 gradient of an FP32 level (pyrlk_cpu::match_particles,
-// pyrlk_tracker.hpp:909-938, taken of the blurred frame): [0] = d/drow,
-// [1] = d/dcol, the terms added left to right as written there, zero on the
-// 1-pixel outer ring.  A thread owns 4 columns (c .. c+3, c a multiple of 4)
-// and slides down rows [r0, r1) keeping three input rows in registers: one
-// 16-byte load and two scalar neighbours per new row.  Rows are 16-byte
-// aligned with pitches a multiple of 16 bytes (reads up to the next multiple
-// of 4 columns stay in the row's padding).
-struct sobel_row4 {
-  float4 v;
-  float l, r;  // columns c - 1 and c + 4 (0 outside the image)
-};
-__device__ __forceinline__ sobel_row4 sobel_load_row(const float* in, int pitch, int row, int c, int nc) {
-  const float* p = px<float>(in, pitch, row, c);
-  sobel_row4 x;
-  x.v = *reinterpret_cast<const float4*>(p);
-  x.l = c > 0 ? p[-1] : 0.f;
-  x.r = c + 4 < nc ? p[4] : 0.f;
-  return x;
-}
-__device__ __forceinline__ float sobel_at(const sobel_row4& x, int j) {
-  return j < 0 ? x.l : j == 0 ? x.v.x : j == 1 ? x.v.y : j == 2 ? x.v.z : j == 3 ? x.v.w : x.r;
-}
-__device__ inline void sobel_rows_body(const float* in, int in_pitch, float2* out, int out_pitch, int nr, int nc,
-                                       int r0, int r1, int c) {
-  if (c >= nc || r0 >= r1) return;
-  sobel_row4 up, mid, dn;
-  up = sobel_load_row(in, in_pitch, max(r0 - 1, 0), c, nc);
-  mid = sobel_load_row(in, in_pitch, r0, c, nc);
-  for (int r = r0; r < r1; r++) {
-    dn = sobel_load_row(in, in_pitch, min(r + 1, nr - 1), c, nc);
-    float2 g[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int cc = c + j;
-      g[j] = make_float2(0.f, 0.f);
-      if (r > 0 && r < nr - 1 && cc > 0 && cc < nc - 1) {
-        const float pmm = sobel_at(up, j - 1), pm0 = sobel_at(up, j), pmp = sobel_at(up, j + 1);
-        const float p0m = sobel_at(mid, j - 1), p0p = sobel_at(mid, j + 1);
-        const float ppm = sobel_at(dn, j - 1), pp0 = sobel_at(dn, j), ppp = sobel_at(dn, j + 1);
-        float gr = add_rn(ppm, mul_rn(2.f, pp0));
-        gr = add_rn(gr, ppp);
-        gr = sub_rn(gr, pmm);
-        gr = sub_rn(gr, mul_rn(2.f, pm0));
-        gr = sub_rn(gr, pmp);
-        float gc = add_rn(pmp, mul_rn(2.f, p0p));
-        gc = add_rn(gc, ppp);
-        gc = sub_rn(gc, pmm);
-        gc = sub_rn(gc, mul_rn(2.f, p0m));
-        gc = sub_rn(gc, ppm);
-        g[j] = make_float2(mul_rn(gr, 0.125f), mul_rn(gc, 0.125f));
-      }
-    }
-    float2* o = px<float2>(out, out_pitch, r, c);
-    if (c + 4 <= nc) {
-      reinterpret_cast<float4*>(o)[0] = make_float4(g[0].x, g[0].y, g[1].x, g[1].y);
-      reinterpret_cast<float4*>(o)[1] = make_float4(g[2].x, g[2].y, g[3].x, g[3].y);
-    } else {
-      for (int j = 0; j < nc - c; j++) o[j] = g[j];
-    }
-    up = mid;
-    mid = dn;
-  }
-}
-
 // ------------------------------------------------------------ fast path
 // Windows up to 7x7 on FP32 frames whose gradient image is given (a frame
 // step computes it once per level and frame): the same operations as

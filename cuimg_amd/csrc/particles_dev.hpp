@@ -37,9 +37,6 @@
 
 namespace cuimg {
 
-static __constant__ int8_t kC8R[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
-static __constant__ int8_t kC8C[8] = {1, 1, 1, 0, 0, -1, -1, -1};
-
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 
 __device__ __forceinline__ int tzf(float v) { return (int)v; }
@@ -214,6 +211,82 @@ __device__ inline uint8_t nms_box_rows(const detect_args& a, float fast_th, int 
   return skip ? (uint8_t)0xFF : nms_code(a, fast_th, pr, pc, B, vmax, best);
 }
 
+// Box side 3 (the tracker's default): a thread takes four neighbouring boxes,
+// columns 12q + 1 .. 12q + 12 of the box row, and reads the 16 columns
+// 12q .. 12q + 15 of each of its 3 rows with one 16-byte load of saliency and
+// four of the sparse image (pixel (0,0) and the pitch of both 16-byte aligned,
+// checked by the caller).  Quads
+// whose 16 columns do not all lie inside the domain take the per-box path.
+// Rows are read up to nr - 1 (row nr, reached when 3 nbr = nr, is masked).
+// Pixels outside the domain read as empty and 0, as in nms_box_fixed.  Codes
+// and sub-row counts are those of the per-box path.
+__device__ inline void nms_row3_body(const detect_args& a, float fast_th, int bi, uint32_t* cnts) {
+  const int nq = (a.nbc + 3) >> 2;
+  for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+    const int pr = bi * 3 + 1, c0 = q * 12;
+    if (c0 + 16 > a.nc) {
+      for (int bj = 4 * q; bj < min(4 * q + 4, a.nbc); bj++) {
+        const uint8_t code = nms_box_fixed<3>(a, fast_th, pr, bj * 3 + 1);
+        a.boxcode[(size_t)bi * a.nbc + bj] = code;
+        if (code != 0xFF) atomicAdd(&cnts[code / 3], 1u);
+      }
+      continue;
+    }
+    uint32_t sp[3][16];
+    uint32_t sv[3][4];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      const int rl = min(pr + r, a.nr - 1);  // a row past the domain is masked below
+      const uint4* s4 = reinterpret_cast<const uint4*>(px<uint32_t>(a.sparse, a.sparse_pitch, rl, c0));
+      const uint4 v = *reinterpret_cast<const uint4*>(px<uint8_t>(a.sal, a.sal_pitch, rl, c0));
+      sv[r][0] = v.x;
+      sv[r][1] = v.y;
+      sv[r][2] = v.z;
+      sv[r][3] = v.w;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const uint4 w = s4[k];
+        sp[r][4 * k] = w.x;
+        sp[r][4 * k + 1] = w.y;
+        sp[r][4 * k + 2] = w.z;
+        sp[r][4 * k + 3] = w.w;
+      }
+    }
+    uint8_t codes[4];
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      const int pc = c0 + 1 + 3 * b;
+      bool skip = false;
+      float vmax = 0.f;
+      int best = 0;
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const int j = 1 + 3 * b + c;  // column within the 16 loaded
+          const bool in = pr + r < a.nr && pc + c < a.nc;
+          skip |= in && sp[r][j] != kEmpty;
+          const float v = in ? (float)((sv[r][j >> 2] >> (8 * (j & 3))) & 0xFFu) : 0.f;
+          if (r == 0 && c == 0) {
+            vmax = v;
+          } else if (vmax < v) {
+            vmax = v;
+            best = r * 3 + c;
+          }
+        }
+      codes[b] = skip ? (uint8_t)0xFF : nms_code(a, fast_th, pr, pc, 3, vmax, best);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      const int bj = 4 * q + b;
+      if (bj < a.nbc) {
+        a.boxcode[(size_t)bi * a.nbc + bj] = codes[b];
+        if (codes[b] != 0xFF) atomicAdd(&cnts[codes[b] / 3], 1u);
+      }
+    }
+  }
+}
+
 // One workgroup per box row bi; each thread one box at a time.  The row's
 // points are counted per sub-row (row offset dr) into subcnt.
 __device__ inline void nms_row_body(const detect_args& a, float fast_th, int bi) {
@@ -221,6 +294,14 @@ __device__ inline void nms_row_body(const detect_args& a, float fast_th, int bi)
   const int B = a.box;
   if (threadIdx.x < B) cnts[threadIdx.x] = 0;
   __syncthreads();
+  const bool aligned16 = (((uintptr_t)a.sparse | (uintptr_t)a.sal | (uintptr_t)a.sparse_pitch |
+                            (uintptr_t)a.sal_pitch) & 15) == 0;
+  if (B == 3 && aligned16) {
+    nms_row3_body(a, fast_th, bi, cnts);
+    __syncthreads();
+    if (threadIdx.x < B) a.subcnt[bi * B + threadIdx.x] = cnts[threadIdx.x];
+    return;
+  }
   for (int bj = threadIdx.x; bj < a.nbc; bj += blockDim.x) {
     const int pr = bi * B + 1, pc = bj * B + 1;
     uint8_t code;
@@ -241,15 +322,16 @@ __device__ inline void nms_row_body(const detect_args& a, float fast_th, int bi)
 }
 
 // Compaction count: flags (alive, owner of its sparse pixel) and alive count
-// per 1024-particle block (256 threads).  Ownership is read before any index
-// is rewritten.
+// per 1024-particle block (256 threads; thread t takes particles t, t + 256,
+// t + 512, t + 768 of the block, so a wave's loads are contiguous).  Ownership
+// is read before any index is rewritten.
 __device__ inline void compact_count_body(const detect_args& a, int bid) {
   __shared__ uint32_t tmp[64];
   const uint32_t n = a.cnt->n;
-  const uint32_t base = bid * kCompactBlock + threadIdx.x * 4;
+  const uint32_t base = bid * kCompactBlock + threadIdx.x;
   uint32_t c = 0;
   for (int k = 0; k < 4; k++) {
-    const uint32_t i = base + k;
+    const uint32_t i = base + k * (kCompactBlock / 4);
     if (i < n) {
       const cuimg_particle_f p = a.parts[i];
       uint8_t f = 0;
@@ -307,31 +389,29 @@ __device__ inline void compact_scan_body(const detect_args& a) {
 }
 
 // Compaction scatter of 1024-particle block bid whose first kept particle goes
-// to index off0.
+// to index off0: four rounds of 256 consecutive particles (thread t takes
+// particle t of the round, so a wave's loads and stores are contiguous), each
+// an exclusive scan of the kept flags after the previous rounds' total.
 __device__ inline void compact_scatter_at(const detect_args& a, int bid, uint32_t off0, uint32_t n_old) {
   __shared__ uint32_t tmp[64];
-  const uint32_t base = bid * kCompactBlock + threadIdx.x * 4;
-  uint32_t c = 0;
-  uint8_t f[4];
+  uint32_t off = off0;
   for (int k = 0; k < 4; k++) {
-    const uint32_t i = base + k;
-    f[k] = i < n_old ? a.flags[i] : 0;
-    c += f[k] & 1;
-  }
-  uint32_t off = off0 + block_excl_scan(c, tmp, nullptr);
-  for (int k = 0; k < 4; k++) {
-    const uint32_t i = base + k;
-    if (i >= n_old) break;
-    if (f[k] & 1) {
-      const cuimg_particle_f p = a.parts[i];
-      a.parts_out[off] = p;
-      if (a.feat) a.feat_out[off] = a.feat[i];
-      a.matches[i] = off;
-      if (f[k] & 2) *px<uint32_t>(a.sparse, a.sparse_pitch, tzf(p.pos_r), tzf(p.pos_c)) = off;
-      off++;
-    } else {
-      a.matches[i] = kEmpty;
+    const uint32_t i = bid * kCompactBlock + k * (kCompactBlock / 4) + threadIdx.x;
+    const uint8_t f = i < n_old ? a.flags[i] : 0;
+    uint32_t total;
+    const uint32_t o = off + block_excl_scan(f & 1u, tmp, &total);
+    if (i < n_old) {
+      if (f & 1) {
+        const cuimg_particle_f p = a.parts[i];
+        a.parts_out[o] = p;
+        if (a.feat) a.feat_out[o] = a.feat[i];
+        a.matches[i] = o;
+        if (f & 2) *px<uint32_t>(a.sparse, a.sparse_pitch, tzf(p.pos_r), tzf(p.pos_c)) = o;
+      } else {
+        a.matches[i] = kEmpty;
+      }
     }
+    off += total;
   }
 }
 
@@ -499,18 +579,40 @@ __device__ inline void merge_scan_body(const merge_args& a, int bid, int nb) {
     if (i < n) {
       const cuimg_particle_f p = a.parts[i];
       if (p.age != 0) {
+        // c8 neighbours in the CPU table order (c8_h, neighb2d_data.h:80-85);
+        // all eight sparse reads are issued first (clamped addresses, pixels
+        // outside the domain read as empty), then the neighbours' particles
+        constexpr int dr8[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
+        constexpr int dc8[8] = {1, 1, 1, 0, 0, -1, -1, -1};
         const int r = tzf(p.pos_r), c = tzf(p.pos_c);
+        uint32_t bs[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-          const uint32_t b = sparse_get(a.sparse, a.sparse_pitch, a.nr, a.nc, r + kC8R[k], c + kC8C[k]);
-          if (b == kEmpty || b >= n) continue;  // indices past the vector are treated as empty
-          const cuimg_particle_f q = a.parts[b];
-          if ((int)q.age > (int)p.age + 2 &&
-              norml2_rn(__fsub_rn(p.speed_r, q.speed_r), __fsub_rn(p.speed_c, q.speed_c)) < 2.f) {
-            rec.mask |= 1u << k;
-            rec.b[k] = (int32_t)b;
+          const int rr = r + dr8[k], cc = c + dc8[k];
+          const bool in = rr >= 0 && rr < a.nr && cc >= 0 && cc < a.nc;
+          const uint32_t v = *px<uint32_t>(a.sparse, a.sparse_pitch, clampi(rr, 0, a.nr - 1), clampi(cc, 0, a.nc - 1));
+          bs[k] = in && v < n ? v : kEmpty;  // indices past the vector are treated as empty
+        }
+        uint32_t qa[8];
+        float qr[8], qc[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          qa[k] = 0;
+          qr[k] = qc[k] = 0.f;
+          if (bs[k] != kEmpty) {
+            const cuimg_particle_f q = a.parts[bs[k]];
+            qa[k] = q.age;
+            qr[k] = q.speed_r;
+            qc[k] = q.speed_c;
           }
         }
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+          if (bs[k] != kEmpty && (int)qa[k] > (int)p.age + 2 &&
+              norml2_rn(__fsub_rn(p.speed_r, qr[k]), __fsub_rn(p.speed_c, qc[k])) < 2.f) {
+            rec.mask |= 1u << k;
+            rec.b[k] = (int32_t)bs[k];
+          }
       }
     }
     const bool act = rec.mask != 0;

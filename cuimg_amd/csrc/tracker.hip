@@ -28,6 +28,7 @@
 // dense particle vectors (compaction ping-pongs between them), the matches
 // vector, and per-particle scratch for compaction and the merge.
 #include "bc2s_dev.hpp"
+#include "blur_grad_dev.hpp"
 #include "fast_dev.hpp"
 #include "gaussian_dev.hpp"
 #include "lk_dev.hpp"
@@ -76,15 +77,13 @@ struct job_params {
   int bc2s;  // generic_strategy with the bc2s descriptor (else pyrlk)
 };
 
-// Sobel/8 gradient of a blurred level: rows of `in` into `out`.
+// Sobel/8 gradient of a blurred level `in` into `out` (its own pass or with the blur).
 struct grad_job {
   const float* in;
   float2* out;
   int in_pitch, out_pitch;
   int nr, nc;
 };
-constexpr int kGradRows = 2;     // rows per gradient workgroup
-constexpr int kGradCols = 1024;  // columns per gradient workgroup (4 per thread)
 
 struct level_job {
   int stream, s;
@@ -120,7 +119,7 @@ struct frame_in {
 };
 
 enum {
-  K_PYR, K_BLUR, K_GRAD, K_FILL, K_LK, K_PMASK, K_FAST, K_DETC, K_SCATI, K_MSCAN, K_MR1,
+  K_PYR, K_BLUR, K_FILL, K_LK, K_PMASK, K_FAST, K_DETC, K_SCATI, K_MSCAN, K_MR1,
   K_BPREP, K_BMATCH, K_FSTAT, K_FUSE, K_FILT, K_NKINDS
 };
 constexpr int kRing = 8;
@@ -255,23 +254,17 @@ __global__ __launch_bounds__(kPyrThreads) void step_pyramid_kernel(const stream_
   pyramid_tile<uint8_t>(a, jb.lb);
 }
 
-// Blurred-frame profile: sigma / K Gaussian of every level of the new frame.
+// Blurred-frame profile: sigma / K Gaussian of every level of the new frame
+// and the Sobel/8 gradient of the blurred level (the matcher reads it as the
+// previous frame's gradient next step), in one pass.
 template <int K>
-__global__ __launch_bounds__(64) void step_blur_kernel(const level_job* jobs, const uint32_t* prefix, int njobs,
-                                                       coef_pack cp) {
+__global__ __launch_bounds__(64) void step_blurgrad_kernel(const level_job* jobs, const uint32_t* prefix, int njobs,
+                                                           coef_pack cp) {
   CUIMG_JOB(prefix, njobs);
-  const gauss_job G = uni(J.blur[J.cnt0->steps & 1u]);
-  gauss_tile<uint8_t, float, 4, K>(G, lb % G.tiles_x, lb / G.tiles_x, cp);
-}
-
-// Blurred-frame profile: the Sobel/8 gradient of every blurred level of the
-// new frame (the matcher reads it as the previous frame's gradient next step).
-__global__ __launch_bounds__(256) void step_grad_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
-  CUIMG_JOB(prefix, njobs);
-  const grad_job g = uni(J.grad[J.cnt0->steps & 1u]);
-  const int tx = (g.nc + kGradCols - 1) / kGradCols;
-  const int r0 = (lb / tx) * kGradRows, c = (lb % tx) * kGradCols + 4 * (int)threadIdx.x;
-  sobel_rows_body(g.in, g.in_pitch, g.out, g.out_pitch, g.nr, g.nc, r0, min(r0 + kGradRows, g.nr), c);
+  const uint32_t slot = J.cnt0->steps & 1u;
+  const gauss_job G = uni(J.blur[slot]);
+  const grad_job g = uni(J.grad[slot]);
+  gauss_sobel_tile<K>(G, g.out, g.out_pitch, lb % G.tiles_x, lb / G.tiles_x, cp);
 }
 
 // before_matching (sparse <- -1, particle_container.hpp:187-193) and the
@@ -314,8 +307,10 @@ __global__ __launch_bounds__(256) void step_fast_kernel(const level_job* jobs, c
   const step_flags f = flags_of(J);
   if (!f.detect) return;
   const bool bc = J.prm.bc2s;
-  fast_tile_body<false>(J.u8[f.slot], J.u8_pitch, bc ? J.bc.con : nullptr, bc ? J.bc.con_pitch : 0, J.sal,
-                        J.sal_pitch, J.nr, J.nc, lb, J.fast_tiles_x, J.prm.fast_n, (int)J.prm.fast_th, bc ? 5 : 1);
+  // the tracker's images: clamp-filled border 3, 128 bytes before column 0,
+  // rows 128-byte aligned; fast_th >= 0 (checked on the host)
+  fast_tile4_body(J.u8[f.slot], J.u8_pitch, bc ? J.bc.con : nullptr, bc ? J.bc.con_pitch : 0, J.sal, J.sal_pitch,
+                  J.nr, J.nc, lb, J.fast_tiles_x, J.prm.fast_n, (int)J.prm.fast_th, bc ? 5 : 1);
 }
 
 // bc2s strategy: contrast and feature images of the new frame.
@@ -872,7 +867,6 @@ static int build_maps(cuimg_hip_batch* b) {
     const level_job& J = b->hj[j];
     const long cap = R.cap;
     cnt[K_BLUR].push_back(b->P0.profile == 1 ? (uint32_t)(ceil_div(R.nc, 256) * ceil_div(R.nr, th)) : 0u);
-    cnt[K_GRAD].push_back(b->P0.profile == 1 ? (uint32_t)(ceil_div(R.nr, kGradRows) * ceil_div(R.nc, kGradCols)) : 0u);
     cnt[K_FILL].push_back((uint32_t)J.fill_sparse_blocks);
     cnt[K_PMASK].push_back((uint32_t)std::max<long>(1, std::min<long>(512, (cap + 255) / 256)));
     cnt[K_FAST].push_back((uint32_t)(J.fast_tiles_x * ceil_div(R.nr, kFTH)));
@@ -1021,7 +1015,7 @@ static void launch_blur(cuimg_hip_batch* b) {
   hipStream_t st = b->ctx->stream;
   switch (b->P0.blur_K) {
 #define CUIMG_K(k) \
-  case k: step_blur_kernel<k><<<grid, 64, 0, st>>>(b->dj, b->prefix_d[K_BLUR], b->njobs, b->cp); break;
+  case k: step_blurgrad_kernel<k><<<grid, 64, 0, st>>>(b->dj, b->prefix_d[K_BLUR], b->njobs, b->cp); break;
     CUIMG_K(1) CUIMG_K(2) CUIMG_K(3) CUIMG_K(4) CUIMG_K(5) CUIMG_K(6) CUIMG_K(7) CUIMG_K(8)
     CUIMG_K(9) CUIMG_K(10) CUIMG_K(11) CUIMG_K(12) CUIMG_K(13) CUIMG_K(14) CUIMG_K(15) CUIMG_K(16)
 #undef CUIMG_K
@@ -1091,13 +1085,12 @@ static int enqueue_step(cuimg_hip_batch* b, bool capturing, bool with_pyramid = 
     CUIMG_HIP(ctx, hipStreamWaitEvent(b->aux, b->fork_ev, 0));
   }
   CUIMG_TRY(launch(ctx, "fast_saliency", [&] { step_fast_kernel<<<b->total[K_FAST], 256, 0, fst>>>(J, pf[K_FAST], nj); }));
+  // the sparse reset only has to precede the matcher: it runs on FAST's branch
+  CUIMG_TRY(launch(ctx, "fills", [&] { step_fill_kernel<<<b->total[K_FILL], 256, 0, fst>>>(J, pf[K_FILL], nj); }));
   if (fork) CUIMG_HIP(ctx, hipEventRecord(b->join_ev, b->aux));
-  if (b->total[K_BLUR]) CUIMG_TRY(launch(ctx, "gaussian_sep", [&] { launch_blur(b); }));
-  if (b->total[K_GRAD])
-    CUIMG_TRY(launch(ctx, "gradient", [&] { step_grad_kernel<<<b->total[K_GRAD], 256, 0, st>>>(J, pf[K_GRAD], nj); }));
-  CUIMG_TRY(launch(ctx, "fills", [&] { step_fill_kernel<<<b->total[K_FILL], 256, 0, st>>>(J, pf[K_FILL], nj); }));
-  // join before the matcher: FAST takes about as long as blur + fill, and a
-  // FAST still running would hold CU slots the matcher's workgroups wait for
+  if (b->total[K_BLUR]) CUIMG_TRY(launch(ctx, "blur_gradient", [&] { launch_blur(b); }));
+  // join before the matcher (the fill's sparse image is the matcher's); a FAST
+  // still running would hold CU slots the matcher's workgroups wait for
   if (fork) CUIMG_HIP(ctx, hipStreamWaitEvent(st, b->join_ev, 0));
   const bool fast = b->P0.profile == 1 && b->P0.lk_ws == 7 && b->P0.lk_top_n >= 49;
   CUIMG_TRY(launch(ctx, "lk_track", [&] {
