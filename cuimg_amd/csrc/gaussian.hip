@@ -23,6 +23,9 @@ static int dispatch_k(cuimg_hip_ctx* ctx, const gauss_jobs& jobs, int blocks, co
   return set_error(ctx, CUIMG_E_UNSUPPORTED, "gaussian: half width %d not supported (1..%d)", K, kMaxK);
 }
 
+int convolve_sep_mfma(cuimg_hip_ctx* ctx, gauss_jobs jobs, const coef_pack& cp, int K, bool rgba, int th_override);
+constexpr int kMfmaMinK = 9;
+
 static int pick_th(int nr, int tiles_x, int total_tiles_other) {
   // aim for >= ~2048 waves in flight over the whole launch, 16..128 rows per tile
   long want = 2048 - total_tiles_other;
@@ -66,6 +69,16 @@ int convolve_sep_multi(cuimg_hip_ctx* ctx, const cuimg_img* ins, const cuimg_img
     J.tiles_x = ceil_div(a.ncols, W);
     J.vec_ok = ((uintptr_t)a.begin % (vec * es_in) == 0) && (a.pitch % (vec * es_in) == 0) &&
                ((uintptr_t)b.begin % (vec * es_out) == 0) && (b.pitch % (vec * es_out) == 0);
+  }
+  // FP32 / FP32x4 images with 19 or more taps: the matrix-core kernel (the
+  // vector kernel is VALU-bound there; measured faster from K = 9 on)
+  if (K >= kMfmaMinK && in_dt != CUIMG_U8) {
+    size_t ob = 0;
+    for (int i = 0; i < n; i++) ob += (size_t)jobs.j[i].nr * jobs.j[i].nc * es_out;
+    for (int i = 0; i < n; i++) jobs.j[i].stream_out = ob > kStreamOutBytes;
+    coef_pack cp{};
+    for (int m = 0; m < 2 * K + 1; m++) cp.c[m] = coefs[m];
+    return convolve_sep_mfma(ctx, jobs, cp, K, in_dt == CUIMG_F32X4, th_override);
   }
   // tile height: sized on the largest image
   int big = 0;

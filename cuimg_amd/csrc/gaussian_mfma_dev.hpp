@@ -1,0 +1,266 @@
+// Separable Gaussian on the matrix cores (FP32 and FP32x4 images, wide kernels).
+//
+// Semantics: meta_convolve's row and column passes over taps -K..K with
+// clamp-to-edge reads (cuimg/gpu/meta_convolve.h:25-128), in FP32; only the
+// order in which each output's products are added differs from the VALU
+// kernel (gaussian_dev.hpp), so results agree with it and with the
+// reference's FP32 arithmetic to within FP32 rounding of the sums.
+//
+// Both passes are banded Toeplitz products on v_mfma_f32_16x16x4f32
+// (D[16x16] += A[16x4] B[4x16]; lane l holds A[l%16][l/16], B[l/16][l%16] and
+// D[4(l/16)+i][l%16] in register i):
+//  * row pass, per 16 x 16 block H of row-filtered values: A = 16 input rows x
+//    4 input columns of the block's (16 + 2K)-column window, B = the taps, NH =
+//    ceil((16 + 2K) / 4) steps (which window column a lane group takes at a
+//    step is the staging layout's choice, gmfma).  A's row m is input row
+//    4 (m % 4) + m / 4 of the block, which puts rows 4i .. 4i + 3 of H in
+//    register i of the four lane groups;
+//  * column pass, per 16 x 16 output block: B = one register of H (four
+//    consecutive H rows in the four lane groups, column l % 16), A = the taps
+//    for those rows, over the H blocks above, at and below the output block;
+//    steps whose rows are all outside the taps are skipped at compile time
+//    (10 for K = 12, 8 for K = 6).
+// So the row pass's D registers are the column pass's B operands as they
+// stand: no data movement between the passes.  The matrix cores run FP32 at
+// twice the non-packed VALU rate; the wide kernels (13 and 25 taps) are
+// VALU-bound on the vector path.
+//
+// A wave owns a strip of 64 columns (FP32: 4 column blocks) or 16 pixels
+// (FP32x4: the 4 channels as 4 blocks, so a lane holds all channels of its
+// output pixels and stores 16 bytes; FP32 output blocks pass through LDS for
+// 16-byte stores), and a tile of rows aligned to 16.  It
+// walks the tile's 16-row blocks top to bottom: block b + 1's clamped input is
+// loaded into registers while block b, staged in LDS, is filtered, and output
+// block b - 1 is written once H blocks b - 2 .. b exist.
+#pragma once
+
+#include "gaussian_dev.hpp"
+
+namespace cuimg {
+
+typedef float v4f_mfma __attribute__((ext_vector_type(4)));
+
+// Layout of the per-wave staging block for pixel type E and half width K.
+// FP32: staged float x of a row is column c0 - KA + x (KA = K rounded up to 4,
+// so interior strips load 16-byte chunks); the row stride RS is 2 mod 32, and
+// lane group k takes window column 4 s + k at step s, so the 32 lanes of a
+// half-wave read 32 different banks.  FP32x4: staged pixel x is column c0 - K
+// + x, lane group k takes pixels NH k .. NH k + NH - 1.
+template <typename E, int K>
+struct gmfma {
+  static constexpr bool RGBA = sizeof(E) == 16;
+  static constexpr int NH = (16 + 2 * K + 3) / 4;  // row-pass steps
+  static constexpr int SW = RGBA ? 16 : 64;        // strip width in pixels
+  // FP32
+  static constexpr int KA = (K + 3) / 4 * 4, OFF = KA - K;
+  static constexpr int CW = (OFF + 48 + 4 * NH + 3) / 4;  // 16-byte chunks per staged row
+  static constexpr int LWF = 4 * CW;                       // staged floats per row
+  static constexpr int RS = LWF + ((2 - LWF % 32) + 32) % 32;
+  static constexpr int NLC = (16 * CW + 63) / 64;         // chunk loads per lane per block
+  // FP32x4
+  static constexpr int LWP = 4 * NH;                       // staged pixels per row
+  static constexpr int NL = (16 * LWP + 63) / 64;          // pixel loads per lane per block
+  static constexpr int LDS_FLOATS = RGBA ? 64 * NL * 4 : 16 * RS;
+  static constexpr int OS = 68;                            // output transpose row stride (FP32)
+  static_assert(RGBA || 16 * OS <= LDS_FLOATS, "transpose fits the staging block");
+};
+
+template <typename E, int K>
+__device__ __forceinline__ void gauss_mfma_tile(const gauss_job& J, int tx, int ty, const coef_pack& cp,
+                                                float* lds) {
+  using G = gmfma<E, K>;
+  constexpr int NH = G::NH, P = 2 * K + 1;
+  const int lane = threadIdx.x;
+  const int m = lane & 15, kg = lane >> 4;
+  const int c0 = tx * G::SW;
+  const int r0 = ty * J.th;  // th is a multiple of 16
+  const int rows = min(J.th, J.nr - r0);
+  const int nob = (rows + 15) >> 4;
+  const int ob0 = r0 >> 4;
+
+  // row-pass taps: B[k][n] of step s (k = lane group, n = lane % 16) is the
+  // tap of window column (FP32: 4 s + k; FP32x4: NH k + s) for output column n
+  float bh[NH];
+#pragma unroll
+  for (int s = 0; s < NH; s++) {
+    const int t = (G::RGBA ? NH * kg + s : 4 * s + kg) - m;
+    bh[s] = (t >= 0 && t < P) ? cp.c[t] : 0.f;
+  }
+  // column-pass taps: A[m][k] for H block d (-1, 0, +1 around the output
+  // block) and register i: H row 16 d + 4 i + k, output row m
+  float av[3][4];
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int t = 16 * (d - 1) + 4 * i + kg - m + K;
+      av[d][i] = (t >= 0 && t < P) ? cp.c[t] : 0.f;
+    }
+
+  // prefetch registers of one input block (rows 16 b .. 16 b + 15, clamped)
+  constexpr int NPRE = G::RGBA ? G::NL : 4 * G::NLC;
+  using V = std::conditional_t<G::RGBA, v4f_t, float>;
+  // FP32 strips whose staged window lies inside the image load 16-byte chunks
+  const int s0 = c0 - G::KA;
+  const bool interior = !G::RGBA && J.vec_ok && s0 >= 0 && s0 + G::LWF <= J.nc;
+  auto load_block = [&](int b, V (&v)[NPRE]) {
+    if constexpr (G::RGBA) {
+#pragma unroll
+      for (int q = 0; q < G::NL; q++) {
+        const int e = min(64 * q + lane, 16 * G::LWP - 1);
+        const int r = e / G::LWP, x = e - r * G::LWP;
+        const int gr = clampi(16 * b + r, 0, J.nr - 1), gc = clampi(c0 - K + x, 0, J.nc - 1);
+        v[q] = *reinterpret_cast<const v4f_t*>(px<E>(J.in, J.in_pitch, gr, gc));
+      }
+    } else {
+      // 16-byte chunks of the staged rows: one vector load inside the image,
+      // four clamped element loads at its edges
+#pragma unroll
+      for (int q = 0; q < G::NLC; q++) {
+        const int e = min(64 * q + lane, 16 * G::CW - 1);
+        const int r = e / G::CW, cx = e - r * G::CW;
+        const int gr = clampi(16 * b + r, 0, J.nr - 1), gc = s0 + 4 * cx;
+        v4f_t w;
+        if (interior) {
+          w = *reinterpret_cast<const v4f_t*>(px<float>(J.in, J.in_pitch, gr, gc));
+        } else {
+#pragma unroll
+          for (int k = 0; k < 4; k++) w[k] = *px<float>(J.in, J.in_pitch, gr, clampi(gc + k, 0, J.nc - 1));
+        }
+        v[4 * q] = w[0];
+        v[4 * q + 1] = w[1];
+        v[4 * q + 2] = w[2];
+        v[4 * q + 3] = w[3];
+      }
+    }
+  };
+  auto stage_block = [&](const V (&v)[NPRE]) {
+    if constexpr (G::RGBA) {
+#pragma unroll
+      for (int q = 0; q < G::NL; q++) reinterpret_cast<v4f_t*>(lds)[64 * q + lane] = v[q];  // tail: padding
+    } else {
+#pragma unroll
+      for (int q = 0; q < G::NLC; q++) {
+        const int e = 64 * q + lane;
+        if (e < 16 * G::CW) {
+          const int r = e / G::CW, cx = e - r * G::CW;
+          float* d = lds + r * G::RS + 4 * cx;  // 8-byte aligned (RS even)
+          reinterpret_cast<v2f_t*>(d)[0] = v2f_t{v[4 * q], v[4 * q + 1]};
+          reinterpret_cast<v2f_t*>(d)[1] = v2f_t{v[4 * q + 2], v[4 * q + 3]};
+        }
+      }
+    }
+  };
+  auto wave_lds_sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+
+  v4f_mfma h2[4], h1[4], h0[4];  // H blocks b - 2, b - 1, b (per column block / channel)
+#pragma unroll
+  for (int j = 0; j < 4; j++) h2[j] = h1[j] = h0[j] = v4f_mfma{0.f, 0.f, 0.f, 0.f};
+  V pre[NPRE];
+  load_block(ob0 - 1, pre);
+  const int nblk = nob + 2;
+  const int arow = 4 * (m & 3) + (m >> 2);  // A row m is input row 4 (m % 4) + m / 4
+  for (int bi = 0; bi < nblk; bi++) {
+    const int b = ob0 - 1 + bi;
+    // one wave per workgroup: ordering the wave's own LDS accesses suffices
+    // (the previous block's reads before this block's writes, then the writes
+    // before the reads)
+    wave_lds_sync();
+    stage_block(pre);
+    wave_lds_sync();
+    if (bi + 1 < nblk) load_block(b + 1, pre);
+
+    // row pass of block b
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      h2[j] = h1[j];
+      h1[j] = h0[j];
+      h0[j] = v4f_mfma{0.f, 0.f, 0.f, 0.f};
+    }
+    if constexpr (G::RGBA) {
+      const v4f_t* row = reinterpret_cast<const v4f_t*>(lds) + arow * G::LWP + NH * kg;
+#pragma unroll
+      for (int s = 0; s < NH; s++) {
+        const v4f_t a = row[s];
+        h0[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], bh[s], h0[0], 0, 0, 0);
+        h0[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], bh[s], h0[1], 0, 0, 0);
+        h0[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], bh[s], h0[2], 0, 0, 0);
+        h0[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], bh[s], h0[3], 0, 0, 0);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const float* row = lds + arow * G::RS + G::OFF + 16 * j + kg;
+#pragma unroll
+        for (int s = 0; s < NH; s++)
+          h0[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(row[4 * s], bh[s], h0[j], 0, 0, 0);
+      }
+    }
+    if (bi < 2) continue;
+
+    // column pass of output block b - 1 from H blocks b - 2, b - 1, b
+    const int ob = b - 1;
+    auto column_pass = [&](int j) {
+      v4f_mfma o = v4f_mfma{0.f, 0.f, 0.f, 0.f};
+      static_for<3>([&](auto dc) {
+        constexpr int d = decltype(dc)::value;
+        static_for<4>([&](auto ic) {
+          constexpr int i = decltype(ic)::value;
+          constexpr int lo = 16 * (d - 1) + 4 * i;  // H rows lo .. lo + 3 relative to the output block
+          if constexpr (lo + 3 + K >= 0 && lo - 15 + K <= 2 * K) {
+            const v4f_mfma& hb = d == 0 ? h2[j] : (d == 1 ? h1[j] : h0[j]);
+            o = __builtin_amdgcn_mfma_f32_16x16x4f32(av[d][i], hb[i], o, 0, 0, 0);
+          }
+        });
+      });
+      return o;
+    };
+    if constexpr (G::RGBA) {
+      // lane: output rows 16 ob + 4 kg + i of pixel c0 + m, all four channels
+      const v4f_mfma o0 = column_pass(0), o1 = column_pass(1), o2 = column_pass(2), o3 = column_pass(3);
+      const int c = c0 + m;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int r = 16 * ob + 4 * kg + i;
+        if (r < r0 + rows && c < J.nc) {
+          float4* p = px<float4>(J.out, J.out_pitch, r, c);
+          const v4f_t w = {o0[i], o1[i], o2[i], o3[i]};
+          if (J.stream_out) store_nt(p, w);
+          else *reinterpret_cast<v4f_t*>(p) = w;
+        }
+      }
+    } else {
+      // the 16 x 64 output block through LDS (the staging block is free once
+      // the row pass has read it), then 16 bytes per lane per row segment
+      wave_lds_sync();
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const v4f_mfma o = column_pass(j);
+#pragma unroll
+        for (int i = 0; i < 4; i++) lds[(4 * kg + i) * G::OS + 16 * j + m] = o[i];
+      }
+      wave_lds_sync();
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int f = 64 * q + lane, rr = f >> 4, cx = f & 15;
+        const int r = 16 * ob + rr, c = c0 + 4 * cx;
+        const v4f_t w = *reinterpret_cast<const v4f_t*>(lds + rr * G::OS + 4 * cx);
+        if (r < r0 + rows && c < J.nc) {
+          float* p = px<float>(J.out, J.out_pitch, r, c);
+          if (J.vec_ok && c + 4 <= J.nc) {
+            if (J.stream_out) store_nt(p, w);
+            else *reinterpret_cast<v4f_t*>(p) = w;
+          } else {
+            for (int k = 0; k < 4 && c + k < J.nc; k++) p[k] = w[k];
+          }
+        }
+      }
+    }
+  }
+}
+
+}  // namespace cuimg

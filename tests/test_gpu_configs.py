@@ -62,7 +62,8 @@ def test_gaussian_roofline_shapes(ctx, orc, images, kind, sigma):
     err = rel_err(got, ref)
     print(f"  max relative error {err:.3g}", flush=True)
     assert err < 1e-5
-    assert np.array_equal(got, orc.gaussian(img, coef, K, fused=True))
+    if K < 9:  # the vector kernel: the oracle's fused mode bit for bit (19+ taps run on the matrix cores)
+        assert np.array_equal(got, orc.gaussian(img, coef, K, fused=True))
     src.free()
     dst.free()
 
