@@ -679,7 +679,7 @@ __device__ inline void bitonic_sort(unsigned long long* keys, uint32_t np) {
           }
         }
       }
-      if (GLOBAL) __threadfence();
+      if (GLOBAL) __threadfence_block();  // one workgroup
       __syncthreads();
     }
 }
@@ -745,7 +745,7 @@ __device__ inline void replay_chunk(const merge_args& a, unsigned long long key)
     st32(&a.zeroer[victim], ai);
     a.parts[victim].age = 0;
   }
-  __threadfence();
+  __threadfence_block();
 }
 
 // Round 1 for every actor, over many workgroups (the decisions of one round
@@ -774,8 +774,25 @@ __device__ inline void merge_round1_body(const merge_args& a, int bid, int nb) {
   }
 }
 
+// Resets the zeroer / minunres entries of actors [0, count) to kNoIdx (the
+// state between merges), workgroup bid of nb.
+__device__ inline void merge_restore_body(const merge_args& a, uint32_t count, int bid, int nb) {
+  for (uint32_t k = bid * blockDim.x + threadIdx.x; k < count; k += nb * blockDim.x) {
+    const merge_actor A = a.actors[k];
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      if (A.mask & (1u << j)) {
+        st32(&a.minunres[A.b[j]], kNoIdx);
+        st32(&a.zeroer[A.b[j]], kNoIdx);
+      }
+  }
+}
+
 // One workgroup: later rounds over the actors round 1 left open, serial replay
-// of the rest, restoration of the scratch.
+// of the rest, restoration of the scratch (or, in a frame step, the count of
+// entries for the next step's fill to restore).  The workgroup is the only
+// reader and writer of the scratch while it runs, so its fences are
+// workgroup-scope.
 __device__ inline void merge_resolve_body(const merge_args& a) {
   __shared__ unsigned long long keys_lds[kPendLds];
   __shared__ unsigned long long small[64];
@@ -805,7 +822,7 @@ __device__ inline void merge_resolve_body(const merge_args& a) {
       for (int j = 0; j < 8; j++)
         if (A.mask & (1u << j)) atomicMin(&a.minunres[A.b[j]], round_tag(r) | (uint32_t)A.i);
     }
-    __threadfence();
+    __threadfence_block();
     __syncthreads();
     for (uint32_t s = threadIdx.x; s < np; s += blockDim.x) {
       const unsigned long long key = kget(s);
@@ -813,7 +830,7 @@ __device__ inline void merge_resolve_body(const merge_args& a) {
       if (merge_decide(a, a.actors[(uint32_t)key], r)) kset(s, ~0ull);
       else atomicAdd(&nopen[r & 1], 1u);
     }
-    __threadfence();
+    __threadfence_block();
     __syncthreads();
     open = nopen[r & 1];
   }
@@ -832,7 +849,7 @@ __device__ inline void merge_resolve_body(const merge_args& a) {
     // after kMaxRounds: full sort (open keys first), replay in chunks of 64
     const uint32_t np2 = pow2_at_least(np);
     for (uint32_t k = np + threadIdx.x; k < np2; k += blockDim.x) kset(k, ~0ull);
-    __threadfence();
+    __threadfence_block();
     __syncthreads();
     if (np2 <= kPendLds) bitonic_sort<false>(keys, np2);
     else bitonic_sort<true>(keys, np2);
@@ -843,15 +860,11 @@ __device__ inline void merge_resolve_body(const merge_args& a) {
       }
   }
   __syncthreads();
-  // restore the scratch for the next call
-  for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
-    const merge_actor A = a.actors[k];
-#pragma unroll
-    for (int j = 0; j < 8; j++)
-      if (A.mask & (1u << j)) {
-        st32(&a.minunres[A.b[j]], kNoIdx);
-        st32(&a.zeroer[A.b[j]], kNoIdx);
-      }
+  // restore the scratch for the next call, or leave it to the next frame step
+  if (a.defer_restore) {
+    if (threadIdx.x == 0) a.cnt->merge_restore = m;
+  } else {
+    merge_restore_body(a, m, 0, 1);
   }
   if (threadIdx.x == 0) {
     a.cnt->actors = 0;

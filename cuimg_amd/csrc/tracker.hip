@@ -267,10 +267,13 @@ __global__ __launch_bounds__(64) void step_blurgrad_kernel(const level_job* jobs
   gauss_sobel_tile<K>(G, g.out, g.out_pitch, lb % G.tiles_x, lb / G.tiles_x, cp);
 }
 
-// before_matching (sparse <- -1, particle_container.hpp:187-193) and the
-// detector mask reset (mask <- 255).
+// The merge scratch entries the last step's merge touched back to their idle
+// state (every step, so a step after clear_scale, which skips the matcher,
+// still merges on a clean scratch), then before_matching (sparse <- -1,
+// particle_container.hpp:187-193).
 __global__ __launch_bounds__(256) void step_fill_kernel(const level_job* jobs, const uint32_t* prefix, int njobs) {
   CUIMG_JOB(prefix, njobs);
+  merge_restore_body(uni(J.mg[0]), J.cnt->merge_restore, lb, nb);  // J.mg[0] and [1] share the scratch
   const step_flags f = flags_of(J);
   if (!f.match) return;
   const int b = J.sparse_border;
@@ -421,6 +424,7 @@ __global__ __launch_bounds__(1024) void step_mres_kernel(const level_job* jobs) 
   }
   __syncthreads();
   if (threadIdx.x == 0) {
+    if (!f.merge) J.cnt->merge_restore = 0;
     J.cnt->cur = c;
     J.cnt->compact_has_run = f.detect ? 1u : 0u;
     J.cnt->frame = f.frame + 1;
@@ -761,6 +765,7 @@ static void build_job(cuimg_hip_batch* b, int i, int s) {
     m.sparse = J.sparse;
     m.sparse_pitch = J.sparse_pitch;
     m.cnt = J.cnt;
+    m.defer_restore = 1;  // step_fill_kernel resets the scratch
   }
   for (int slot = 0; slot < 2; slot++) {
     grad_job& gj = J.grad[slot];

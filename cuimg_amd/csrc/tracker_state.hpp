@@ -27,6 +27,9 @@ struct level_counters {
   uint32_t pend1;    // merge: actors left open by round 1 (in merge_args::pend)
   uint32_t steps;    // scale 0 only: frame steps of the stream since clear (its
                      // parity picks the frame slot of every scale)
+  uint32_t merge_restore;  // merge actors of the last frame step whose scratch
+                           // entries the next step's fill resets
+  uint32_t reserved_;
   unsigned long long live_total;      // particles (age > 0) run through the matcher
   unsigned long long accepted_total;  // of which accepted (moved)
 };
@@ -145,6 +148,8 @@ struct merge_args {
   uint32_t* minunres;        // cap; 0xFFFFFFFF between calls
   unsigned long long* pend;  // pend_cap: open actors beyond the LDS list
   uint32_t cap, pend_cap;    // pend_cap: power of two >= cap; cap < 2^24
+  int defer_restore;         // 1: the resolve leaves the scratch reset to the
+                             // next frame step (merge_restore_body)
 };
 constexpr uint32_t kMergeMaxCap = (1u << 24) - 1;
 constexpr uint32_t kMaxParticles = 1u << 30;  // standalone container calls
