@@ -12,7 +12,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void ga
   const gauss_job& J = jobs.j[k];
   b -= J.tile_begin;
   __shared__ __align__(16) float lds[gmfma<E, K>::LDS_FLOATS];
-  gauss_mfma_tile<E, K>(J, b % J.tiles_x, b / J.tiles_x, cp, lds);
+  using G = gmfma<E, K>;
+  const int tx = b % J.tiles_x, ty = b / J.tiles_x;
+  const int s0 = tx * G::SW - G::KA;
+  if (!G::RGBA && J.vec_ok && s0 >= 0 && s0 + G::LWF <= J.nc)
+    gauss_mfma_tile<E, K, true>(J, tx, ty, cp, lds);
+  else
+    gauss_mfma_tile<E, K, false>(J, tx, ty, cp, lds);
 }
 
 template <typename E>

@@ -65,7 +65,11 @@ struct gmfma {
   static_assert(RGBA || 16 * OS <= LDS_FLOATS, "transpose fits the staging block");
 };
 
-template <typename E, int K>
+// INTERIOR (FP32): the strip's staged window lies inside the image and the
+// rows are 16-byte aligned, so every chunk is one 16-byte load; otherwise four
+// clamped element loads.  A separate instance per case keeps the load loop
+// free of branches, so the next block's loads stay in flight under the MFMAs.
+template <typename E, int K, bool INTERIOR>
 __device__ __forceinline__ void gauss_mfma_tile(const gauss_job& J, int tx, int ty, const coef_pack& cp,
                                                 float* lds) {
   using G = gmfma<E, K>;
@@ -100,9 +104,7 @@ __device__ __forceinline__ void gauss_mfma_tile(const gauss_job& J, int tx, int 
   // prefetch registers of one input block (rows 16 b .. 16 b + 15, clamped)
   constexpr int NPRE = G::RGBA ? G::NL : 4 * G::NLC;
   using V = std::conditional_t<G::RGBA, v4f_t, float>;
-  // FP32 strips whose staged window lies inside the image load 16-byte chunks
   const int s0 = c0 - G::KA;
-  const bool interior = !G::RGBA && J.vec_ok && s0 >= 0 && s0 + G::LWF <= J.nc;
   auto load_block = [&](int b, V (&v)[NPRE]) {
     if constexpr (G::RGBA) {
 #pragma unroll
@@ -113,15 +115,14 @@ __device__ __forceinline__ void gauss_mfma_tile(const gauss_job& J, int tx, int 
         v[q] = *reinterpret_cast<const v4f_t*>(px<E>(J.in, J.in_pitch, gr, gc));
       }
     } else {
-      // 16-byte chunks of the staged rows: one vector load inside the image,
-      // four clamped element loads at its edges
+      // 16-byte chunks of the staged rows
 #pragma unroll
       for (int q = 0; q < G::NLC; q++) {
         const int e = min(64 * q + lane, 16 * G::CW - 1);
         const int r = e / G::CW, cx = e - r * G::CW;
         const int gr = clampi(16 * b + r, 0, J.nr - 1), gc = s0 + 4 * cx;
         v4f_t w;
-        if (interior) {
+        if constexpr (INTERIOR) {
           w = *reinterpret_cast<const v4f_t*>(px<float>(J.in, J.in_pitch, gr, gc));
         } else {
 #pragma unroll
@@ -192,36 +193,39 @@ __device__ __forceinline__ void gauss_mfma_tile(const gauss_job& J, int tx, int 
         h0[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], bh[s], h0[3], 0, 0, 0);
       }
     } else {
+      // the four column blocks' accumulator chains interleaved step by step
+      const float* row = lds + arow * G::RS + G::OFF + kg;
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const float* row = lds + arow * G::RS + G::OFF + 16 * j + kg;
+      for (int s = 0; s < NH; s++)
 #pragma unroll
-        for (int s = 0; s < NH; s++)
-          h0[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(row[4 * s], bh[s], h0[j], 0, 0, 0);
-      }
+        for (int j = 0; j < 4; j++)
+          h0[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(row[16 * j + 4 * s], bh[s], h0[j], 0, 0, 0);
     }
     if (bi < 2) continue;
 
     // column pass of output block b - 1 from H blocks b - 2, b - 1, b
     const int ob = b - 1;
-    auto column_pass = [&](int j) {
-      v4f_mfma o = v4f_mfma{0.f, 0.f, 0.f, 0.f};
-      static_for<3>([&](auto dc) {
-        constexpr int d = decltype(dc)::value;
-        static_for<4>([&](auto ic) {
-          constexpr int i = decltype(ic)::value;
-          constexpr int lo = 16 * (d - 1) + 4 * i;  // H rows lo .. lo + 3 relative to the output block
-          if constexpr (lo + 3 + K >= 0 && lo - 15 + K <= 2 * K) {
-            const v4f_mfma& hb = d == 0 ? h2[j] : (d == 1 ? h1[j] : h0[j]);
-            o = __builtin_amdgcn_mfma_f32_16x16x4f32(av[d][i], hb[i], o, 0, 0, 0);
+    // the four blocks' chains interleaved step by step
+    v4f_mfma o[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) o[j] = v4f_mfma{0.f, 0.f, 0.f, 0.f};
+    static_for<3>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      static_for<4>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        constexpr int lo = 16 * (d - 1) + 4 * i;  // H rows lo .. lo + 3 relative to the output block
+        if constexpr (lo + 3 + K >= 0 && lo - 15 + K <= 2 * K) {
+#pragma unroll
+          for (int jj = 0; jj < 4; jj++) {
+            const v4f_mfma& hb = d == 0 ? h2[jj] : (d == 1 ? h1[jj] : h0[jj]);
+            o[jj] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[d][i], hb[i], o[jj], 0, 0, 0);
           }
-        });
+        }
       });
-      return o;
-    };
+    });
     if constexpr (G::RGBA) {
       // lane: output rows 16 ob + 4 kg + i of pixel c0 + m, all four channels
-      const v4f_mfma o0 = column_pass(0), o1 = column_pass(1), o2 = column_pass(2), o3 = column_pass(3);
+      const v4f_mfma o0 = o[0], o1 = o[1], o2 = o[2], o3 = o[3];
       const int c = c0 + m;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
@@ -238,11 +242,9 @@ __device__ __forceinline__ void gauss_mfma_tile(const gauss_job& J, int tx, int 
       // the row pass has read it), then 16 bytes per lane per row segment
       wave_lds_sync();
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const v4f_mfma o = column_pass(j);
+      for (int j = 0; j < 4; j++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) lds[(4 * kg + i) * G::OS + 16 * j + m] = o[i];
-      }
+        for (int i = 0; i < 4; i++) lds[(4 * kg + i) * G::OS + 16 * j + m] = o[j][i];
       wave_lds_sync();
 #pragma unroll
       for (int q = 0; q < 4; q++) {
