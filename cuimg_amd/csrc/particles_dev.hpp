@@ -637,20 +637,29 @@ __device__ inline void merge_scan_body(const merge_args& a, int bid, int nb) {
 // targets b, so whatever this thread reads there gives the same answer.
 __device__ inline bool merge_decide(const merge_args& a, const merge_actor& A, uint32_t r) {
   const uint32_t tg = 255u - r, i = (uint32_t)A.i;
-  auto blocked = [&](uint32_t x) {
-    const uint32_t v = ld32(&a.minunres[x]);
-    return (v >> 24) == tg && (v & 0xFFFFFFu) < i;
-  };
-  if (blocked(i)) return false;
-  if (ld32(&a.zeroer[i]) < i) return true;  // removed before its turn
+  // every value the decision can need, read in one round trip (minunres does
+  // not change during a round's decisions, and zeroer only where the answer
+  // does not depend on it)
+  const uint32_t mi = ld32(&a.minunres[i]), zi = ld32(&a.zeroer[i]);
+  uint32_t mb[8], zb[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    mb[j] = zb[j] = kNoIdx;
+    if (A.mask & (1u << j)) {
+      mb[j] = ld32(&a.minunres[A.b[j]]);
+      zb[j] = ld32(&a.zeroer[A.b[j]]);
+    }
+  }
+  auto blocked = [&](uint32_t v) { return (v >> 24) == tg && (v & 0xFFFFFFu) < i; };
+  if (blocked(mi)) return false;
+  if (zi < i) return true;  // removed before its turn
   int32_t victim = -1;
 #pragma unroll
   for (int j = 0; j < 8; j++)
     if (victim < 0 && (A.mask & (1u << j))) {
-      const uint32_t b = (uint32_t)A.b[j];
-      if (blocked(b)) return false;
-      if (ld32(&a.zeroer[b]) < i) continue;
-      victim = (int32_t)b;
+      if (blocked(mb[j])) return false;
+      if (zb[j] < i) continue;
+      victim = A.b[j];
     }
   if (victim >= 0) {
     st32(&a.zeroer[victim], i);
