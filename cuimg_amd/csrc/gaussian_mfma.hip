@@ -5,7 +5,7 @@
 namespace cuimg {
 
 template <typename E, int K>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void gauss_mfma_kernel(gauss_jobs jobs, coef_pack cp) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void gauss_mfma_kernel(gauss_jobs jobs, coef_pack cp) {
   int b = xcd_order(blockIdx.x, gridDim.x);
   int k = 0;
   while (k + 1 < jobs.n && b >= jobs.j[k + 1].tile_begin) k++;
@@ -15,10 +15,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void ga
   using G = gmfma<E, K>;
   const int tx = b % J.tiles_x, ty = b / J.tiles_x;
   const int s0 = tx * G::SW - G::KA;
-  if (!G::RGBA && J.vec_ok && s0 >= 0 && s0 + G::LWF <= J.nc)
-    gauss_mfma_tile<E, K, true>(J, tx, ty, cp, lds);
-  else
-    gauss_mfma_tile<E, K, false>(J, tx, ty, cp, lds);
+  // interior strips: FP32, the staged window inside the image with 16-byte
+  // rows; FP32x4, all 16 output pixels inside the image
+  const bool interior = G::RGBA ? (tx + 1) * G::SW <= J.nc : J.vec_ok && s0 >= 0 && s0 + G::LWF <= J.nc;
+  if (interior && J.stream_out) gauss_mfma_tile<E, K, true, true>(J, tx, ty, cp, lds);
+  else if (interior) gauss_mfma_tile<E, K, true, false>(J, tx, ty, cp, lds);
+  else gauss_mfma_tile<E, K, false, false>(J, tx, ty, cp, lds);
 }
 
 template <typename E>
@@ -33,15 +35,17 @@ static void launch_mfma_k(cuimg_hip_ctx* ctx, const gauss_jobs& jobs, int blocks
 }
 
 // Tiles: 64-column (FP32) or 16-pixel (FP32x4) strips, rows in multiples of
-// 16 (th_override rounded up).  By default about 2048 waves over the launch:
-// at 3 resident waves per SIMD (the VGPR budget for K >= 9) a launch of more
-// than 3 per SIMD runs a last, thinly occupied round.
+// 16 (th_override rounded up).  2 waves are resident per SIMD (the VGPR budget
+// with two blocks in flight per wave).  By default FP32 launches about 2048
+// waves, one round; FP32x4, whose strips read 2.5 times their width, about
+// 4096, which halves each wave's walk (measured faster on 3840 x 2160).
 int convolve_sep_mfma(cuimg_hip_ctx* ctx, gauss_jobs jobs, const coef_pack& cp, int K, bool rgba,
                       int th_override) {
   const int sw = rgba ? 16 : 64;
   long strips = 0;
   for (int i = 0; i < jobs.n; i++) strips += (long)jobs.j[i].nr * ceil_div(jobs.j[i].nc, sw);
-  int th = th_override > 0 ? th_override : (int)std::min<long>(256, std::max<long>(16, (strips + 2047) / 2048));
+  const long want = rgba ? 4096 : 2048;
+  int th = th_override > 0 ? th_override : (int)std::min<long>(256, std::max<long>(16, (strips + want - 1) / want));
   th = (th + 15) / 16 * 16;
   int total = 0;
   for (int i = 0; i < jobs.n; i++) {

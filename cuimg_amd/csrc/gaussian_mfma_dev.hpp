@@ -19,19 +19,22 @@
 //    consecutive H rows in the four lane groups, column l % 16), A = the taps
 //    for those rows, over the H blocks above, at and below the output block;
 //    steps whose rows are all outside the taps are skipped at compile time
-//    (10 for K = 12, 8 for K = 6).
-// So the row pass's D registers are the column pass's B operands as they
+//    (10 for K = 12, 8 for K = 6).  FP32 computes the transposed product
+//    (A = that H register, B = the taps), which leaves each lane with four
+//    consecutive output columns of one row.
+// So the row pass's D registers are the column pass's operands as they
 // stand: no data movement between the passes.  The matrix cores run FP32 at
 // twice the non-packed VALU rate; the wide kernels (13 and 25 taps) are
 // VALU-bound on the vector path.
 //
 // A wave owns a strip of 64 columns (FP32: 4 column blocks) or 16 pixels
 // (FP32x4: the 4 channels as 4 blocks, so a lane holds all channels of its
-// output pixels and stores 16 bytes; FP32 output blocks pass through LDS for
-// 16-byte stores), and a tile of rows aligned to 16.  It
-// walks the tile's 16-row blocks top to bottom: block b + 1's clamped input is
-// loaded into registers while block b, staged in LDS, is filtered, and output
-// block b - 1 is written once H blocks b - 2 .. b exist.
+// output pixels and stores 16 bytes; FP32: a lane stores 16 bytes of one
+// output row per column block), and a tile of rows aligned to 16.  It
+// walks the tile's 16-row blocks top to bottom: the clamped input of blocks
+// b + 1 and b + 2 is in flight into registers while block b, staged in LDS,
+// is filtered, and output block b - 1 is written once H blocks b - 2 .. b
+// exist.
 #pragma once
 
 #include "gaussian_dev.hpp"
@@ -39,6 +42,12 @@
 namespace cuimg {
 
 typedef float v4f_mfma __attribute__((ext_vector_type(4)));
+typedef uint32_t v4u_t __attribute__((ext_vector_type(4)));
+
+// Raw buffer descriptor word 3 on gfx9 (32-bit data format, no swizzle) and
+// the store cache policy with the non-temporal bit (gfx940 and later).
+constexpr int kBufferRsrcWord3 = 0x00020000;
+constexpr int kCachePolicyNT = 2;
 
 // Layout of the per-wave staging block for pixel type E and half width K.
 // FP32: staged float x of a row is column c0 - KA + x (KA = K rounded up to 4,
@@ -61,15 +70,18 @@ struct gmfma {
   static constexpr int LWP = 4 * NH;                       // staged pixels per row
   static constexpr int NL = (16 * LWP + 63) / 64;          // pixel loads per lane per block
   static constexpr int LDS_FLOATS = RGBA ? 64 * NL * 4 : 16 * RS;
-  static constexpr int OS = 68;                            // output transpose row stride (FP32)
-  static_assert(RGBA || 16 * OS <= LDS_FLOATS, "transpose fits the staging block");
 };
 
-// INTERIOR (FP32): the strip's staged window lies inside the image and the
-// rows are 16-byte aligned, so every chunk is one 16-byte load; otherwise four
-// clamped element loads.  A separate instance per case keeps the load loop
-// free of branches, so the next block's loads stay in flight under the MFMAs.
-template <typename E, int K, bool INTERIOR>
+// INTERIOR: FP32, the strip's staged window lies inside the image and the rows
+// are 16-byte aligned, so every chunk is one 16-byte load (otherwise four
+// clamped element loads); both types, every output column of the strip exists,
+// so the stores go through a buffer descriptor bounded at the image's last row
+// and need no branch.  With loads and stores free of branches the compiler's
+// wait before each staging leaves the next blocks' loads in flight under the
+// MFMAs; the edge instance waits more conservatively.  NT (interior only):
+// non-temporal stores (gauss_job::stream_out), fixed per instance so that the
+// stores stay straight-line code.
+template <typename E, int K, bool INTERIOR, bool NT>
 __device__ __forceinline__ void gauss_mfma_tile(const gauss_job& J, int tx, int ty, const coef_pack& cp,
                                                 float* lds) {
   using G = gmfma<E, K>;
@@ -143,7 +155,7 @@ __device__ __forceinline__ void gauss_mfma_tile(const gauss_job& J, int tx, int 
 #pragma unroll
       for (int q = 0; q < G::NLC; q++) {
         const int e = 64 * q + lane;
-        if (e < 16 * G::CW) {
+        if (q + 1 < G::NLC || e < 16 * G::CW) {
           const int r = e / G::CW, cx = e - r * G::CW;
           float* d = lds + r * G::RS + 4 * cx;  // 8-byte aligned (RS even)
           reinterpret_cast<v2f_t*>(d)[0] = v2f_t{v[4 * q], v[4 * q + 1]};
@@ -161,19 +173,26 @@ __device__ __forceinline__ void gauss_mfma_tile(const gauss_job& J, int tx, int 
   v4f_mfma h2[4], h1[4], h0[4];  // H blocks b - 2, b - 1, b (per column block / channel)
 #pragma unroll
   for (int j = 0; j < 4; j++) h2[j] = h1[j] = h0[j] = v4f_mfma{0.f, 0.f, 0.f, 0.f};
-  V pre[NPRE];
-  load_block(ob0 - 1, pre);
+  // two blocks in flight: pre[0] holds even blocks of the walk, pre[1] odd
+  // ones; a block's registers are refilled with the block two further on as
+  // soon as they are staged.  The refill is unconditional (the last two steps
+  // reload the tile's last block, which is in L2), so at every staging the
+  // other block's loads are still outstanding and the wait is only for this
+  // block's
+  V pre[2][NPRE];
   const int nblk = nob + 2;
+  load_block(ob0 - 1, pre[0]);
+  load_block(ob0, pre[1]);  // nblk >= 3
   const int arow = 4 * (m & 3) + (m >> 2);  // A row m is input row 4 (m % 4) + m / 4
-  for (int bi = 0; bi < nblk; bi++) {
+  auto step = [&](int bi, V (&cur)[NPRE]) {
     const int b = ob0 - 1 + bi;
     // one wave per workgroup: ordering the wave's own LDS accesses suffices
     // (the previous block's reads before this block's writes, then the writes
     // before the reads)
     wave_lds_sync();
-    stage_block(pre);
+    stage_block(cur);
     wave_lds_sync();
-    if (bi + 1 < nblk) load_block(b + 1, pre);
+    load_block(min(b + 2, ob0 + nob), cur);
 
     // row pass of block b
 #pragma unroll
@@ -201,28 +220,65 @@ __device__ __forceinline__ void gauss_mfma_tile(const gauss_job& J, int tx, int 
         for (int j = 0; j < 4; j++)
           h0[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(row[16 * j + 4 * s], bh[s], h0[j], 0, 0, 0);
     }
-    if (bi < 2) continue;
+    // the walk's first two blocks only feed H
+    if constexpr (!INTERIOR) {
+      if (bi < 2) return;
+    }
 
-    // column pass of output block b - 1 from H blocks b - 2, b - 1, b
+    // column pass of output block b - 1 from H blocks b - 2, b - 1, b, the
+    // four blocks' chains interleaved step by step.  FP32x4: D = C H (lane:
+    // output rows 16 ob + 4 kg + i of pixel c0 + m, one channel per chain).
+    // FP32: D = (C H)^T = H^T C^T; the H register that is C H's B operand is
+    // H^T's A operand as it stands, and the tap register likewise, so a lane
+    // ends with output row 16 ob + m, columns 16 jj + 4 kg .. 16 jj + 4 kg + 3
+    // of column block jj: 16 contiguous bytes, stored without a transpose.
     const int ob = b - 1;
-    // the four blocks' chains interleaved step by step
     v4f_mfma o[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) o[j] = v4f_mfma{0.f, 0.f, 0.f, 0.f};
-    static_for<3>([&](auto dc) {
-      constexpr int d = decltype(dc)::value;
-      static_for<4>([&](auto ic) {
-        constexpr int i = decltype(ic)::value;
-        constexpr int lo = 16 * (d - 1) + 4 * i;  // H rows lo .. lo + 3 relative to the output block
-        if constexpr (lo + 3 + K >= 0 && lo - 15 + K <= 2 * K) {
+    if (bi >= 2) {
+      static_for<3>([&](auto dc) {
+        constexpr int d = decltype(dc)::value;
+        static_for<4>([&](auto ic) {
+          constexpr int i = decltype(ic)::value;
+          constexpr int lo = 16 * (d - 1) + 4 * i;  // H rows lo .. lo + 3 relative to the output block
+          if constexpr (lo + 3 + K >= 0 && lo - 15 + K <= 2 * K) {
 #pragma unroll
-          for (int jj = 0; jj < 4; jj++) {
-            const v4f_mfma& hb = d == 0 ? h2[jj] : (d == 1 ? h1[jj] : h0[jj]);
-            o[jj] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[d][i], hb[i], o[jj], 0, 0, 0);
+            for (int jj = 0; jj < 4; jj++) {
+              const v4f_mfma& hb = d == 0 ? h2[jj] : (d == 1 ? h1[jj] : h0[jj]);
+              if constexpr (G::RGBA) o[jj] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[d][i], hb[i], o[jj], 0, 0, 0);
+              else o[jj] = __builtin_amdgcn_mfma_f32_16x16x4f32(hb[i], av[d][i], o[jj], 0, 0, 0);
+            }
           }
-        }
+        });
       });
-    });
+    }
+    if constexpr (INTERIOR) {
+      // Every output column of the strip exists.  The four stores are issued
+      // at every step, so that the compiler's waits for the loads count them
+      // exactly: rows past the image's last, and the whole block in the walk's
+      // first two steps, lie outside the buffer descriptor and are dropped.
+      const int valid = bi >= 2 ? min(16, J.nr - 16 * ob) : 0;
+      const int orow = bi >= 2 ? 16 * ob : 0;
+      const int pitch = J.out_pitch;
+      const __amdgpu_buffer_rsrc_t out_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+          px<E>(J.out, pitch, orow, c0), 0, valid * pitch, kBufferRsrcWord3);
+      auto store16 = [&](const v4u_t& w, int off) {
+        __builtin_amdgcn_raw_buffer_store_b128(w, out_rsrc, off, 0, NT ? kCachePolicyNT : 0);
+      };
+      if constexpr (G::RGBA) {
+        // lane: output rows 16 ob + 4 kg + i of pixel c0 + m, all four channels
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const v4f_t w = {o[0][i], o[1][i], o[2][i], o[3][i]};
+          store16(__builtin_bit_cast(v4u_t, w), (4 * kg + i) * pitch + 16 * m);
+        }
+      } else {
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) store16(__builtin_bit_cast(v4u_t, o[jj]), m * pitch + 4 * (16 * jj + 4 * kg));
+      }
+      return;
+    }
     if constexpr (G::RGBA) {
       // lane: output rows 16 ob + 4 kg + i of pixel c0 + m, all four channels
       const v4f_mfma o0 = o[0], o1 = o[1], o2 = o[2], o3 = o[3];
@@ -238,20 +294,12 @@ __device__ __forceinline__ void gauss_mfma_tile(const gauss_job& J, int tx, int 
         }
       }
     } else {
-      // the 16 x 64 output block through LDS (the staging block is free once
-      // the row pass has read it), then 16 bytes per lane per row segment
-      wave_lds_sync();
+      const int r = 16 * ob + m;
+      if (r < r0 + rows) {
 #pragma unroll
-      for (int j = 0; j < 4; j++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) lds[(4 * kg + i) * G::OS + 16 * j + m] = o[j][i];
-      wave_lds_sync();
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int f = 64 * q + lane, rr = f >> 4, cx = f & 15;
-        const int r = 16 * ob + rr, c = c0 + 4 * cx;
-        const v4f_t w = *reinterpret_cast<const v4f_t*>(lds + rr * G::OS + 4 * cx);
-        if (r < r0 + rows && c < J.nc) {
+        for (int jj = 0; jj < 4; jj++) {
+          const int c = c0 + 16 * jj + 4 * kg;
+          const v4f_t w = o[jj];
           float* p = px<float>(J.out, J.out_pitch, r, c);
           if (J.vec_ok && c + 4 <= J.nc) {
             if (J.stream_out) store_nt(p, w);
@@ -262,7 +310,19 @@ __device__ __forceinline__ void gauss_mfma_tile(const gauss_job& J, int tx, int 
         }
       }
     }
+  };
+  // Steps 0 and 1 peeled and both steps of an iteration unconditional: every
+  // path into a staging then has issued the same loads and stores since the
+  // awaited block's loads, so the compiler's wait leaves them in flight.
+  step(0, pre[0]);
+  step(1, pre[1]);
+  int bi = 2;
+  for (; bi + 1 < nblk; bi += 2) {
+    step(bi, pre[0]);
+    step(bi + 1, pre[1]);
   }
+  if (bi < nblk) step(bi, pre[0]);
+  wait_vm<0>();  // the final refills land before the wave ends
 }
 
 }  // namespace cuimg
