@@ -246,38 +246,31 @@ __device__ __forceinline__ void load_patch(float* dst, int q, const void* img, i
   }
 }
 
-// load_patch for FP32 images through a raw buffer view: row offsets are one
-// 24-bit multiply-add, no 64-bit address arithmetic.
-template <int PS, int GL, int MAXQ>
-__device__ __forceinline__ void load_patch_buf(float* dst, int q, __amdgpu_buffer_rsrc_t r, int pitch, int nr,
-                                               int nc, int r0, int c0, int lane) {
+// load_patch for a square FP32 patch of Q = PS whole rows through a raw
+// buffer view: row offsets are one 24-bit multiply-add, no 64-bit address
+// arithmetic, and every lane loads and stores unconditionally.
+template <int PS, int GL>
+__device__ __forceinline__ void load_patch_buf(float* dst, __amdgpu_buffer_rsrc_t r, int pitch, int nr, int nc,
+                                               int r0, int c0, int lane) {
   static_assert(GL % PS == 0, "whole rows per pass");
-  constexpr int RPP = GL / PS;
-  constexpr int MP = (MAXQ + RPP - 1) / RPP;
+  constexpr int Q = PS, RPP = GL / PS;
+  constexpr int MP = (Q + RPP - 1) / RPP;
+  static_assert(MP * RPP == Q, "whole passes");
   const int col = lane % PS, row = lane / PS;
-  const bool on = col < q;
   float v[MP];
-  if (r0 >= 0 && r0 + q <= nr && c0 >= 0 && c0 + q <= nc) {
+  if (r0 >= 0 && r0 + Q <= nr && c0 >= 0 && c0 + Q <= nc) {
     // inside the image: one offset, rows through the uniform SGPR offset
     const uint32_t o = __umul24((uint32_t)(r0 + row), (uint32_t)pitch) + (uint32_t)(c0 + col) * 4u;
 #pragma unroll
-    for (int m = 0; m < MP; m++) {
-      const int i = row + m * RPP;
-      if (on && i < q) v[m] = ld_f32(r, o, m * RPP * pitch);
-    }
+    for (int m = 0; m < MP; m++) v[m] = ld_f32(r, o, m * RPP * pitch);
   } else {
     const uint32_t cb = (uint32_t)clampi(c0 + col, 0, nc - 1) * 4u;
 #pragma unroll
-    for (int m = 0; m < MP; m++) {
-      const int i = row + m * RPP;
-      if (on && i < q) v[m] = ld_f32(r, __umul24((uint32_t)clampi(r0 + i, 0, nr - 1), (uint32_t)pitch) + cb, 0);
-    }
+    for (int m = 0; m < MP; m++)
+      v[m] = ld_f32(r, __umul24((uint32_t)clampi(r0 + row + m * RPP, 0, nr - 1), (uint32_t)pitch) + cb, 0);
   }
 #pragma unroll
-  for (int m = 0; m < MP; m++) {
-    const int i = row + m * RPP;
-    if (on && i < q) dst[i * PS + col] = v[m];
-  }
+  for (int m = 0; m < MP; m++) dst[(row + m * RPP) * PS + col] = v[m];
 }
 
 // Orders the wave's LDS writes before its later LDS reads (one wave, in-order
@@ -551,39 +544,52 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
 // step computes it once per level and frame): the same operations as
 // wave_match, organised around the common case of an "exact" window.
 //
-// A window of half width h around coordinate x is exact when x - h >= 0 and
-// x + h stays below the next power of two above x: then every x + r, |r| <= h,
-// is computed exactly, truncates to tz(x) + r and has the fraction of x, so
-// all samples of the window share the bilinear weights of x itself (the same
-// bits wave_match computes per sample).  Samples then need no weights and no
+// A window of half width h around coordinate x is exact when every x + r,
+// |r| <= h, is computed exactly: then x + r truncates to tz(x) + r and has the
+// fraction of x, so all samples of the window share the bilinear weights of x
+// itself (the same bits wave_match computes per sample).  For x - h >= 0 the
+// sums with r <= 0 are exact, and those with r > 0 are exact when x + h stays
+// below the next power of two above x, or when x's last mantissa bit is zero:
+// x is then a multiple of the next binade's spacing, and x + h <= 2x never
+// reaches the binade after it.  Samples then need no weights and no
 // per-sample address: lane 8 h' + c reads column c of rows h' + 2k of the
-// (ws+1)^2 grid of corners.  Other windows (near a power of two, near the
-// image edge, negative coordinates) take the per-sample path.
+// (ws+1)^2 grid of corners.  Other windows (an odd last bit near a power of
+// two, near the image edge, negative coordinates) take the per-sample path.
 
 __device__ __forceinline__ bool exact_axis(float x, int h) {
   const float lo = sub_rn(x, (float)h), hi = add_rn(x, (float)h);
   const float top = __uint_as_float(((__float_as_uint(x) >> 23) + 1u) << 23);
-  return lo >= 0.f && hi < top;
+  return lo >= 0.f && (hi < top || (__float_as_uint(x) & 1u) == 0u);
 }
 
-// A sufficient interval for exact_axis around a start coordinate x0 > 0 in
-// binade [2^E, 2^(E+1)): every x in [max(2^E, h), 2^(E+1) - h) is exact (then
-// x + h < 2^(E+1) exactly).  Empty for x0 <= 0 or NaN.  Tested per step
-// instead of exact_axis; outside it the per-sample path gives the same bits.
+// A sufficient test for exact_axis around a start coordinate x0: every x in
+// [max(2^E, h), 2^(E+1) - h), x0 in binade [2^E, 2^(E+1)), is exact (then
+// x + h < 2^(E+1) exactly; the interval is empty for x0 <= 0 or NaN), and so
+// is every x >= h with a zero last mantissa bit.  Tested per step instead of
+// exact_axis; where it fails the per-sample path gives the same bits.
 struct exact_iv {
-  float lo, hi;
-  __device__ __forceinline__ bool has(float x) const { return x >= lo && x < hi; }
+  float lo, hi, h;
+  __device__ __forceinline__ bool has(float x) const {
+    return (x >= lo && x < hi) || ((__float_as_uint(x) & 1u) == 0u && x >= h);
+  }
 };
 __device__ __forceinline__ exact_iv exact_interval(float x0, int h) {
-  if (!(x0 > 0.f)) return exact_iv{1.f, 0.f};
+  if (!(x0 > 0.f)) return exact_iv{1.f, 0.f, (float)h};
   const uint32_t e = __float_as_uint(x0) >> 23;
-  return exact_iv{fmaxf(__uint_as_float(e << 23), (float)h), sub_rn(__uint_as_float((e + 1u) << 23), (float)h)};
+  return exact_iv{fmaxf(__uint_as_float(e << 23), (float)h), sub_rn(__uint_as_float((e + 1u) << 23), (float)h),
+                  (float)h};
 }
+
+// Motion margin of the fast path's current-frame patch: 4 pixels make the
+// patch 16 x 16, whole rows of its 16-float stride, and let the estimate move
+// 3 pixels from the prediction before the patch is reloaded.
+constexpr int kMB16 = 4;
 
 struct lk_lds16 {
   static constexpr int WSMAX = 7;
-  static constexpr int QB = WSMAX + 1 + 2 * kMB;  // current-frame patch side (14)
-  static constexpr int PW = 16;                   // its row stride
+  static constexpr int QB = WSMAX + 1 + 2 * kMB16;  // current-frame patch side (16)
+  static constexpr int PW = 16;                     // its row stride
+  static_assert(QB == PW, "whole-row patch loads");
   static constexpr int TPL = (WSMAX - 1) * (WSMAX / 2) + WSMAX;
   float B[QB * PW];
   float tpl[TPL];
@@ -607,10 +613,10 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
   const __amdgpu_buffer_rsrc_t rA = raw_rsrc(L.A), rG = raw_rsrc(L.G);
 
   // ---- current-frame patch around the prediction (LDS, clamped reads)
-  int oBr = tz(v.x) - hws - kMB, oBc = tz(v.y) - hws - kMB;
+  int oBr = tz(v.x) - hws - kMB16, oBc = tz(v.y) - hws - kMB16;
   const __amdgpu_buffer_rsrc_t rB = raw_rsrc(L.B);
   wave_sync_lds();
-  load_patch_buf<PW, GL, lk_lds16::QB>(S.B, QB, rB, L.pitchB, L.nr, L.nc, oBr, oBc, lane);
+  load_patch_buf<PW, GL>(S.B, rB, L.pitchB, L.nr, L.nc, oBr, oBc, lane);
 
   // ---- template: bilinear samples of A and of its gradient
   float2 gs[NS];
@@ -705,12 +711,12 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
   auto ensure_covered = [&]() {
     const int vr = tz(v.x), vc = tz(v.y);
     if ((uint32_t)(vr - cov_r) <= (uint32_t)span && (uint32_t)(vc - cov_c) <= (uint32_t)span) return;
-    oBr = vr - hws - kMB;
-    oBc = vc - hws - kMB;
+    oBr = vr - hws - kMB16;
+    oBc = vc - hws - kMB16;
     cov_r = oBr + hws + 1;
     cov_c = oBc + hws + 1;
     wave_sync_lds();
-    load_patch_buf<PW, GL, lk_lds16::QB>(S.B, QB, rB, L.pitchB, L.nr, L.nc, oBr, oBc, lane);
+    load_patch_buf<PW, GL>(S.B, rB, L.pitchB, L.nr, L.nc, oBr, oBc, lane);
     wave_sync_lds();
   };
   const exact_iv ivr = exact_interval(v.x, hws), ivc = exact_interval(v.y, hws);
