@@ -342,7 +342,7 @@ __device__ __forceinline__ lane_win<NPL> make_lane_win(int ws, int lane) {
 template <typename TI, int NPL, bool SEL>
 __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk_params_dev& P,
                                                 const lane_win<NPL>& W, float2 p, float2 pred_tr,
-                                                lk_lds<NPL, SEL>& S) {
+                                                lk_lds<NPL, SEL>& S, bool want_dist) {
   using LD = lk_lds<NPL, SEL>;
   constexpr int PW = LD::PW, PWA = LD::PWA;
   const match_out fail{make_float2(-1.f, -1.f), 1000.f};
@@ -429,7 +429,7 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
         const float2 g00 = g[0], g10 = g[QG], g01 = g[1], g11 = g[QG + 1];
         gs[k] = make_float2(blend(w, g00.x, g10.x, g01.x, g11.x), blend(w, g00.y, g10.y, g01.y, g11.y));
       }
-      if (i < LD::TPL) S.tpl[i] = as[k];
+      if (want_dist && i < LD::TPL) S.tpl[i] = as[k];
       if constexpr (SEL) S.mag[i] = sel[k] ? norml2(gs[k].x, gs[k].y) : -1.f;
     }
   }
@@ -520,6 +520,10 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
     const int vr = tz(v.x), vc = tz(v.y);
     if (!(vr >= rlo && vr <= rhi && vc >= clo && vc <= chi)) return fail;
   }
+  match_out o;
+  o.tr = make_float2(sub_rn(v.x, p.x), sub_rn(v.y, p.y));
+  o.dist = 0.f;
+  if (!want_dist) return o;
 
   // error with the reference's index (r+hws)*hws + (c+hws) and counter
   ensure_covered();
@@ -531,8 +535,6 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
       e = add_rn(e, fabsf(sub_rn(S.tpl[idx], sampleB(k))));
     }
   const float err = group_sum<GL>(e);
-  match_out o;
-  o.tr = make_float2(sub_rn(v.x, p.x), sub_rn(v.y, p.y));
   o.dist = err / (nsel + (float)n);
   return o;
 }
@@ -602,7 +604,8 @@ struct lk_lds16 {
 // inside the patch or image, and their results are discarded: no lane masks
 // on the exact paths.
 __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const lk_params_dev& P,
-                                                  const lane_win<1>& W, float2 p, float2 pred_tr, lk_lds16& S) {
+                                                  const lane_win<1>& W, float2 p, float2 pred_tr, lk_lds16& S,
+                                                  bool want_dist) {
   constexpr int GL = 16, NS = lk_cfg<1>::NS, PW = lk_lds16::PW;
   constexpr int ws = 7, hws = ws / 2, n = ws * ws, QB = lk_lds16::QB;
   const match_out fail{make_float2(-1.f, -1.f), 1000.f};
@@ -672,7 +675,7 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
   }
 #pragma unroll
   for (int k = 0; k < NS; k++)
-    if (W.in[k] && W.idx[k] < lk_lds16::TPL) S.tpl[W.idx[k]] = as[k];
+    if (want_dist && W.in[k] && W.idx[k] < lk_lds16::TPL) S.tpl[W.idx[k]] = as[k];
 
   // window sums with no lane masks: an unselected pixel has a zero gradient,
   // and adding fma(0, finite, acc) leaves acc (never -0 here) unchanged
@@ -770,6 +773,10 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
     const int vr = tz(v.x), vc = tz(v.y);
     if ((uint32_t)(vr - rlo) > (uint32_t)(rhi - rlo) || (uint32_t)(vc - clo) > (uint32_t)(chi - clo)) return fail;
   }
+  match_out o;
+  o.tr = make_float2(sub_rn(v.x, p.x), sub_rn(v.y, p.y));
+  o.dist = 0.f;
+  if (!want_dist) return o;
 
   // error with the reference's index (r+hws)*hws + (c+hws) and counter
   ensure_covered();
@@ -783,8 +790,6 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
       e = add_rn(e, fabsf(sub_rn(S.tpl[idx], smp[k])));
     }
   const float err = group_sum<GL>(e);
-  match_out o;
-  o.tr = make_float2(sub_rn(v.x, p.x), sub_rn(v.y, p.y));
   o.dist = err / (nsel + (float)n);
   return o;
 }
@@ -874,8 +879,10 @@ __device__ inline void lk_body(const lk_args& a, int bid, int nbid, int ppw) {
           continue;
         }
         match_out m;
-        if constexpr (FASTG) m = wave_match16(lv, P, W, p, ptr, S);
-        else m = wave_match<TI, NPL, SEL>(lv, P, W, p, ptr, S);
+        // an ancestor's match gives only its translation (the flow): its
+        // error is not computed
+        if constexpr (FASTG) m = wave_match16(lv, P, W, p, ptr, S, j == s);
+        else m = wave_match<TI, NPL, SEL>(lv, P, W, p, ptr, S, j == s);
         const int mr = tz(add_rn(p.x, m.tr.x)), mc = tz(add_rn(p.y, m.tr.y));
         if (j > s) {
           flow = (mr >= 0 && mr < lv.nr && mc >= 0 && mc < lv.nc) ? m.tr : make_float2(0.f, 0.f);
