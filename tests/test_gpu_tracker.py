@@ -347,3 +347,62 @@ def test_update_input_then_step_equals_run(ctx, orc):
                 assert np.array_equal(po[k], pg[k]), (f, s, k)
             assert np.array_equal(o.sparse(s), g.sparse(s)), (f, s)
     assert sum(len(g.particles(s)) for s in range(3)) > 300
+
+
+def _near_power_of_two(nr, nc, rng):
+    """Particles whose row or column lies within 4.6 px below (or 2.5 above)
+    2^k, in pairs one ulp apart (both parities of the last mantissa bit), with
+    three speeds; the other coordinate uniform inside the frame."""
+    out = []
+    for axis, n, m in ((0, nr, nc), (1, nc, nr)):
+        k = 3
+        while (1 << k) + 12 < n:
+            for d in np.linspace(-2.5, 4.6, 15):
+                x = np.float32((1 << k) - d)
+                for xv in (x, np.nextafter(x, np.float32(0))):
+                    for sp in (0.0, 0.37, -1.45):
+                        o = np.float32(rng.uniform(16, m - 16))
+                        out.append((xv, o, sp) if axis == 0 else (o, xv, sp))
+            k += 1
+    p = np.zeros(len(out), oracle.PARTICLE_DTYPE)
+    for i, (r, c, sp) in enumerate(out):
+        p[i]["pos_r"], p[i]["pos_c"] = r, c
+        p[i]["speed_r"] = p[i]["speed_c"] = sp
+        p[i]["age"] = 2
+    return p
+
+
+def test_matcher_windows_across_powers_of_two(ctx, orc):
+    """Windows whose coordinate + half-width crosses a power of two take the
+    matcher's per-sample path unless the coordinate's last mantissa bit is zero
+    (then every integer offset is exact): particles seeded on both sides of
+    that rule at every scale of a 1080 x 1920 frame decide bit for bit like
+    the oracle."""
+    nr, nc, L = 1080, 1920, 4
+    video = cu.synth_video(0xC0FFEE05, nr, nc, 2, 1)
+    o = orc.tracker(nr, nc, oracle.profile_params("B", L, 20.0, sum_lanes=oracle.gpu_sum_lanes(7)))
+    g = cu.Tracker(ctx, nr, nc, cu.profile_params("B", L, 20.0))
+    o.run(video[0])
+    g.run(video[0])
+    rng = np.random.default_rng(5)
+    pre = [_near_power_of_two(nr >> s, nc >> s, rng) for s in range(L)]
+    crossing = {0: 0, 1: 0}
+    for s in range(L):
+        for f in ("pos_r", "pos_c"):
+            x = pre[s][f]
+            top = 2.0 ** np.floor(np.log2(x.astype(np.float64)) + 1)
+            cross = x.astype(np.float64) + 3 >= top
+            lsb = x.view(np.uint32) & 1
+            crossing[0] += int((cross & (lsb == 0)).sum())
+            crossing[1] += int((cross & (lsb == 1)).sum())
+        o.set_particles(s, pre[s])
+        g.set_particles(s, to_cu(pre[s]))
+    assert min(crossing.values()) > 100, crossing
+    o.run(video[1])
+    g.run(video[1])
+    st = lockstep_new_stats()
+    lockstep_compare(o, g, L, st, pre)
+    print(st)
+    assert st["tracked"] > 1000
+    assert_bit_exact(st, 1)
+    g.close()
