@@ -1,5 +1,6 @@
 // Box pyramid: standalone launch (the device code and its reference notes are
-// in pyramid_dev.hpp).
+// in pyramid_dev.hpp), and the blur-then-decimate level.
+#include "gaussian_dev.hpp"
 #include "pyramid_dev.hpp"
 
 #include <algorithm>
@@ -72,9 +73,10 @@ int pyramid_build(cuimg_hip_ctx* ctx, const cuimg_img* frame, const cuimg_img* l
 // K = 6, clamp-to-edge, row pass then column pass, each sum from tap +K down to
 // tap -K) and the sample at 2p, out domain = in / 2 (floor).
 //
-// Fused: a tile is 16 x 64 output pixels.  Its (2*16 - 1 + 12)-row input
-// block is staged in LDS (columns 2*c0 - 8 .. 2*c0 + 136: whole 16-byte chunks
-// for aligned F32 images), rows are filtered at the even columns only, then
+// Block kernel (launched for U8 input; F32 images take the streamed kernel
+// below, measured 1.7x faster), fused: a tile is 16 x 64 output pixels.  Its
+// (2*16 - 1 + 12)-row input block is staged in LDS (columns 2*c0 - 8 ..
+// 2*c0 + 136), rows are filtered at the even columns only, then
 // the even rows are filtered: no full-resolution intermediate reaches memory.
 // A workgroup walks down a run of kDecRun tiles of one tile column, loading the
 // next tile's block into registers while it filters the current one, so HBM
@@ -98,7 +100,8 @@ struct dec_args {
   float* out;
   int out_pitch, onr, onc;
   int tiles_x, tiles_y, runs_y;
-  int vec_ok;
+  int vec_ok;      // input 16-byte aligned (rows and start)
+  int out_vec_ok;  // output 8-byte aligned
   dec_coefs cf;
 };
 
@@ -240,6 +243,151 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
 #undef CUIMG_DEC_LOAD_CLAMPED
 #undef CUIMG_DEC_PUT
 
+// F32 input: one wave per strip of 256 input columns (128 output columns, two
+// per lane) and kDecSTH output rows.  The wave streams the strip's input rows
+// top to bottom into an LDS ring with global_load_lds, kDecD rows ahead (no
+// registers held while they are in flight), filters every input row at its
+// even columns into a register ring of the last 2K+1 filtered rows, and on
+// every other row (input row 2o + K) sums the ring into output row o.  The
+// arithmetic is the block kernel's: the same taps in the same order.  Strips
+// inside aligned images load each row as one 16-byte access per lane; the last
+// (partial) strip and unaligned images load 4 bytes per lane with clamped
+// columns.  Waits are counted as in gauss_tile_dma: loads complete in order,
+// so at most OPS (kDecD - 1) outstanding operations means the row has landed.
+// Odd tile rows stream bottom-up (UP), so vertically adjacent tiles read the
+// input rows they share at about the same time, from one XCD's L2.
+constexpr int kDecSW = 256;                     // input columns per strip
+constexpr int kDecSTH = 16;                     // output rows per tile
+constexpr int kDecD = 4;                        // input rows in flight
+constexpr int kDecHL = 8;                       // left pad: strip 16-byte aligned in LDS
+constexpr int kDecRW = kDecHL + kDecSW + 8;     // ring row: pad, strip, right halo
+constexpr int kDecR = kDecD + 1;                // ring rows
+
+template <bool WIDE, bool UP>
+__device__ __forceinline__ void dec_tile_dma(const dec_args& a, int tx, int ty, float* ring_mem) {
+  constexpr int K = kDecK, P = 2 * K + 1;
+  constexpr int OPS = WIDE ? 3 : 6;  // DMA instructions per row: strip (1 or 4) and two halos
+  static_assert(OPS * (kDecD - 1) < 64, "vmcnt range");
+  static_assert(kDecHL >= K && kDecRW >= kDecHL + kDecSW + K, "halos fit the ring row");
+  float (*ring)[kDecRW] = reinterpret_cast<float (*)[kDecRW]>(ring_mem);
+  const int lane = threadIdx.x;
+  const int ic0 = tx * kDecSW;            // first input column of the strip
+  const int oc = ic0 / 2 + 2 * lane;      // this lane's output columns oc, oc + 1
+  const int o0 = ty * kDecSTH, o1 = min(o0 + kDecSTH, a.onr);
+  const int first = 2 * o0 - K, last = 2 * (o1 - 1) + K;
+  const int nsteps = last - first + 1;    // step q stages input row row_of(q)
+  const int hcl = max(ic0 - K + lane, 0), hcr = min(ic0 + kDecSW + lane, a.nc - 1);
+
+  float c[P];
+#pragma unroll
+  for (int m = 0; m < P; m++) c[m] = a.cf.c[m];
+
+  auto row_of = [&](int q) { return UP ? last - q : first + q; };
+  auto issue = [&](int q) {
+    float* row_lds = ring[__builtin_amdgcn_readfirstlane(q % kDecR)];
+    const float* row = px<float>(a.in, a.in_pitch, clampi(row_of(q), 0, a.nr - 1), 0);
+    if constexpr (WIDE) {
+      __builtin_amdgcn_global_load_lds(glb_addr(row + ic0 + 4 * lane), lds_addr(row_lds + kDecHL), 16u, 0, 0);
+    } else {
+#pragma unroll
+      for (int m = 0; m < 4; m++)
+        __builtin_amdgcn_global_load_lds(glb_addr(row + min(ic0 + 64 * m + lane, a.nc - 1)),
+                                         lds_addr(row_lds + kDecHL + 64 * m), 4u, 0, 0);
+    }
+    if (lane < K) {
+      __builtin_amdgcn_global_load_lds(glb_addr(row + hcl), lds_addr(row_lds + kDecHL - K), 4u, 0, 0);
+      __builtin_amdgcn_global_load_lds(glb_addr(row + hcr), lds_addr(row_lds + kDecHL + kDecSW), 4u, 0, 0);
+    }
+  };
+  auto store = [&](int o, float v0, float v1) {
+    float* orow = px<float>(a.out, a.out_pitch, o, 0);
+    if (WIDE && a.out_vec_ok) {
+      const v2f_t w = {v0, v1};
+      *reinterpret_cast<v2f_t*>(orow + oc) = w;
+    } else {
+      if (oc < a.onc) orow[oc] = v0;
+      if (oc + 1 < a.onc) orow[oc + 1] = v1;
+    }
+  };
+#pragma unroll
+  for (int d = 0; d < kDecD; d++)
+    if (d < nsteps) issue(d);
+
+  float hist[P][2];
+  float pend0 = 0.f, pend1 = 0.f;  // output row of the previous step, stored after this step's wait
+  int pend_row = -1;
+  for (int qb = 0; qb < nsteps; qb += P) {
+    static_for<P>([&](auto tc) {
+      constexpr int t = decltype(tc)::value;
+      const int q = qb + t;
+      if (q >= nsteps) return;
+      if (q + kDecD - 1 >= nsteps) wait_vm<0>();
+      else wait_vm<OPS * (kDecD - 1)>();
+      if (pend_row >= 0) store(pend_row, pend0, pend1);
+      pend_row = -1;
+      // staged elements 4 lane .. 4 lane + 19; x[m] = staged element HL - K + 4 lane + m,
+      // input column ic0 + 4 lane - K + m
+      float4 ch[5];
+      lds_read16_run<5>(lds_u32(ring[q % kDecR]) + 16u * (uint32_t)lane, ch);
+      const float* x = reinterpret_cast<const float*>(ch) + (kDecHL - K);
+      // the slot of step q + kDecD held step q - 1, read by the previous step
+      if (q + kDecD < nsteps) issue(q + kDecD);
+      // row pass at input columns ic0 + 4 lane and ic0 + 4 lane + 2
+      float s0 = c[2 * K] * x[2 * K], s1 = c[2 * K] * x[2 * K + 2];
+#pragma unroll
+      for (int m = 2 * K - 1; m >= 0; m--) {
+        s0 = fmaf(c[m], x[m], s0);
+        s1 = fmaf(c[m], x[m + 2], s1);
+      }
+      hist[t][0] = s0;
+      hist[t][1] = s1;
+      // every other step from 2K on completes an output row o, whose tap m
+      // (weight c[m]) is input row 2 o - K + m, added from m = 2K down to 0.
+      // Going down, step q stages row 2 o + K and tap m was filtered at step
+      // q - 2K + m (slot (t + m + 1) mod P); going up, step q stages row
+      // 2 o - K and tap m was filtered at step q - m (slot (t + P - m) mod P).
+      if (q >= 2 * K && ((q - 2 * K) & 1) == 0) {
+        float v0, v1;
+        if constexpr (!UP) {
+          v0 = c[2 * K] * hist[t][0];
+          v1 = c[2 * K] * hist[t][1];
+#pragma unroll
+          for (int m = 2 * K - 1; m >= 0; m--) {
+            v0 = fmaf(c[m], hist[(t + m + 1) % P][0], v0);
+            v1 = fmaf(c[m], hist[(t + m + 1) % P][1], v1);
+          }
+        } else {
+          v0 = c[2 * K] * hist[(t + 1) % P][0];
+          v1 = c[2 * K] * hist[(t + 1) % P][1];
+#pragma unroll
+          for (int m = 2 * K - 1; m >= 0; m--) {
+            v0 = fmaf(c[m], hist[(t + P - m) % P][0], v0);
+            v1 = fmaf(c[m], hist[(t + P - m) % P][1], v1);
+          }
+        }
+        pend0 = v0;
+        pend1 = v1;
+        pend_row = UP ? o1 - 1 - (q - 2 * K) / 2 : o0 + (q - 2 * K) / 2;
+      }
+    });
+  }
+  if (pend_row >= 0) store(pend_row, pend0, pend1);
+}
+
+__global__ __launch_bounds__(64) void gauss_decimate_dma_kernel(dec_args a) {
+  __shared__ __align__(16) float ring[kDecR * kDecRW];
+  const int b = xcd_order(blockIdx.x, gridDim.x);
+  const int tx = b % a.tiles_x, ty = b / a.tiles_x;
+  const bool wide = a.vec_ok && (tx + 1) * kDecSW <= a.nc;
+  if (ty & 1) {
+    if (wide) dec_tile_dma<true, true>(a, tx, ty, ring);
+    else dec_tile_dma<false, true>(a, tx, ty, ring);
+  } else {
+    if (wide) dec_tile_dma<true, false>(a, tx, ty, ring);
+    else dec_tile_dma<false, false>(a, tx, ty, ring);
+  }
+}
+
 static int gauss_decimate(cuimg_hip_ctx* ctx, const cuimg_img* in, const cuimg_img* out) {
   if ((in->dtype != CUIMG_U8 && in->dtype != CUIMG_F32) || out->dtype != CUIMG_F32)
     return set_error(ctx, CUIMG_E_UNSUPPORTED, "pyr_down: Gaussian decimation reads U8 or F32 and writes F32");
@@ -259,16 +407,22 @@ static int gauss_decimate(cuimg_hip_ctx* ctx, const cuimg_img* in, const cuimg_i
   if (a.onr == 0 || a.onc == 0) return 0;
   if ((uint64_t)in->nrows * (uint64_t)in->pitch >= (1ull << 32))
     return set_error(ctx, CUIMG_E_UNSUPPORTED, "pyr_down: Gaussian decimation needs an input below 4 GiB");
+  a.vec_ok = aligned16(in->begin, in->pitch);
+  a.out_vec_ok = ((uintptr_t)out->begin % 8 == 0) && (out->pitch % 8 == 0);
+  if (in->dtype == CUIMG_F32) {
+    // streamed strips of 128 output columns
+    a.tiles_x = ceil_div(a.onc, kDecSW / 2);
+    a.tiles_y = ceil_div(a.onr, kDecSTH);
+    const int blocks = a.tiles_x * a.tiles_y;
+    return launch(ctx, "pyr_gauss_decimate",
+                  [&] { gauss_decimate_dma_kernel<<<blocks, 64, 0, ctx->stream>>>(a); });
+  }
   a.tiles_x = ceil_div(a.onc, kDecTW);
   a.tiles_y = ceil_div(a.onr, kDecTH);
   a.runs_y = ceil_div(a.tiles_y, kDecRun);
-  a.vec_ok = aligned16(in->begin, in->pitch);
   const int blocks = a.tiles_x * a.runs_y;
-  return launch(ctx, "pyr_gauss_decimate", [&] {
-    if (in->dtype == CUIMG_U8) gauss_decimate_kernel<uint8_t, false><<<blocks, 256, 0, ctx->stream>>>(a);
-    else if (a.vec_ok) gauss_decimate_kernel<float, true><<<blocks, 256, 0, ctx->stream>>>(a);
-    else gauss_decimate_kernel<float, false><<<blocks, 256, 0, ctx->stream>>>(a);
-  });
+  return launch(ctx, "pyr_gauss_decimate",
+                [&] { gauss_decimate_kernel<uint8_t, false><<<blocks, 256, 0, ctx->stream>>>(a); });
 }
 
 }  // namespace cuimg

@@ -220,8 +220,11 @@ def test_pyr_down_leaves_borders(ctx, orc):
 
 
 @pytest.mark.parametrize("shape", [(2, 2), (3, 5), (17, 13), (50, 70), (131, 257), (1080, 1920), (4096, 4096)])
-@pytest.mark.parametrize("kind", ["u8", "f32"])
+@pytest.mark.parametrize("kind", ["u8", "f32", "f32_unaligned"])
 def test_pyr_down_gauss_decimate(ctx, orc, shape, kind):
+    """F32 images stream strips of 256 input columns (16-byte row loads when
+    aligned, clamped 4-byte loads for the last partial strip and unaligned
+    images); U8 images take the block kernel."""
     print(f"gauss decimate {kind} {shape}", flush=True)
     rng = np.random.default_rng(shape[0] * 31 + shape[1])
     if kind == "u8":
@@ -229,10 +232,15 @@ def test_pyr_down_gauss_decimate(ctx, orc, shape, kind):
         src = ctx.upload(img, border=3)
     else:
         img = rng.uniform(0, 255, shape).astype(np.float32)
-        src = ctx.upload(img)
+        # a one-pixel border puts the first pixel 4 bytes past a 16-byte boundary
+        src = ctx.upload(img, border=1 if kind == "f32_unaligned" else 0)
     dst = ctx.image(shape[0] // 2, shape[1] // 2, cu.F32)
     ctx.pyr_down(src, dst, cu.PYR_GAUSS_DECIMATE)
-    assert rel_err(dst.download(), orc.pyr_down_gauss(img)) < 1e-5
+    got = dst.download()
+    assert rel_err(got, orc.pyr_down_gauss(img)) < 1e-5
+    # the taps in the oracle's fused order: bit-exact
+    g = orc.gaussian(img, orc.coefs(1, 6), 6, fused=True)
+    assert np.array_equal(got, g[0:2 * (shape[0] // 2):2, 0:2 * (shape[1] // 2):2])
 
 
 def test_pyr_down_mode_errors(ctx):
