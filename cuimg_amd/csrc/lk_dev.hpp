@@ -561,7 +561,8 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
 __device__ __forceinline__ bool exact_axis(float x, int h) {
   const float lo = sub_rn(x, (float)h), hi = add_rn(x, (float)h);
   const float top = __uint_as_float(((__float_as_uint(x) >> 23) + 1u) << 23);
-  return lo >= 0.f && (hi < top || (__float_as_uint(x) & 1u) == 0u);
+  // bitwise, not short-circuit: no exec-mask branches for a few compares
+  return (lo >= 0.f) & ((hi < top) | ((__float_as_uint(x) & 1u) == 0u));
 }
 
 // A sufficient test for exact_axis around a start coordinate x0: every x in
@@ -572,7 +573,7 @@ __device__ __forceinline__ bool exact_axis(float x, int h) {
 struct exact_iv {
   float lo, hi, h;
   __device__ __forceinline__ bool has(float x) const {
-    return (x >= lo && x < hi) || ((__float_as_uint(x) & 1u) == 0u && x >= h);
+    return ((x >= lo) & (x < hi)) | (((__float_as_uint(x) & 1u) == 0u) & (x >= h));
   }
 };
 __device__ __forceinline__ exact_iv exact_interval(float x0, int h) {
@@ -628,8 +629,8 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
   const int pr = tz(p.x), pc = tz(p.y);
   // one row and column of margin beyond the window's corners: the grid's row 7
   // and column 7 lanes read there
-  const bool exact_t = exact_axis(p.x, hws) && exact_axis(p.y, hws) && pr + hws + 2 <= L.nr - 1 &&
-                       pc + hws + 2 <= L.nc - 1;
+  const bool exact_t = exact_axis(p.x, hws) & exact_axis(p.y, hws) & (pr + hws + 2 <= L.nr - 1) &
+                       (pc + hws + 2 <= L.nc - 1);
   if (exact_t) {
     const bw w = weights(p.x, p.y);
     const uint32_t r0 = (uint32_t)(pr - hws + hr);
@@ -725,7 +726,7 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
   const exact_iv ivr = exact_interval(v.x, hws), ivc = exact_interval(v.y, hws);
   // current-frame samples at v + (r, c) for the lane's pixels
   auto sample_window = [&](float (&out)[NS]) {
-    if (ivr.has(v.x) && ivc.has(v.y)) {
+    if (ivr.has(v.x) & ivc.has(v.y)) {
       const bw w = weights(v.x, v.y);
       const float* b0 = &S.B[(tz(v.x) - hws + hr - oBr) * PW + (tz(v.y) - hws + hc - oBc)];
 #pragma unroll
