@@ -499,7 +499,7 @@ __device__ __forceinline__ match_out wave_match(const lk_level_view& L, const lk
   float nk0 = 1.f, nk1 = 1.f;
   const int rlo = 3, rhi = L.nr - 4, clo = 3, chi = L.nc - 4;
   // sqrtf(|delta|^2) >= eps and |pred - v| > 4 ws as the equivalent squared tests
-  for (int it = 0; it < P.max_iters && add_rn(mul_rn(nk0, nk0), mul_rn(nk1, nk1)) >= P.step_sq; it++) {
+  for (int it = 0; (it < P.max_iters) & (add_rn(mul_rn(nk0, nk0), mul_rn(nk1, nk1)) >= P.step_sq); it++) {
     ensure_covered();
     float b0 = 0.f, b1 = 0.f;
 #pragma unroll
@@ -714,7 +714,7 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
   int cov_r = oBr + hws + 1, cov_c = oBc + hws + 1;
   auto ensure_covered = [&]() {
     const int vr = tz(v.x), vc = tz(v.y);
-    if ((uint32_t)(vr - cov_r) <= (uint32_t)span && (uint32_t)(vc - cov_c) <= (uint32_t)span) return;
+    if (((uint32_t)(vr - cov_r) <= (uint32_t)span) & ((uint32_t)(vc - cov_c) <= (uint32_t)span)) return;
     oBr = vr - hws - kMB16;
     oBc = vc - hws - kMB16;
     cov_r = oBr + hws + 1;
@@ -751,7 +751,7 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
   float nk0 = 1.f, nk1 = 1.f;
   const int rlo = 3, rhi = L.nr - 4, clo = 3, chi = L.nc - 4;
   // sqrtf(|delta|^2) >= eps and |pred - v| > 4 ws as the equivalent squared tests
-  for (int it = 0; it < P.max_iters && add_rn(mul_rn(nk0, nk0), mul_rn(nk1, nk1)) >= P.step_sq; it++) {
+  for (int it = 0; (it < P.max_iters) & (add_rn(mul_rn(nk0, nk0), mul_rn(nk1, nk1)) >= P.step_sq); it++) {
     ensure_covered();
     float smp[NS];
     sample_window(smp);
@@ -772,7 +772,7 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
     const float dr = sub_rn(prediction.x, v.x), dc = sub_rn(prediction.y, v.y);
     if (add_rn(mul_rn(dr, dr), mul_rn(dc, dc)) > P.far_sq) return fail;
     const int vr = tz(v.x), vc = tz(v.y);
-    if ((uint32_t)(vr - rlo) > (uint32_t)(rhi - rlo) || (uint32_t)(vc - clo) > (uint32_t)(chi - clo)) return fail;
+    if (((uint32_t)(vr - rlo) > (uint32_t)(rhi - rlo)) | ((uint32_t)(vc - clo) > (uint32_t)(chi - clo))) return fail;
   }
   match_out o;
   o.tr = make_float2(sub_rn(v.x, p.x), sub_rn(v.y, p.y));
@@ -856,7 +856,7 @@ __device__ inline void lk_body(const lk_args& a, int bid, int nbid, int ppw) {
     float2 tr = make_float2(0.f, 0.f);
     float dist = 0.f;
     const float2 pos = make_float2(part.pos_r, part.pos_c);
-    if (part.age > 0 && tz(pos.x) >= pd_r1 && tz(pos.x) <= pd_r2 && tz(pos.y) >= pd_c1 && tz(pos.y) <= pd_c2) {
+    if ((part.age > 0) & (tz(pos.x) >= pd_r1) & (tz(pos.x) <= pd_r2) & (tz(pos.y) >= pd_c1) & (tz(pos.y) <= pd_c2)) {
       float2 flow = make_float2(0.f, 0.f);
       for (int j = s + 1 < nlev ? nlev - 1 : s; j >= s; j--) {
         const lk_level_view lv = uni(lvs[j]);
@@ -867,13 +867,13 @@ __device__ inline void lk_body(const lk_args& a, int bid, int nbid, int ppw) {
           p = make_float2(ldexpf(pos.x, s - j), ldexpf(pos.y, s - j));
           ptr = j == nlev - 1 ? make_float2(0.f, 0.f) : make_float2(2.f * flow.x, 2.f * flow.y);
           const int pr = tz(add_rn(p.x, ptr.x)), pc = tz(add_rn(p.y, ptr.y));
-          run = pr >= 3 && pr <= lv.nr - 4 && pc >= 3 && pc <= lv.nc - 4;
+          run = (pr >= 3) & (pr <= lv.nr - 4) & (pc >= 3) & (pc <= lv.nc - 4);
         } else {
           p = pos;
           ptr = s + 1 < nlev ? make_float2(2.f * flow.x, 2.f * flow.y)
                              : (part.age == 1 ? make_float2(0.f, 0.f) : make_float2(part.speed_r, part.speed_c));
           const int pr = tz(add_rn(p.x, ptr.x)), pc = tz(add_rn(p.y, ptr.y));
-          run = pr >= pd_r1 && pr <= pd_r2 && pc >= pd_c1 && pc <= pd_c2;
+          run = (pr >= pd_r1) & (pr <= pd_r2) & (pc >= pd_c1) & (pc <= pd_c2);
         }
         if (!run) {
           flow = make_float2(0.f, 0.f);
@@ -886,7 +886,7 @@ __device__ inline void lk_body(const lk_args& a, int bid, int nbid, int ppw) {
         else m = wave_match<TI, NPL, SEL>(lv, P, W, p, ptr, S, j == s);
         const int mr = tz(add_rn(p.x, m.tr.x)), mc = tz(add_rn(p.y, m.tr.y));
         if (j > s) {
-          flow = (mr >= 0 && mr < lv.nr && mc >= 0 && mc < lv.nc) ? m.tr : make_float2(0.f, 0.f);
+          flow = ((mr >= 0) & (mr < lv.nr) & (mc >= 0) & (mc < lv.nc)) ? m.tr : make_float2(0.f, 0.f);
         } else {
           const float2 pred = make_float2(add_rn(p.x, ptr.x), add_rn(p.y, ptr.y));
           tr = m.tr;
