@@ -770,9 +770,11 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
     v.x = add_rn(v.x, nk0);
     v.y = add_rn(v.y, nk1);
     const float dr = sub_rn(prediction.x, v.x), dc = sub_rn(prediction.y, v.y);
-    if (add_rn(mul_rn(dr, dr), mul_rn(dc, dc)) > P.far_sq) return fail;
+    // too far from the prediction, or outside domain - border: one exit
     const int vr = tz(v.x), vc = tz(v.y);
-    if (((uint32_t)(vr - rlo) > (uint32_t)(rhi - rlo)) | ((uint32_t)(vc - clo) > (uint32_t)(chi - clo))) return fail;
+    if ((add_rn(mul_rn(dr, dr), mul_rn(dc, dc)) > P.far_sq) | ((uint32_t)(vr - rlo) > (uint32_t)(rhi - rlo)) |
+        ((uint32_t)(vc - clo) > (uint32_t)(chi - clo)))
+      return fail;
   }
   match_out o;
   o.tr = make_float2(sub_rn(v.x, p.x), sub_rn(v.y, p.y));
