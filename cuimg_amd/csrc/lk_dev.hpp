@@ -612,6 +612,11 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
   const match_out fail{make_float2(-1.f, -1.f), 1000.f};
   const int lane = threadIdx.x & (GL - 1);
   const int hr = lane >> 3, hc = lane & 7;  // this lane's grid row parity and column
+  // the lane's window geometry, recomputed where used rather than held in
+  // registers: sample k is window pixel (hr + 2k - hws, hc - hws)
+  auto in_w = [&](int k) { return (hc < ws) & (hr + 2 * k < ws); };
+  auto fr_w = [&](int k) { return (float)(hr + 2 * k - hws); };
+  const float fc_w = (float)(hc - hws);
   const float2 prediction = make_float2(add_rn(p.x, pred_tr.x), add_rn(p.y, pred_tr.y));
   float2 v = prediction;
   const __amdgpu_buffer_rsrc_t rA = raw_rsrc(L.A), rG = raw_rsrc(L.G);
@@ -638,7 +643,7 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
     const uint32_t og0 = __umul24(r0, (uint32_t)L.pitchG) + (uint32_t)(pc - hws + hc) * 8u;
 #pragma unroll
     for (int k = 0; k < NS; k++) {
-      sel[k] = W.in[k];
+      sel[k] = in_w(k);
       // rows r0 + 2k and r0 + 2k + 1 (the latter through the uniform pitch offset)
       const uint32_t oa = oa0 + (uint32_t)(2 * k) * (uint32_t)L.pitchA;
       const uint32_t og = og0 + (uint32_t)(2 * k) * (uint32_t)L.pitchG;
@@ -646,8 +651,8 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
                   a11 = ld_f32(rA, oa + 4, L.pitchA);
       const float2 g00 = ld_f32x2(rG, og, 0), g01 = ld_f32x2(rG, og + 8, 0), g10 = ld_f32x2(rG, og, L.pitchG),
                    g11 = ld_f32x2(rG, og + 8, L.pitchG);
-      as[k] = W.in[k] ? blend(w, a00, a10, a01, a11) : 0.f;
-      gs[k] = W.in[k] ? make_float2(blend(w, g00.x, g10.x, g01.x, g11.x), blend(w, g00.y, g10.y, g01.y, g11.y))
+      as[k] = in_w(k) ? blend(w, a00, a10, a01, a11) : 0.f;
+      gs[k] = in_w(k) ? make_float2(blend(w, g00.x, g10.x, g01.x, g11.x), blend(w, g00.y, g10.y, g01.y, g11.y))
                       : make_float2(0.f, 0.f);
     }
   } else {
@@ -656,8 +661,8 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
       gs[k] = make_float2(0.f, 0.f);
       as[k] = 0.f;
       sel[k] = false;
-      if (W.in[k]) {
-        const bw w = weights(add_rn(p.x, W.fr[k]), add_rn(p.y, W.fc[k]));
+      if (in_w(k)) {
+        const bw w = weights(add_rn(p.x, fr_w(k)), add_rn(p.y, fc_w));
         if (w.xr >= 0 && w.xr < L.nr && w.xc >= 0 && w.xc < L.nc) {
           sel[k] = true;
           const uint32_t r0 = (uint32_t)w.xr, r1 = (uint32_t)min(w.xr + 1, L.nr - 1);
@@ -676,7 +681,7 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
   }
 #pragma unroll
   for (int k = 0; k < NS; k++)
-    if (want_dist && W.in[k] && W.idx[k] < lk_lds16::TPL) S.tpl[W.idx[k]] = as[k];
+    if (want_dist && in_w(k) && (hr + 2 * k) * ws + hc < lk_lds16::TPL) S.tpl[(hr + 2 * k) * ws + hc] = as[k];
 
   // window sums with no lane masks: an unselected pixel has a zero gradient,
   // and adding fma(0, finite, acc) leaves acc (never -0 here) unchanged
@@ -738,8 +743,8 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
 #pragma unroll
       for (int k = 0; k < NS; k++) {
         out[k] = 0.f;
-        if (W.in[k]) {
-          const bw w = weights(add_rn(v.x, W.fr[k]), add_rn(v.y, W.fc[k]));
+        if (in_w(k)) {
+          const bw w = weights(add_rn(v.x, fr_w(k)), add_rn(v.y, fc_w));
           const float* b = &S.B[(w.xr - oBr) * PW + (w.xc - oBc)];
           out[k] = blend(w, b[0], b[PW], b[1], b[PW + 1]);
         }
@@ -788,8 +793,8 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
   float e = 0.f;
 #pragma unroll
   for (int k = 0; k < NS; k++)
-    if (W.in[k]) {
-      const int idx = (W.r[k] + hws) * hws + (W.c[k] + hws);
+    if (in_w(k)) {
+      const int idx = (hr + 2 * k) * hws + hc;
       e = add_rn(e, fabsf(sub_rn(S.tpl[idx], smp[k])));
     }
   const float err = group_sum<GL>(e);
