@@ -604,8 +604,8 @@ struct lk_lds16 {
 // outside the window (column 7, row 7) compute samples too, from addresses
 // inside the patch or image, and their results are discarded: no lane masks
 // on the exact paths.
-__device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const lk_params_dev& P,
-                                                  const lane_win<1>& W, float2 p, float2 pred_tr, lk_lds16& S,
+__device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const lk_params_dev& P, float2 p,
+                                                  float2 pred_tr, lk_lds16& S,
                                                   bool want_dist) {
   constexpr int GL = 16, NS = lk_cfg<1>::NS, PW = lk_lds16::PW;
   constexpr int ws = 7, hws = ws / 2, n = ws * ws, QB = lk_lds16::QB;
@@ -889,7 +889,7 @@ __device__ inline void lk_body(const lk_args& a, int bid, int nbid, int ppw) {
         match_out m;
         // an ancestor's match gives only its translation (the flow): its
         // error is not computed
-        if constexpr (FASTG) m = wave_match16(lv, P, W, p, ptr, S, j == s);
+        if constexpr (FASTG) m = wave_match16(lv, P, p, ptr, S, j == s);
         else m = wave_match<TI, NPL, SEL>(lv, P, W, p, ptr, S, j == s);
         const int mr = tz(add_rn(p.x, m.tr.x)), mc = tz(add_rn(p.y, m.tr.y));
         if (j > s) {
