@@ -802,6 +802,22 @@ __device__ __forceinline__ match_out wave_match16(const lk_level_view& L, const 
   return o;
 }
 
+// Level view j read with scalar loads: the table is written before the launch
+// and never during it, so it is read as constant memory (SGPRs, no LDS round
+// trip and no per-field readfirstlane).
+__device__ __forceinline__ lk_level_view level_view_scalar(const lk_level_view* lv, int j) {
+  constexpr int NW = (int)(sizeof(lk_level_view) / 4);
+  static_assert(sizeof(lk_level_view) % 4 == 0, "whole words");
+  const __attribute__((address_space(4))) uint32_t* src =
+      (const __attribute__((address_space(4))) uint32_t*)(lv + j);
+  uint32_t w[NW];
+#pragma unroll
+  for (int k = 0; k < NW; k++) w[k] = src[k];
+  lk_level_view out;
+  __builtin_memcpy(&out, w, sizeof out);
+  return out;
+}
+
 // One particle per group of lanes.  The particle at scale s and its synthetic age-1
 // ancestors at p/2, p/4, ... are solved in one loop from the coarsest scale
 // down (recursive_prediction / get_flow_at): an ancestor whose prediction
@@ -821,18 +837,19 @@ __device__ inline void lk_body(const lk_args& a, int bid, int nbid, int ppw) {
   constexpr int GL = lk_cfg<NPL>::GL, GPW = lk_cfg<NPL>::GPW;
   using LDS = std::conditional_t<FASTG, lk_lds16, lk_lds<NPL, SEL>>;
   __shared__ LDS lds_all[kLkWaves * GPW];
-  __shared__ lk_level_view lvs[kMaxLevels];
   __shared__ uint32_t blk_live, blk_acc;
   const int wave = threadIdx.x >> 6;
   const int grp = (threadIdx.x & 63) / GL;
   const int lane = threadIdx.x & (GL - 1);
   LDS& S = lds_all[wave * GPW + grp];
   // Every argument is read once, before this kernel stores anything, so the
-  // reads are scalar and the values stay in registers (or LDS for the views)
-  // whether `a` is a kernel argument or an entry of a job table.
+  // reads are scalar and the values stay in registers whether `a` is a kernel
+  // argument or an entry of a job table; the level views are read per level
+  // with scalar loads from their table.
   const lk_params_dev P = uni(a.P);
   const int s = uni(a.s), nlev = uni(a.nlevels);
   const int pd_r1 = uni(a.pd_r1), pd_r2 = uni(a.pd_r2), pd_c1 = uni(a.pd_c1), pd_c2 = uni(a.pd_c2);
+  const lk_level_view* const lvp = a.lv;
   cuimg_particle_f* const parts = uni(a.parts);
   level_counters* const cnt = uni(a.cnt);
   uint32_t* const sparse = uni(a.sparse);
@@ -845,8 +862,6 @@ __device__ inline void lk_body(const lk_args& a, int bid, int nbid, int ppw) {
   const uint32_t nchunks = (n + per_wg - 1) / per_wg;
   if (bid == 0 && threadIdx.x == 0) cnt->n_matched = n;
   if ((uint32_t)bid >= nchunks) return;
-  if (threadIdx.x < kMaxLevels * (int)(sizeof(lk_level_view) / 4))
-    reinterpret_cast<int*>(lvs)[threadIdx.x] = reinterpret_cast<const int*>(a.lv)[threadIdx.x];
   if (threadIdx.x == 0) {
     blk_live = 0;
     blk_acc = 0;
@@ -866,7 +881,7 @@ __device__ inline void lk_body(const lk_args& a, int bid, int nbid, int ppw) {
     if ((part.age > 0) & (tz(pos.x) >= pd_r1) & (tz(pos.x) <= pd_r2) & (tz(pos.y) >= pd_c1) & (tz(pos.y) <= pd_c2)) {
       float2 flow = make_float2(0.f, 0.f);
       for (int j = s + 1 < nlev ? nlev - 1 : s; j >= s; j--) {
-        const lk_level_view lv = uni(lvs[j]);
+        const lk_level_view lv = level_view_scalar(lvp, j);
         float2 p, ptr;
         bool run;
         if (j > s) {
